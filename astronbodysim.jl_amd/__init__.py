@@ -1,0 +1,24 @@
+"""MI355X-native direct-summation gravity engine: the (DirectSum, GPU) path of
+AstroNbodySim.jl on hand-written HIP kernels for gfx950, behind a C-ABI
+(include/nbody_mi355x.h, libnbody_mi355x.so) and this host-side mirror of the
+reference's Simulation / run API."""
+from . import _native
+from ._native import NBError, device_info, version
+from .simulation import (
+    AllPairs, BLACKHOLE, BULGE, CPU, DISK, DirectSum, Euler, GAS, GPU, HALO, Leapfrog, STAR,
+    Simulation, StarData, Tiled, advance_and_find_timestep, compute_force, compute_potential,
+    find_next_sync_point_and_drift, get_all_data, get_local_data, init_timesteps,
+    leapfrog_half_kick, run, set_softlen, set_softlen_, softlen, step, total_angular_momentum,
+    total_energy, total_kinetic, total_momentum, total_potential,
+)
+from . import ics
+
+__all__ = [
+    "NBError", "device_info", "version", "AllPairs", "Tiled", "CPU", "GPU", "DirectSum",
+    "Leapfrog", "Euler", "GAS", "HALO", "DISK", "BULGE", "STAR", "BLACKHOLE", "Simulation",
+    "StarData", "run", "step", "compute_force", "compute_potential", "total_kinetic",
+    "total_potential", "total_energy", "total_momentum", "total_angular_momentum",
+    "get_all_data", "get_local_data", "softlen", "set_softlen", "set_softlen_",
+    "init_timesteps", "leapfrog_half_kick", "find_next_sync_point_and_drift",
+    "advance_and_find_timestep", "ics",
+]

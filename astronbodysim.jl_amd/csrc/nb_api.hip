@@ -1,0 +1,844 @@
+// nb_api.hip -- the C-ABI (include/nbody_mi355x.h) for one-system contexts.
+//
+// Device state is SoA in an internal particle order (perm maps internal ->
+// caller index; nullptr = identity).  Positions/velocities/masses are FP64
+// except in NB_MODE_FP32, where they are FP32 as in the reference's
+// floattype = Float32 GPU simulations.  Accelerations and potentials are FP64.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "nb_integrate.h"
+#include "nb_internal.h"
+#include "nb_ctx.h"
+
+#define NB_VERSION_MAJOR 1
+#define NB_VERSION_MINOR 0
+#define NB_VERSION_PATCH 0
+
+namespace nb {
+
+static thread_local std::string g_err;
+
+void set_error(const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+void clear_error() { g_err.clear(); }
+const char *last_error() { return g_err.c_str(); }
+
+}  // namespace nb
+
+using namespace nb;
+
+namespace {
+
+int default_tpt(int64_t n) { return n >= (1 << 17) ? 2 : 1; }
+
+int default_segments(int64_t n, int tpt) {
+    const int64_t nblk = (n + (int64_t)BS * tpt - 1) / ((int64_t)BS * tpt);
+    const int64_t ntiles = (n + BS - 1) / BS;
+    int64_t nseg = (2048 + nblk - 1) / std::max<int64_t>(nblk, 1);
+    nseg = std::min<int64_t>(std::max<int64_t>(nseg, 1), std::max<int64_t>(ntiles, 1));
+    nseg = std::min<int64_t>(nseg, 64);
+    return (int)nseg;
+}
+
+int check_ctx(nb_ctx *c, bool need_upload = true) {
+    clear_error();
+    if (!c) {
+        set_error("null context");
+        return NB_ERR_INVALID;
+    }
+    hipError_t e = hipSetDevice(c->dev);
+    if (e != hipSuccess) {
+        set_error("hipSetDevice(%d) failed: %s", c->dev, hipGetErrorString(e));
+        return NB_ERR_HIP;
+    }
+    if (need_upload && !c->uploaded) {
+        set_error("no particle data uploaded (call nb_upload first)");
+        return NB_ERR_STATE;
+    }
+    return NB_OK;
+}
+
+void drop_graph(nb_ctx *c) {
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->g_steps = 0;
+}
+
+int ensure_part(nb_ctx *c, int nseg) {
+    if (nseg <= 1 || nseg <= c->part_cap) return NB_OK;
+    drop_graph(c);
+    dfree(c->part);
+    c->part_cap = 0;
+    NB_TRY(dalloc(&c->part, sizeof(double) * 4 * (size_t)c->n * (size_t)nseg));
+    c->part_cap = nseg;
+    return NB_OK;
+}
+
+struct Geometry {
+    int tpt, nseg, nblk_i;
+    int64_t seg_len;
+};
+
+Geometry geometry(int64_t nt, int64_t ns, int tpt, int want_seg) {
+    Geometry g;
+    g.tpt = tpt;
+    g.nblk_i = (int)((nt + (int64_t)BS * tpt - 1) / ((int64_t)BS * tpt));
+    const int64_t ntiles = std::max<int64_t>((ns + BS - 1) / BS, 1);
+    int64_t nseg = std::min<int64_t>(std::max(want_seg, 1), ntiles);
+    const int64_t tiles_per_seg = (ntiles + nseg - 1) / nseg;
+    g.seg_len = tiles_per_seg * BS;
+    g.nseg = (int)((ntiles + tiles_per_seg - 1) / tiles_per_seg);
+    return g;
+}
+
+int record_start(nb_ctx *c, hipEvent_t *e0) {
+    *e0 = nullptr;
+    if (!c->timing) return NB_OK;
+    NB_HIP(hipEventCreate(e0));
+    NB_HIP(hipEventRecord(*e0, c->stream));
+    return NB_OK;
+}
+
+int record_stop(nb_ctx *c, hipEvent_t e0) {
+    if (!c->timing || !e0) return NB_OK;
+    hipEvent_t e1;
+    NB_HIP(hipEventCreate(&e1));
+    NB_HIP(hipEventRecord(e1, c->stream));
+    c->pending.emplace_back(e0, e1);
+    return NB_OK;
+}
+
+int collect_timing(nb_ctx *c) {
+    for (auto &p : c->pending) {
+        NB_HIP(hipEventSynchronize(p.second));
+        float ms = 0.f;
+        NB_HIP(hipEventElapsedTime(&ms, p.first, p.second));
+        c->tms += ms;
+        c->tcount += 1;
+        (void)hipEventDestroy(p.first);
+        (void)hipEventDestroy(p.second);
+    }
+    c->pending.clear();
+    return NB_OK;
+}
+
+// Refresh the compacted FP64 copies of the central sources.
+__global__ void k_gather_central(const double *__restrict__ x, const double *__restrict__ y,
+                                 const double *__restrict__ z, const double *__restrict__ m,
+                                 const int64_t *__restrict__ idx, int64_t k, double *cx,
+                                 double *cy, double *cz, double *cm) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= k) return;
+    const int64_t i = idx[t];
+    cx[t] = x[i];
+    cy[t] = y[i];
+    cz[t] = z[i];
+    cm[t] = m[i];
+}
+
+// The all-pairs evaluation for every particle (acc, and potential when asked).
+int run_force(nb_ctx *c, bool with_pot, bool allow_timing) {
+    hipStream_t s = c->stream;
+    const int64_t n = c->n;
+    if (n == 0) return NB_OK;
+    if (c->sort.interval > 0 && c->force_calls % c->sort.interval == 0 && c->mode != NB_MODE_FP64) {
+        NB_TRY(morton_reorder(c));
+    }
+    c->force_calls++;
+    NB_TRY(launch_tile_boxes(c->x, c->y, c->z, c->is_double, n, c->tile_box, s));
+    const bool f64 = c->mode == NB_MODE_FP64;
+    const int tpt = f64 ? 1 : c->tun.targets_per_thread;
+    Geometry g = geometry(n, n, tpt, c->tun.j_segments);
+    NB_TRY(ensure_part(c, g.nseg));
+    hipEvent_t e0 = nullptr;
+    if (allow_timing) NB_TRY(record_start(c, &e0));
+    if (c->is_double) {
+        ForceArgs<double> a{};
+        a.tx = (const double *)c->x; a.ty = (const double *)c->y; a.tz = (const double *)c->z;
+        a.th = c->h; a.th_const = 0.0; a.nt = n;
+        a.sx = (const double *)c->x; a.sy = (const double *)c->y; a.sz = (const double *)c->z;
+        a.sm = (const double *)c->m; a.ns = n;
+        a.tile_box = c->tile_box;
+        a.use_origin = 1;
+        a.nseg = g.nseg; a.seg_len = g.seg_len; a.nblk_i = g.nblk_i;
+        a.G = c->G;
+        a.ax = c->ax; a.ay = c->ay; a.az = c->az; a.pot = c->pot;
+        a.part = c->part;
+        a.skip_src = (c->mode == NB_MODE_SPLIT_CENTRAL && c->ncentral > 0) ? c->central : nullptr;
+        a.accumulate = 0;
+        if (f64) NB_TRY(launch_force_f64(a, with_pot, s));
+        else NB_TRY(launch_force_split(a, tpt, with_pot, s));
+    } else {
+        ForceArgs<float> a{};
+        a.tx = (const float *)c->x; a.ty = (const float *)c->y; a.tz = (const float *)c->z;
+        a.th = c->h; a.th_const = 0.0; a.nt = n;
+        a.sx = (const float *)c->x; a.sy = (const float *)c->y; a.sz = (const float *)c->z;
+        a.sm = (const float *)c->m; a.ns = n;
+        a.tile_box = c->tile_box;
+        a.use_origin = 0;
+        a.nseg = g.nseg; a.seg_len = g.seg_len; a.nblk_i = g.nblk_i;
+        a.G = c->G;
+        a.ax = c->ax; a.ay = c->ay; a.az = c->az; a.pot = c->pot;
+        a.part = c->part;
+        a.skip_src = nullptr;
+        a.accumulate = 0;
+        NB_TRY(launch_force_f32(a, tpt, with_pot, s));
+    }
+    if (allow_timing) NB_TRY(record_stop(c, e0));
+    if (g.nseg > 1)
+        NB_TRY(launch_finalize(c->part, g.nseg, n, c->G, c->ax, c->ay, c->az, c->pot, with_pot, s));
+    // FP64 central-body terms: every target, sources = the central particles.
+    if (c->mode == NB_MODE_SPLIT_CENTRAL && c->ncentral > 0) {
+        const int64_t k = c->ncentral;
+        k_gather_central<<<(unsigned)((k + 255) / 256), 256, 0, s>>>(
+            (const double *)c->x, (const double *)c->y, (const double *)c->z,
+            (const double *)c->m, c->cidx, k, c->cx, c->cy, c->cz, c->cm);
+        NB_HIP(hipGetLastError());
+        Geometry gc = geometry(n, k, 1, 1);
+        ForceArgs<double> a{};
+        a.tx = (const double *)c->x; a.ty = (const double *)c->y; a.tz = (const double *)c->z;
+        a.th = c->h; a.nt = n;
+        a.sx = c->cx; a.sy = c->cy; a.sz = c->cz; a.sm = c->cm; a.ns = k;
+        a.tile_box = nullptr;
+        a.nseg = 1; a.seg_len = gc.seg_len; a.nblk_i = gc.nblk_i;
+        a.G = c->G;
+        a.ax = c->ax; a.ay = c->ay; a.az = c->az; a.pot = c->pot;
+        a.accumulate = 1;
+        NB_TRY(launch_force_f64(a, with_pot, s));
+    }
+    for (const BgField &f : c->bg)
+        NB_TRY(background(c->x, c->y, c->z, n, c->is_double, f, c->ax, c->ay, c->az,
+                          with_pot ? c->pot : nullptr, s));
+    return NB_OK;
+}
+
+int upload_scalar_h(nb_ctx *c) { return fill_h(c->coll, c->n, c->soft, c->h, c->stream); }
+
+__global__ void k_mark_central(const int64_t *__restrict__ perm, int64_t n, int64_t k,
+                               uint8_t *__restrict__ flags) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t u = perm ? perm[i] : i;
+    flags[i] = u < k ? 1 : 0;
+}
+__global__ void k_central_index(const int64_t *__restrict__ perm, int64_t n, int64_t k,
+                                int64_t *__restrict__ idx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t u = perm ? perm[i] : i;
+    if (u < k) idx[u] = i;
+}
+
+int rebuild_central(nb_ctx *c) {
+    if (c->ncentral <= 0 || c->n == 0) return NB_OK;
+    const unsigned g = (unsigned)((c->n + 255) / 256);
+    k_mark_central<<<g, 256, 0, c->stream>>>(c->perm, c->n, c->ncentral, c->central);
+    NB_HIP(hipGetLastError());
+    k_central_index<<<g, 256, 0, c->stream>>>(c->perm, c->n, c->ncentral, c->cidx);
+    NB_HIP(hipGetLastError());
+    return NB_OK;
+}
+}  // namespace
+
+namespace nb {
+int ctx_run_force(nb_ctx *c, bool with_pot, bool allow_timing) {
+    return run_force(c, with_pot, allow_timing);
+}
+int ctx_rebuild_central(nb_ctx *c) { return rebuild_central(c); }
+void ctx_drop_graph(nb_ctx *c) { drop_graph(c); }
+}  // namespace nb
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+int nb_version(void) {
+    return 10000 * NB_VERSION_MAJOR + 100 * NB_VERSION_MINOR + NB_VERSION_PATCH;
+}
+
+const char *nb_last_error(void) { return nb::last_error(); }
+
+int nb_device_info(nb_devinfo *out) {
+    clear_error();
+    if (!out) {
+        set_error("null output");
+        return NB_ERR_INVALID;
+    }
+    int dev = 0;
+    NB_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    NB_HIP(hipGetDeviceProperties(&p, dev));
+    memset(out, 0, sizeof *out);
+    snprintf(out->name, sizeof out->name, "%s", p.name);
+    snprintf(out->arch, sizeof out->arch, "%s", p.gcnArchName);
+    out->device = dev;
+    out->compute_units = p.multiProcessorCount;
+    out->clock_khz = p.clockRate;
+    out->lds_per_block = (int32_t)p.sharedMemPerBlock;
+    out->warp_size = p.warpSize;
+    out->max_threads_per_block = p.maxThreadsPerBlock;
+    out->global_mem_bytes = (int64_t)p.totalGlobalMem;
+    out->l2_bytes = (int64_t)p.l2CacheSize;
+    return NB_OK;
+}
+
+int nb_create(nb_ctx **out, int64_t n, int32_t precision_mode) {
+    clear_error();
+    if (!out) {
+        set_error("null output handle");
+        return NB_ERR_INVALID;
+    }
+    *out = nullptr;
+    if (n < 0 || n > ((int64_t)1 << 31)) {
+        set_error("particle count %lld out of range [0, 2^31]", (long long)n);
+        return NB_ERR_INVALID;
+    }
+    if (precision_mode < NB_MODE_FP64 || precision_mode > NB_MODE_SPLIT_CENTRAL) {
+        set_error("unknown precision mode %d", precision_mode);
+        return NB_ERR_INVALID;
+    }
+    nb_ctx *c = new (std::nothrow) nb_ctx();
+    if (!c) {
+        set_error("host allocation failed");
+        return NB_ERR_NOMEM;
+    }
+    c->n = n;
+    c->mode = precision_mode;
+    c->is_double = precision_mode != NB_MODE_FP32;
+    int rc = NB_OK;
+    do {
+        hipError_t e = hipGetDevice(&c->dev);
+        if (e != hipSuccess) {
+            set_error("hipGetDevice failed: %s", hipGetErrorString(e));
+            rc = NB_ERR_HIP;
+            break;
+        }
+        e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            set_error("hipStreamCreate failed: %s", hipGetErrorString(e));
+            rc = NB_ERR_HIP;
+            break;
+        }
+        const size_t es = c->is_double ? sizeof(double) : sizeof(float);
+        const size_t nn = (size_t)n;
+        if ((rc = dalloc(&c->x, es * nn)) || (rc = dalloc(&c->y, es * nn)) ||
+            (rc = dalloc(&c->z, es * nn)) || (rc = dalloc(&c->vx, es * nn)) ||
+            (rc = dalloc(&c->vy, es * nn)) || (rc = dalloc(&c->vz, es * nn)) ||
+            (rc = dalloc(&c->m, es * nn)) || (rc = dalloc(&c->h, sizeof(double) * nn)) ||
+            (rc = dalloc(&c->coll, nn)) || (rc = dalloc(&c->ax, sizeof(double) * nn)) ||
+            (rc = dalloc(&c->ay, sizeof(double) * nn)) ||
+            (rc = dalloc(&c->az, sizeof(double) * nn)) ||
+            (rc = dalloc(&c->pot, sizeof(double) * nn)) ||
+            (rc = dalloc(&c->tile_box, sizeof(double) * 8 * (size_t)((n + BS - 1) / BS + 1))) ||
+            (rc = dalloc(&c->stage, sizeof(double) * 3 * nn)) ||
+            (rc = dalloc(&c->red, sizeof(double) * (ENERGY_BLOCKS * 8 + 8))) ||
+            (rc = dalloc(&c->time, sizeof(double) * 2)) || (rc = dalloc(&c->flag, sizeof(unsigned))) ||
+            (rc = dalloc(&c->central, nn)))
+            break;
+        e = hipMemsetAsync(c->ax, 0, sizeof(double) * nn, c->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(c->ay, 0, sizeof(double) * nn, c->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(c->az, 0, sizeof(double) * nn, c->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(c->pot, 0, sizeof(double) * nn, c->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(c->central, 0, nn, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            set_error("initialising device buffers failed: %s", hipGetErrorString(e));
+            rc = NB_ERR_HIP;
+            break;
+        }
+        c->tun.targets_per_thread = default_tpt(n);
+        c->tun.j_segments = default_segments(n, c->tun.targets_per_thread);
+        c->tun.sort_interval = 0;
+        tuner_lookup(c->mode, n, &c->tun);
+        c->sort.interval = c->tun.sort_interval;
+    } while (0);
+    if (rc != NB_OK) {
+        std::string keep = nb::last_error();
+        nb_destroy(c);
+        set_error("%s", keep.c_str());
+        return rc;
+    }
+    *out = c;
+    return NB_OK;
+}
+
+int nb_destroy(nb_ctx *c) {
+    clear_error();
+    if (!c) return NB_OK;
+    (void)hipSetDevice(c->dev);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto &p : c->pending) {
+        (void)hipEventDestroy(p.first);
+        (void)hipEventDestroy(p.second);
+    }
+    drop_graph(c);
+    dfree(c->x); dfree(c->y); dfree(c->z);
+    dfree(c->vx); dfree(c->vy); dfree(c->vz);
+    dfree(c->m); dfree(c->h); dfree(c->coll);
+    dfree(c->ax); dfree(c->ay); dfree(c->az); dfree(c->pot);
+    dfree(c->perm);
+    dfree(c->tile_box); dfree(c->part); dfree(c->stage); dfree(c->red);
+    dfree(c->time); dfree(c->flag); dfree(c->central);
+    dfree(c->cx); dfree(c->cy); dfree(c->cz); dfree(c->cm); dfree(c->cidx);
+    sort_free(c);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return NB_OK;
+}
+
+int nb_set_params(nb_ctx *c, double G, const double soft[6]) {
+    NB_TRY(check_ctx(c, false));
+    if (!soft || !isfinite(G)) {
+        set_error("invalid G or softening table");
+        return NB_ERR_INVALID;
+    }
+    for (int k = 0; k < 6; ++k) {
+        if (!(soft[k] >= 0.0) || !isfinite(soft[k])) {
+            set_error("softening length %d must be finite and >= 0 (got %g)", k + 1, soft[k]);
+            return NB_ERR_INVALID;
+        }
+    }
+    drop_graph(c);
+    c->G = G;
+    for (int k = 0; k < 6; ++k) c->soft.h[k] = soft[k];
+    if (c->uploaded) {
+        NB_TRY(upload_scalar_h(c));
+        NB_HIP(hipStreamSynchronize(c->stream));
+    }
+    return NB_OK;
+}
+
+int nb_upload(nb_ctx *c, const double *pos, const double *vel, const double *mass,
+              const uint8_t *collection) {
+    NB_TRY(check_ctx(c, false));
+    const int64_t n = c->n;
+    if (n > 0 && (!pos || !mass || !collection)) {
+        set_error("pos, mass and collection are required");
+        return NB_ERR_INVALID;
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        if (collection[i] < 1 || collection[i] > 6) {
+            set_error("collection[%lld] = %d outside 1..6", (long long)i, (int)collection[i]);
+            return NB_ERR_INVALID;
+        }
+    }
+    drop_graph(c);
+    // Upload resets the internal order to the caller's order.
+    dfree(c->perm);
+    c->sort.valid = false;
+    hipStream_t s = c->stream;
+    const size_t nn = (size_t)n;
+    if (n > 0) {
+        NB_HIP(hipMemcpyAsync(c->stage, pos, sizeof(double) * 3 * nn, hipMemcpyHostToDevice, s));
+        NB_TRY(to_soa(c->stage, nullptr, n, c->x, c->y, c->z, c->is_double, s));
+        NB_HIP(hipStreamSynchronize(s));
+        if (vel) {
+            NB_HIP(hipMemcpyAsync(c->stage, vel, sizeof(double) * 3 * nn, hipMemcpyHostToDevice, s));
+            NB_TRY(to_soa(c->stage, nullptr, n, c->vx, c->vy, c->vz, c->is_double, s));
+        } else {
+            const size_t es = c->is_double ? sizeof(double) : sizeof(float);
+            NB_HIP(hipMemsetAsync(c->vx, 0, es * nn, s));
+            NB_HIP(hipMemsetAsync(c->vy, 0, es * nn, s));
+            NB_HIP(hipMemsetAsync(c->vz, 0, es * nn, s));
+        }
+        NB_HIP(hipStreamSynchronize(s));
+        NB_HIP(hipMemcpyAsync(c->stage, mass, sizeof(double) * nn, hipMemcpyHostToDevice, s));
+        NB_TRY(scalar_in(c->stage, nullptr, n, c->m, c->is_double, s));
+        NB_HIP(hipMemcpyAsync(c->coll, collection, nn, hipMemcpyHostToDevice, s));
+        NB_TRY(upload_scalar_h(c));
+        NB_HIP(hipStreamSynchronize(s));
+    }
+    c->uploaded = true;
+    c->force_calls = 0;
+    NB_TRY(rebuild_central(c));
+    NB_HIP(hipStreamSynchronize(s));
+    return NB_OK;
+}
+
+int nb_download(nb_ctx *c, double *pos, double *vel, double *acc, double *pot) {
+    NB_TRY(check_ctx(c));
+    const int64_t n = c->n;
+    if (n == 0) return NB_OK;
+    hipStream_t s = c->stream;
+    const size_t nn = (size_t)n;
+    if (pos) {
+        NB_TRY(to_aos(c->x, c->y, c->z, c->perm, n, c->is_double, c->stage, s));
+        NB_HIP(hipMemcpyAsync(pos, c->stage, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, s));
+        NB_HIP(hipStreamSynchronize(s));
+    }
+    if (vel) {
+        NB_TRY(to_aos(c->vx, c->vy, c->vz, c->perm, n, c->is_double, c->stage, s));
+        NB_HIP(hipMemcpyAsync(vel, c->stage, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, s));
+        NB_HIP(hipStreamSynchronize(s));
+    }
+    if (acc) {
+        NB_TRY(to_aos(c->ax, c->ay, c->az, c->perm, n, true, c->stage, s));
+        NB_HIP(hipMemcpyAsync(acc, c->stage, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, s));
+        NB_HIP(hipStreamSynchronize(s));
+    }
+    if (pot) {
+        NB_TRY(scalar_out(c->pot, c->perm, n, c->stage, s));
+        NB_HIP(hipMemcpyAsync(pot, c->stage, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
+        NB_HIP(hipStreamSynchronize(s));
+    }
+    return NB_OK;
+}
+
+int nb_compute_force(nb_ctx *c) {
+    NB_TRY(check_ctx(c));
+    NB_TRY(run_force(c, false, true));
+    return NB_OK;
+}
+
+int nb_compute_potential(nb_ctx *c, int32_t add_self_term) {
+    NB_TRY(check_ctx(c));
+    // The potential is produced together with the acceleration by the same
+    // kernel; keep the caller's accelerations by saving them around it.
+    hipStream_t s = c->stream;
+    const size_t nb3 = sizeof(double) * (size_t)c->n;
+    if (c->n > 0) {
+        NB_HIP(hipMemcpyAsync(c->stage, c->ax, nb3, hipMemcpyDeviceToDevice, s));
+        NB_HIP(hipMemcpyAsync(c->stage + c->n, c->ay, nb3, hipMemcpyDeviceToDevice, s));
+        NB_HIP(hipMemcpyAsync(c->stage + 2 * c->n, c->az, nb3, hipMemcpyDeviceToDevice, s));
+    }
+    NB_TRY(run_force(c, true, false));
+    if (c->n > 0) {
+        NB_HIP(hipMemcpyAsync(c->ax, c->stage, nb3, hipMemcpyDeviceToDevice, s));
+        NB_HIP(hipMemcpyAsync(c->ay, c->stage + c->n, nb3, hipMemcpyDeviceToDevice, s));
+        NB_HIP(hipMemcpyAsync(c->az, c->stage + 2 * c->n, nb3, hipMemcpyDeviceToDevice, s));
+    }
+    if (add_self_term) NB_TRY(self_term(c->m, c->h, c->n, c->G, c->is_double, c->pot, s));
+    NB_HIP(hipStreamSynchronize(s));
+    return NB_OK;
+}
+
+static int probe_points(nb_ctx *c, int64_t npts, const double *pts, double h, double *acc,
+                        double *pot) {
+    NB_TRY(check_ctx(c));
+    if (npts < 0 || (npts > 0 && !pts) || !(h >= 0.0) || !isfinite(h)) {
+        set_error("invalid probe points or softening");
+        return NB_ERR_INVALID;
+    }
+    if (npts == 0) return NB_OK;
+    hipStream_t s = c->stream;
+    const bool with_pot = pot != nullptr;
+    double *buf = nullptr;
+    // target xyz (double or float), outputs ax ay az pot, packed xyz staging
+    NB_TRY(dalloc(&buf, sizeof(double) * (size_t)npts * 10));
+    double *tx = buf, *ty = buf + npts, *tz = buf + 2 * npts;
+    double *oax = buf + 3 * npts, *oay = buf + 4 * npts, *oaz = buf + 5 * npts, *opot = buf + 6 * npts;
+    double *aos = buf + 7 * npts;
+    int rc = NB_OK;
+    do {
+        hipError_t e = hipMemcpyAsync(aos, pts, sizeof(double) * 3 * (size_t)npts,
+                                      hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) { set_error("probe upload: %s", hipGetErrorString(e)); rc = NB_ERR_HIP; break; }
+        if ((rc = launch_tile_boxes(c->x, c->y, c->z, c->is_double, c->n, c->tile_box, s))) break;
+        Geometry g = geometry(npts, c->n, 1, 1);
+        if (c->is_double) {
+            if ((rc = to_soa(aos, nullptr, npts, tx, ty, tz, true, s))) break;
+            ForceArgs<double> a{};
+            a.tx = tx; a.ty = ty; a.tz = tz; a.th = nullptr; a.th_const = h; a.nt = npts;
+            a.sx = (const double *)c->x; a.sy = (const double *)c->y; a.sz = (const double *)c->z;
+            a.sm = (const double *)c->m; a.ns = c->n;
+            a.tile_box = c->tile_box; a.use_origin = 1;
+            a.nseg = 1; a.seg_len = g.seg_len; a.nblk_i = g.nblk_i; a.G = c->G;
+            a.ax = oax; a.ay = oay; a.az = oaz; a.pot = opot; a.accumulate = 0;
+            rc = c->mode == NB_MODE_FP64 ? launch_force_f64(a, with_pot, s)
+                                         : launch_force_split(a, 1, with_pot, s);
+            if (rc) break;
+        } else {
+            // FP32 state: probe coordinates are rounded to FP32 like the particles.
+            float *fx = (float *)tx, *fy = (float *)ty, *fz = (float *)tz;
+            if ((rc = to_soa(aos, nullptr, npts, fx, fy, fz, false, s))) break;
+            ForceArgs<float> a{};
+            a.tx = fx; a.ty = fy; a.tz = fz; a.th = nullptr; a.th_const = h; a.nt = npts;
+            a.sx = (const float *)c->x; a.sy = (const float *)c->y; a.sz = (const float *)c->z;
+            a.sm = (const float *)c->m; a.ns = c->n;
+            a.tile_box = c->tile_box; a.use_origin = 0;
+            a.nseg = 1; a.seg_len = g.seg_len; a.nblk_i = g.nblk_i; a.G = c->G;
+            a.ax = oax; a.ay = oay; a.az = oaz; a.pot = opot; a.accumulate = 0;
+            if ((rc = launch_force_f32(a, 1, with_pot, s))) break;
+            if ((rc = to_soa(aos, nullptr, npts, tx, ty, tz, true, s))) break;
+        }
+        for (const BgField &f : c->bg)
+            if ((rc = background(tx, ty, tz, npts, true, f, oax, oay, oaz, with_pot ? opot : nullptr, s))) break;
+        if (rc) break;
+        if (acc) {
+            if ((rc = to_aos(oax, oay, oaz, nullptr, npts, true, aos, s))) break;
+            e = hipMemcpyAsync(acc, aos, sizeof(double) * 3 * (size_t)npts, hipMemcpyDeviceToHost, s);
+            if (e != hipSuccess) { set_error("probe download: %s", hipGetErrorString(e)); rc = NB_ERR_HIP; break; }
+        }
+        if (pot) {
+            e = hipMemcpyAsync(pot, opot, sizeof(double) * (size_t)npts, hipMemcpyDeviceToHost, s);
+            if (e != hipSuccess) { set_error("probe download: %s", hipGetErrorString(e)); rc = NB_ERR_HIP; break; }
+        }
+        e = hipStreamSynchronize(s);
+        if (e != hipSuccess) { set_error("probe sync: %s", hipGetErrorString(e)); rc = NB_ERR_HIP; break; }
+    } while (0);
+    (void)hipStreamSynchronize(s);
+    dfree(buf);
+    return rc;
+}
+
+int nb_force_at_points(nb_ctx *c, int64_t npts, const double *pts, double h, double *acc) {
+    if (!acc && npts > 0) {
+        set_error("null output");
+        return NB_ERR_INVALID;
+    }
+    return probe_points(c, npts, pts, h, acc, nullptr);
+}
+
+int nb_potential_at_points(nb_ctx *c, int64_t npts, const double *pts, double h, double *pot) {
+    if (!pot && npts > 0) {
+        set_error("null output");
+        return NB_ERR_INVALID;
+    }
+    return probe_points(c, npts, pts, h, nullptr, pot);
+}
+
+int nb_half_kick(nb_ctx *c, int32_t forward, double dt) {
+    NB_TRY(check_ctx(c));
+    if (!isfinite(dt) || (forward != 1 && forward != -1)) {
+        set_error("invalid half kick (forward=%d, dt=%g)", forward, dt);
+        return NB_ERR_INVALID;
+    }
+    // Vel += Acc * forward * 0.5 * dt
+    NB_TRY(kick(c->vx, c->vy, c->vz, c->ax, c->ay, c->az, c->n, (double)forward * 0.5 * dt,
+                c->is_double, c->stream));
+    return NB_OK;
+}
+
+int nb_drift(nb_ctx *c, double dt) {
+    NB_TRY(check_ctx(c));
+    if (!isfinite(dt)) {
+        set_error("invalid drift dt %g", dt);
+        return NB_ERR_INVALID;
+    }
+    // t_new - t_old == dt exactly when t_old == 0.
+    NB_TRY(set_time(c->time, 0, 0.0, c->stream));
+    NB_TRY(drift(c->x, c->y, c->z, c->vx, c->vy, c->vz, c->n, dt, c->time, 0, 1, c->is_double,
+                 c->stream));
+    return NB_OK;
+}
+
+int nb_kick(nb_ctx *c, double dt) {
+    NB_TRY(check_ctx(c));
+    if (!isfinite(dt)) {
+        set_error("invalid kick dt %g", dt);
+        return NB_ERR_INVALID;
+    }
+    NB_TRY(kick(c->vx, c->vy, c->vz, c->ax, c->ay, c->az, c->n, dt, c->is_double, c->stream));
+    return NB_OK;
+}
+
+static int enqueue_step(nb_ctx *c, int64_t k, double dt, bool allow_timing) {
+    const int rd = (int)(k & 1), wr = 1 - rd;
+    NB_TRY(drift(c->x, c->y, c->z, c->vx, c->vy, c->vz, c->n, dt, c->time, rd, wr, c->is_double,
+                 c->stream));
+    NB_TRY(run_force(c, false, allow_timing));
+    NB_TRY(kick(c->vx, c->vy, c->vz, c->ax, c->ay, c->az, c->n, dt, c->is_double, c->stream));
+    return NB_OK;
+}
+
+int nb_kdk_steps(nb_ctx *c, int64_t nsteps, double dt, double t_begin, int32_t use_graph) {
+    NB_TRY(check_ctx(c));
+    if (nsteps < 0 || !isfinite(dt) || !isfinite(t_begin)) {
+        set_error("invalid kdk arguments (nsteps=%lld, dt=%g)", (long long)nsteps, dt);
+        return NB_ERR_INVALID;
+    }
+    hipStream_t s = c->stream;
+    NB_TRY(set_time(c->time, 0, t_begin, s));
+    // Graph replay needs a static schedule: no re-sorting inside the graph.
+    const bool graph_ok = use_graph && c->sort.interval == 0 && nsteps >= 2;
+    int64_t k = 0;
+    if (graph_ok) {
+        const int64_t chunk = std::min<int64_t>(nsteps & ~(int64_t)1, 16);
+        const bool reuse = c->gexec && c->g_steps == chunk && c->g_dt == dt &&
+                           c->g_tun.targets_per_thread == c->tun.targets_per_thread &&
+                           c->g_tun.j_segments == c->tun.j_segments && c->g_nbg == c->bg.size();
+        if (!reuse) {
+            drop_graph(c);
+            // Make sure partial buffers exist before capture (no allocation inside).
+            Geometry g = geometry(c->n, c->n, c->mode == NB_MODE_FP64 ? 1 : c->tun.targets_per_thread,
+                                  c->tun.j_segments);
+            NB_TRY(ensure_part(c, g.nseg));
+            NB_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            int rc = NB_OK;
+            for (int64_t q = 0; q < chunk && rc == NB_OK; ++q) rc = enqueue_step(c, q, dt, false);
+            hipGraph_t graph = nullptr;
+            hipError_t e = hipStreamEndCapture(s, &graph);
+            if (rc != NB_OK) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            if (e != hipSuccess) {
+                set_error("hipStreamEndCapture failed: %s", hipGetErrorString(e));
+                if (graph) (void)hipGraphDestroy(graph);
+                return NB_ERR_HIP;
+            }
+            e = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) {
+                c->gexec = nullptr;
+                set_error("hipGraphInstantiate failed: %s", hipGetErrorString(e));
+                return NB_ERR_HIP;
+            }
+            c->g_steps = chunk;
+            c->g_dt = dt;
+            c->g_tun = c->tun;
+            c->g_nbg = c->bg.size();
+            // Force calls made inside the capture were counted; undo.
+            c->force_calls -= chunk;
+        }
+        for (; k + c->g_steps <= nsteps; k += c->g_steps) {
+            NB_HIP(hipGraphLaunch(c->gexec, s));
+            c->force_calls += c->g_steps;
+        }
+    }
+    for (; k < nsteps; ++k) NB_TRY(enqueue_step(c, k, dt, true));
+    return NB_OK;
+}
+
+int nb_energy(nb_ctx *c, double *K, double *W, double P[3], double L[3]) {
+    NB_TRY(check_ctx(c));
+    hipStream_t s = c->stream;
+    if (W) NB_TRY(nb_compute_potential(c, 0));
+    double host[8];
+    NB_TRY(energy_reduce(c->x, c->y, c->z, c->vx, c->vy, c->vz, c->m, W ? c->pot : nullptr, c->n,
+                         c->is_double, c->red, s));
+    NB_HIP(hipMemcpyAsync(host, c->red + (size_t)ENERGY_BLOCKS * 8, sizeof host,
+                          hipMemcpyDeviceToHost, s));
+    NB_HIP(hipStreamSynchronize(s));
+    if (K) *K = host[0];
+    if (W) *W = 0.5 * host[1];
+    if (P) { P[0] = host[2]; P[1] = host[3]; P[2] = host[4]; }
+    if (L) { L[0] = host[5]; L[1] = host[6]; L[2] = host[7]; }
+    return NB_OK;
+}
+
+int nb_set_central(nb_ctx *c, int64_t count) {
+    NB_TRY(check_ctx(c, false));
+    if (count < 0 || count > c->n) {
+        set_error("central count %lld out of range", (long long)count);
+        return NB_ERR_INVALID;
+    }
+    if (c->mode != NB_MODE_SPLIT_CENTRAL && count > 0) {
+        set_error("central bodies need NB_MODE_SPLIT_CENTRAL");
+        return NB_ERR_STATE;
+    }
+    drop_graph(c);
+    dfree(c->cx); dfree(c->cy); dfree(c->cz); dfree(c->cm); dfree(c->cidx);
+    c->ncentral = count;
+    if (count > 0) {
+        NB_TRY(dalloc(&c->cx, sizeof(double) * count));
+        NB_TRY(dalloc(&c->cy, sizeof(double) * count));
+        NB_TRY(dalloc(&c->cz, sizeof(double) * count));
+        NB_TRY(dalloc(&c->cm, sizeof(double) * count));
+        NB_TRY(dalloc(&c->cidx, sizeof(int64_t) * count));
+    }
+    NB_HIP(hipMemsetAsync(c->central, 0, (size_t)c->n, c->stream));
+    NB_TRY(rebuild_central(c));
+    NB_HIP(hipStreamSynchronize(c->stream));
+    return NB_OK;
+}
+
+int nb_add_background(nb_ctx *c, int32_t kind, const double *params) {
+    NB_TRY(check_ctx(c, false));
+    if (kind == 0) {
+        drop_graph(c);
+        c->bg.clear();
+        return NB_OK;
+    }
+    if ((kind != 1 && kind != 2) || !params) {
+        set_error("unknown background field kind %d", kind);
+        return NB_ERR_INVALID;
+    }
+    if (c->bg.size() >= 8) {
+        set_error("at most 8 background fields");
+        return NB_ERR_INVALID;
+    }
+    BgField f{};
+    f.kind = kind;
+    const int np = kind == 1 ? 4 : 5;
+    for (int k = 0; k < np; ++k) {
+        if (!isfinite(params[k])) {
+            set_error("non-finite background parameter %d", k);
+            return NB_ERR_INVALID;
+        }
+        f.p[k] = params[k];
+    }
+    drop_graph(c);
+    c->bg.push_back(f);
+    return NB_OK;
+}
+
+int nb_sync(nb_ctx *c) {
+    NB_TRY(check_ctx(c, false));
+    NB_HIP(hipStreamSynchronize(c->stream));
+    return NB_OK;
+}
+
+int nb_set_tuning(nb_ctx *c, const nb_tuning *t) {
+    NB_TRY(check_ctx(c, false));
+    if (!t || (t->targets_per_thread != 1 && t->targets_per_thread != 2 &&
+               t->targets_per_thread != 4) ||
+        t->j_segments < 1 || t->j_segments > 256 || t->sort_interval < 0) {
+        set_error("invalid tuning");
+        return NB_ERR_INVALID;
+    }
+    drop_graph(c);
+    c->tun = *t;
+    c->sort.interval = t->sort_interval;
+    return NB_OK;
+}
+
+int nb_get_tuning(nb_ctx *c, nb_tuning *t) {
+    NB_TRY(check_ctx(c, false));
+    if (!t) {
+        set_error("null output");
+        return NB_ERR_INVALID;
+    }
+    *t = c->tun;
+    return NB_OK;
+}
+
+int nb_autotune(nb_ctx *c) {
+    NB_TRY(check_ctx(c));
+    return autotune_ctx(c);
+}
+
+int nb_set_timing(nb_ctx *c, int32_t enable) {
+    NB_TRY(check_ctx(c, false));
+    NB_TRY(collect_timing(c));
+    c->timing = enable != 0;
+    c->tcount = 0;
+    c->tms = 0.0;
+    return NB_OK;
+}
+
+int nb_get_timing(nb_ctx *c, int64_t *count, double *total_ms) {
+    NB_TRY(check_ctx(c, false));
+    NB_TRY(collect_timing(c));
+    if (count) *count = c->tcount;
+    if (total_ms) *total_ms = c->tms;
+    return NB_OK;
+}
+
+}  // extern "C"

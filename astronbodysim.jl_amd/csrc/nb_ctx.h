@@ -1,0 +1,102 @@
+// nb_ctx.h -- the one-system context behind the opaque nb_ctx handle.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <utility>
+#include <vector>
+
+#include "nb_integrate.h"
+#include "nb_internal.h"
+
+namespace nb {
+
+// Morton-order reordering state (nb_sort.hip).
+struct SortState {
+    int interval = 0;          // re-sort every `interval` force calls (0: never)
+    bool valid = false;        // the internal order is a Morton order
+    void *tmp = nullptr;       // rocPRIM temporary storage
+    size_t tmp_bytes = 0;
+    uint64_t *keys = nullptr;  // [2][n]
+    int64_t *idx = nullptr;    // [2][n]
+    double *box = nullptr;     // global bounding box (8 doubles)
+    void *spare = nullptr;     // n * 8 bytes gather target
+};
+
+}  // namespace nb
+
+struct nb_ctx {
+    int dev = 0;
+    int mode = NB_MODE_SPLIT;
+    int64_t n = 0;
+    bool is_double = true;
+    hipStream_t stream = nullptr;
+    double G = 1.0;
+    nb::SoftTable soft{{1e-4, 1e-4, 1e-4, 1e-4, 1e-4, 1e-4}};
+    void *x = nullptr, *y = nullptr, *z = nullptr;
+    void *vx = nullptr, *vy = nullptr, *vz = nullptr;
+    void *m = nullptr;
+    double *h = nullptr;
+    uint8_t *coll = nullptr;
+    double *ax = nullptr, *ay = nullptr, *az = nullptr, *pot = nullptr;
+    int64_t *perm = nullptr;      // internal -> caller index (nullptr: identity)
+    double *tile_box = nullptr;
+    double *part = nullptr;
+    int part_cap = 0;             // segments the partial buffer holds
+    double *stage = nullptr;      // 3n doubles
+    double *red = nullptr;        // energy scratch
+    double *time = nullptr;       // two FP64 time slots for device-side drift
+    unsigned *flag = nullptr;
+    uint8_t *central = nullptr;   // internal flags: source evaluated in FP64
+    int64_t ncentral = 0;
+    double *cx = nullptr, *cy = nullptr, *cz = nullptr, *cm = nullptr;  // compacted central sources
+    int64_t *cidx = nullptr;      // internal indices of the central sources
+    nb::SortState sort;
+    int64_t force_calls = 0;
+    nb_tuning tun{2, 1, 0, 0};
+    bool uploaded = false;
+    std::vector<nb::BgField> bg;
+    // timing
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    int64_t tcount = 0;
+    double tms = 0.0;
+    // graph cache for nb_kdk_steps
+    hipGraphExec_t gexec = nullptr;
+    int64_t g_steps = 0;
+    double g_dt = 0.0;
+    nb_tuning g_tun{0, 0, 0, 0};
+    size_t g_nbg = 0;
+};
+
+namespace nb {
+template <typename P>
+inline int dalloc(P **p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc((void **)p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        set_error("hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+        return NB_ERR_NOMEM;
+    }
+    return NB_OK;
+}
+
+template <typename P>
+inline void dfree(P *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+// nb_sort.hip
+int morton_reorder(nb_ctx *c);
+void sort_free(nb_ctx *c);
+// nb_tune.hip
+void tuner_lookup(int mode, int64_t n, nb_tuning *t);
+int autotune_ctx(nb_ctx *c);
+// nb_api.hip
+int ctx_run_force(nb_ctx *c, bool with_pot, bool allow_timing);
+int ctx_rebuild_central(nb_ctx *c);
+void ctx_drop_graph(nb_ctx *c);
+}  // namespace nb

@@ -1,0 +1,65 @@
+// nb_internal.h -- host-side internals shared by the C-ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/nbody_mi355x.h"
+
+namespace nb {
+
+// Source tile = one LDS stage = one block of BS threads.
+constexpr int BS = 256;
+
+void set_error(const char *fmt, ...);
+void clear_error();
+
+#define NB_HIP(call)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            ::nb::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                            __LINE__);                                                     \
+            return NB_ERR_HIP;                                                             \
+        }                                                                                  \
+    } while (0)
+
+#define NB_TRY(expr)              \
+    do {                          \
+        int rc_ = (expr);         \
+        if (rc_ != NB_OK) return rc_; \
+    } while (0)
+
+// Arguments of the FP32-pair all-pairs kernel (modes FP32 / SPLIT).
+// Targets and sources may be different arrays (probe points).
+template <typename T>
+struct ForceArgs {
+    const T *tx, *ty, *tz;   // targets (internal order)
+    const double *th;        // per-target softening h (nullptr => th_const)
+    double th_const;
+    int64_t nt;
+    const T *sx, *sy, *sz, *sm;  // sources
+    int64_t ns;
+    const double *tile_box;  // [ntiles][8]: lo xyz, hi xyz, (unused) of each BS-source tile
+    int use_origin;          // 1: per-block FP64 origin (SPLIT), 0: absolute FP32 coordinates
+    int nseg;                // source segments (grid = nblk_i * nseg)
+    int64_t seg_len;         // sources per segment (multiple of BS)
+    int nblk_i;              // target blocks
+    double G;
+    double *ax, *ay, *az, *pot;  // final outputs (nseg == 1)
+    double *part;                // partials [nseg][4][nt] (nseg > 1)
+    const uint8_t *skip_src;     // sources excluded from the FP32 sum (central bodies) or nullptr
+    int accumulate;              // 1: add G*sum to the outputs instead of storing (nseg == 1)
+};
+
+// Launchers (nb_force.hip)
+int launch_tile_boxes(const void *x, const void *y, const void *z, int is_double, int64_t n,
+                      double *box, hipStream_t s);
+int launch_force_f32(const ForceArgs<float> &a, int tpt, bool pot, hipStream_t s);
+int launch_force_split(const ForceArgs<double> &a, int tpt, bool pot, hipStream_t s);
+int launch_force_f64(const ForceArgs<double> &a, bool pot, hipStream_t s);
+int launch_finalize(const double *part, int nseg, int64_t n, double G, double *ax, double *ay,
+                    double *az, double *pot, bool with_pot, hipStream_t s);
+
+}  // namespace nb

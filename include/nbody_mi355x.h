@@ -1,0 +1,189 @@
+/*
+ * nbody_mi355x.h -- C-ABI of the MI355X direct-summation gravity engine.
+ *
+ * Drop-in for the (DirectSum, GPU) methods of AstroNbodySim.jl.  Every entry
+ * point below cites the reference method it replaces (paths relative to the
+ * AstroNbodySim.jl repository root).  Plain pointers and sizes only; no C++
+ * exceptions cross this boundary; every call returns an int status
+ * (NB_OK == 0, negative on error) and nb_last_error() describes the last
+ * failure of the calling host thread.
+ *
+ * Host arrays are caller-owned and copied synchronously.  Vectors are passed
+ * as packed xyz triples (3*n doubles), the memory layout of the reference's
+ * Vector{PVector{Float64}} columns.  Collections are the integers Int(c)
+ * that index the 6-entry ForceSofteningTable (1..6).
+ *
+ * Precision modes (nb_create):
+ *   NB_MODE_FP64  : FP64 pair arithmetic and state (correctness mode).
+ *   NB_MODE_FP32  : FP32 state (positions, velocities, masses), FP32 pair
+ *                   arithmetic; the reference's floattype = Float32 on GPU.
+ *   NB_MODE_SPLIT : FP64 state; FP32 pair arithmetic on positions taken
+ *                   relative to a per-block FP64 origin; per-tile FP32 sums
+ *                   folded into FP64 accumulators; FP64 kick/drift/energy.
+ *   NB_MODE_SPLIT_CENTRAL : NB_MODE_SPLIT plus the pair terms of the
+ *                   particles marked central (nb_set_central) in FP64.
+ *
+ * One context owns one HIP stream on the device that was current when it was
+ * created.  Calls on one context must not overlap; separate contexts are
+ * independent.
+ */
+#ifndef NBODY_MI355X_H
+#define NBODY_MI355X_H
+
+#include <stdint.h>
+
+#if defined(__GNUC__)
+#define NB_API __attribute__((visibility("default")))
+#else
+#define NB_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NB_OK 0
+#define NB_ERR_INVALID (-1)   /* bad argument                                  */
+#define NB_ERR_HIP (-2)       /* HIP runtime error                             */
+#define NB_ERR_NOMEM (-3)     /* device or host allocation failed              */
+#define NB_ERR_STATE (-4)     /* call out of order (e.g. no upload yet)        */
+#define NB_ERR_NONFINITE (-5) /* NaN/Inf found in state or result              */
+#define NB_ERR_TIMESTEP (-6)  /* adaptive timestep below MinStep / bad integer step */
+
+#define NB_MODE_FP64 0
+#define NB_MODE_FP32 1
+#define NB_MODE_SPLIT 2
+#define NB_MODE_SPLIT_CENTRAL 3
+
+typedef struct nb_ctx nb_ctx;
+typedef struct nb_batch nb_batch;
+
+typedef struct nb_devinfo {
+    char name[128];
+    char arch[64];
+    int32_t device;
+    int32_t compute_units;
+    int32_t clock_khz;
+    int32_t lds_per_block;
+    int32_t warp_size;
+    int32_t max_threads_per_block;
+    int64_t global_mem_bytes;
+    int64_t l2_bytes;
+} nb_devinfo;
+
+/* Launch configuration of the all-pairs kernel (chosen by the autotuner). */
+typedef struct nb_tuning {
+    int32_t targets_per_thread; /* 1, 2 or 4                                   */
+    int32_t j_segments;         /* source-range split (>=1; partials reduced in fixed order) */
+    int32_t sort_interval;      /* re-sort into Morton order every k force calls (0 = never) */
+    int32_t reserved;
+} nb_tuning;
+
+/* ---- library / device ------------------------------------------------- */
+
+/* Library version: 10000*major + 100*minor + patch. */
+NB_API int nb_version(void);
+/* Message of the last failed call on this host thread ("" if none). */
+NB_API const char *nb_last_error(void);
+/* Device properties; replaces gpuinfo()  src/base/UtilsGPU.jl:3-21. */
+NB_API int nb_device_info(nb_devinfo *out);
+
+/* ---- one-system context: Simulation(d; device = GPU()) ---------------- */
+
+/* Allocate device state for n particles; replaces the DirectSum & GPU branch
+ * of Simulation  src/base/Config.jl:1072-1089. */
+NB_API int nb_create(nb_ctx **out, int64_t n, int32_t precision_mode);
+NB_API int nb_destroy(nb_ctx *ctx);
+/* Gravitational constant and ForceSofteningTable (6 entries, indexed by
+ * collection-1); replaces sim.config.constants.G / GravityConfig
+ * src/base/Config.jl:137-146 and set_softlen!  src/tools/SofteningLength.jl:79-93. */
+NB_API int nb_set_params(nb_ctx *ctx, double G, const double soft[6]);
+/* Copy particle state host->device (pos/vel 3n, mass n, collection n);
+ * replaces cu(StructArray(d))  src/base/Config.jl:1078.  vel may be NULL (zero). */
+NB_API int nb_upload(nb_ctx *ctx, const double *pos, const double *vel, const double *mass,
+              const uint8_t *collection);
+/* Copy state device->host in the caller's particle order; any pointer may be
+ * NULL.  Replaces get_all_data / get_local_data  src/base/Config.jl:1199-1201,1228-1230. */
+NB_API int nb_download(nb_ctx *ctx, double *pos, double *vel, double *acc, double *pot);
+
+/* ---- forces and potentials -------------------------------------------- */
+
+/* Acc[i] = sum_j spline pair force (self pair contributes 0), every particle;
+ * replaces compute_force(sim, DirectSum, GPU)  src/directsumgpu/gravity.jl:36-45,81-85
+ * (kernel :22-34, pair function src/directsum/gravity.jl:8-28). */
+NB_API int nb_compute_force(nb_ctx *ctx);
+/* Potential[i] = sum_j spline pair potential; replaces
+ * compute_potential(sim, DirectSum, CPU)  src/directsum/potential.jl:142-148
+ * (no GPU method exists in the reference).  add_self_term != 0 also applies
+ * the public compute_potential(sim) self term  src/base/Potential.jl:5-24. */
+NB_API int nb_compute_potential(nb_ctx *ctx, int32_t add_self_term);
+/* Force / potential at npts probe points with a caller-given softening h;
+ * replaces compute_force(sim, pos, h, DirectSum, ...)  src/directsum/gravity.jl:171-194
+ * and compute_potential(sim, pos, h, ...)  src/directsum/potential.jl:155-180. */
+NB_API int nb_force_at_points(nb_ctx *ctx, int64_t npts, const double *pts, double h, double *acc);
+NB_API int nb_potential_at_points(nb_ctx *ctx, int64_t npts, const double *pts, double h, double *pot);
+
+/* ---- constant-timestep leapfrog (KDK) --------------------------------- */
+
+/* Vel += Acc * forward * 0.5 * dt; replaces leapfrog_half_kick (ConstantTimestep)
+ * src/directsumgpu/timestep.jl:194-214. */
+NB_API int nb_half_kick(nb_ctx *ctx, int32_t forward, double dt);
+/* Pos += Vel * dt; replaces drift_particles_kernel  src/base/Timestep.jl:181-191
+ * (the caller passes t_new - t_old, the CPU semantics). */
+NB_API int nb_drift(nb_ctx *ctx, double dt);
+/* Vel += Acc * dt; replaces advance_and_find_timestep (ConstantTimestep)
+ * src/directsumgpu/timestep.jl:156-176. */
+NB_API int nb_kick(nb_ctx *ctx, double dt);
+/* nsteps x step(sim, DirectSum, GPU)  src/directsumgpu/timestep.jl:216-224:
+ * drift by (t+dt)-t with t accumulated from t_begin as the reference does
+ * (src/base/Timestep.jl:198, :183), compute_force, kick by dt.  With
+ * use_graph != 0 the steps are replayed from a captured HIP graph. */
+NB_API int nb_kdk_steps(nb_ctx *ctx, int64_t nsteps, double dt, double t_begin, int32_t use_graph);
+
+/* ---- reductions --------------------------------------------------------- */
+
+/* K = sum 1/2 m v.v, W = 1/2 sum m phi (phi recomputed, unbiased, as
+ * total_potential(sim, DirectSum, CPU)  src/directsum/potential.jl:150-153),
+ * P = sum m v, L = sum m x cross v; FP64, fixed-order reductions.  Replaces
+ * total_kinetic / total_potential / total_momentum / total_angular_momentum
+ * src/base/Energy.jl:13-100.  Any output pointer may be NULL; W == NULL skips
+ * the potential evaluation. */
+NB_API int nb_energy(nb_ctx *ctx, double *K, double *W, double P[3], double L[3]);
+
+/* ---- hierarchical systems, background fields ---------------------------- */
+
+/* Mark the first `count` particles (caller order) as central bodies: in
+ * NB_MODE_SPLIT_CENTRAL their pair terms run in FP64.  No reference
+ * counterpart: the reference evaluates every pair in one precision. */
+NB_API int nb_set_central(nb_ctx *ctx, int64_t count);
+/* Analytic FP64 background field added after the pair sum; replaces the
+ * GPU bgforce closures  src/directsumgpu/gravity.jl:47-53,64-79 and
+ * apply_background_potential  src/base/PostProcessing/Potential.jl:1-7 with a
+ * fixed menu.  kind 0 clears all fields; kind 1 = point mass
+ * params = {GM, cx, cy, cz}; kind 2 = Paczynski-Wiita pseudo-Newtonian mass
+ * params = {GM, cx, cy, cz, r_g}  (src/tools/BlackHole.jl:48-60). */
+NB_API int nb_add_background(nb_ctx *ctx, int32_t kind, const double *params);
+
+/* ---- synchronisation, tuning, timing ------------------------------------ */
+
+NB_API int nb_sync(nb_ctx *ctx);
+NB_API int nb_set_tuning(nb_ctx *ctx, const nb_tuning *t);
+NB_API int nb_get_tuning(nb_ctx *ctx, nb_tuning *t);
+/* Time the candidate launch configurations on this context's data and keep
+ * the fastest (state is restored afterwards).  No reference counterpart
+ * (GPU_NumThreads is a fixed keyword there, src/base/Config.jl:295-305). */
+NB_API int nb_autotune(nb_ctx *ctx);
+/* Force-kernel timing with HIP events on the context's stream: enable, then
+ * read the number of timed force evaluations and their summed duration. */
+NB_API int nb_set_timing(nb_ctx *ctx, int32_t enable);
+NB_API int nb_get_timing(nb_ctx *ctx, int64_t *count, double *total_ms);
+/* Tuning table persistence: entries keyed by (precision mode, ceil(log2 N)),
+ * consulted by nb_create / nb_batch_create. */
+NB_API int nb_tuner_save(const char *path);
+NB_API int nb_tuner_load(const char *path);
+NB_API int nb_tuner_clear(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NBODY_MI355X_H */

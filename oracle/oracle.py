@@ -1,0 +1,318 @@
+"""FP64 CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+ctypes wrapper of liboracle_ds.so (ds_oracle.c, a C restatement of the
+reference's src/directsum CPU path) plus a Python restatement of the
+constant-timestep run loop (src/run.jl:46-184).  Only the tests, the smoke
+check and the CPU-baseline leg of bench.py may import this module; the
+product path never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle_ds.so"
+
+FAITHFUL, TUNED, LONGDOUBLE, SIMD = 0, 1, 2, 3
+VARIANTS = {"faithful": FAITHFUL, "tuned": TUNED, "longdouble": LONGDOUBLE, "simd": SIMD}
+
+_D = C.POINTER(C.c_double)
+_U8 = C.POINTER(C.c_uint8)
+
+
+def build(force: bool = False):
+    """Compile liboracle_ds.so with the committed Makefile."""
+    cmd = ["make", "-C", str(HERE)] + (["-B"] if force else [])
+    subprocess.run(cmd, check=True, capture_output=True)
+
+
+def _lib():
+    if not LIB_PATH.exists():
+        build()
+    lib = C.CDLL(str(LIB_PATH))
+    I64, I = C.c_int64, C.c_int
+    lib.ds_pair_force.argtypes = [_D, _D, C.c_double, C.c_double, C.c_double, _D]
+    lib.ds_pair_force.restype = None
+    lib.ds_pair_potential.argtypes = [_D, _D, C.c_double, C.c_double, C.c_double]
+    lib.ds_pair_potential.restype = C.c_double
+    lib.ds_compute_force.argtypes = [I64, _D, _D, _D, _D, _U8, _D, C.c_double, _U8, I64, I64,
+                                     I, I, I, _D]
+    lib.ds_compute_force.restype = I
+    lib.ds_compute_potential.argtypes = [I64, _D, _D, _D, _D, _U8, _D, C.c_double, I64, I64,
+                                         I, I, I, _D]
+    lib.ds_compute_potential.restype = I
+    lib.ds_remove_self_potential.argtypes = [I64, _D, _U8, _D, C.c_double, _D]
+    lib.ds_remove_self_potential.restype = None
+    lib.ds_force_at_points.argtypes = [I64, _D, C.c_double, I64, _D, _D, _D, _D, C.c_double,
+                                       I, I, _D]
+    lib.ds_force_at_points.restype = I
+    lib.ds_potential_at_points.argtypes = [I64, _D, C.c_double, I64, _D, _D, _D, _D,
+                                           C.c_double, I, I, _D]
+    lib.ds_potential_at_points.restype = I
+    lib.ds_background_point_mass.argtypes = [I64, _D, _D, _D, C.c_double, C.c_double, _D, _D, _D]
+    lib.ds_background_point_mass.restype = None
+    lib.ds_half_kick.argtypes = [I64, _D, _D, I, C.c_double]
+    lib.ds_half_kick.restype = None
+    lib.ds_kick.argtypes = [I64, _D, _D, C.c_double]
+    lib.ds_kick.restype = None
+    lib.ds_drift.argtypes = [I64, _D, _D, _D, _D, C.c_double]
+    lib.ds_drift.restype = None
+    lib.ds_energy.argtypes = [I64, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D]
+    lib.ds_energy.restype = I
+    lib.ds_num_threads.argtypes = []
+    lib.ds_num_threads.restype = I
+    return lib
+
+
+lib = _lib()
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_D)
+
+
+def _soa(pos):
+    pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
+    return (np.ascontiguousarray(pos[:, 0]), np.ascontiguousarray(pos[:, 1]),
+            np.ascontiguousarray(pos[:, 2]))
+
+
+def _table(soft):
+    return np.ascontiguousarray(np.broadcast_to(np.asarray(soft, dtype=np.float64), (6,)))
+
+
+def num_threads() -> int:
+    return lib.ds_num_threads()
+
+
+def pair_force(p, s, mass, G, h):
+    """a1: src/directsum/gravity.jl:8-28 for one pair."""
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    out = np.zeros(3)
+    lib.ds_pair_force(_p(p), _p(s), float(mass), float(G), float(h), _p(out))
+    return out
+
+
+def pair_potential(p, s, mass, G, h):
+    """a2: src/directsum/potential.jl:2-27 for one pair."""
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    return lib.ds_pair_potential(_p(p), _p(s), float(mass), float(G), float(h))
+
+
+def compute_force(pos, mass, soft, G=1.0, collection=None, *, variant="tuned", skip_self=False,
+                  targets=None, active=None, nthreads=0):
+    """a5: Acc[i] for targets [t0, t1) (all by default); returns (n, 3) with
+    zeros for targets outside the range."""
+    x, y, z = _soa(pos)
+    n = x.size
+    m = np.ascontiguousarray(mass, dtype=np.float64)
+    coll = np.ones(n, dtype=np.uint8) if collection is None else np.ascontiguousarray(
+        collection, dtype=np.uint8)
+    t0, t1 = (0, n) if targets is None else targets
+    acc = np.zeros((n, 3))
+    act = None if active is None else np.ascontiguousarray(active, dtype=np.uint8)
+    rc = lib.ds_compute_force(n, _p(x), _p(y), _p(z), _p(m), coll.ctypes.data_as(_U8),
+                              _p(_table(soft)), float(G),
+                              None if act is None else act.ctypes.data_as(_U8), int(t0), int(t1),
+                              1 if skip_self else 0, VARIANTS[variant], int(nthreads), _p(acc))
+    if rc:
+        raise RuntimeError(f"ds_compute_force rc={rc}")
+    return acc
+
+
+def compute_potential(pos, mass, soft, G=1.0, collection=None, *, variant="tuned",
+                      skip_self=False, targets=None, self_term=False, nthreads=0):
+    """a6 (and a7 with self_term=True): Potential[i]."""
+    x, y, z = _soa(pos)
+    n = x.size
+    m = np.ascontiguousarray(mass, dtype=np.float64)
+    coll = np.ones(n, dtype=np.uint8) if collection is None else np.ascontiguousarray(
+        collection, dtype=np.uint8)
+    t0, t1 = (0, n) if targets is None else targets
+    pot = np.zeros(n)
+    tab = _table(soft)
+    rc = lib.ds_compute_potential(n, _p(x), _p(y), _p(z), _p(m), coll.ctypes.data_as(_U8),
+                                  _p(tab), float(G), int(t0), int(t1), 1 if skip_self else 0,
+                                  VARIANTS[variant], int(nthreads), _p(pot))
+    if rc:
+        raise RuntimeError(f"ds_compute_potential rc={rc}")
+    if self_term:
+        lib.ds_remove_self_potential(n, _p(m), coll.ctypes.data_as(_U8), _p(tab), float(G),
+                                     _p(pot))
+    return pot
+
+
+def force_at_points(pts, h, pos, mass, G=1.0, *, variant="tuned", nthreads=0):
+    """a8: probe-point accelerations."""
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+    x, y, z = _soa(pos)
+    m = np.ascontiguousarray(mass, dtype=np.float64)
+    acc = np.zeros_like(pts)
+    rc = lib.ds_force_at_points(pts.shape[0], _p(pts), float(h), x.size, _p(x), _p(y), _p(z),
+                                _p(m), float(G), VARIANTS[variant], int(nthreads), _p(acc))
+    if rc:
+        raise RuntimeError(f"ds_force_at_points rc={rc}")
+    return acc
+
+
+def potential_at_points(pts, h, pos, mass, G=1.0, *, variant="tuned", nthreads=0):
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+    x, y, z = _soa(pos)
+    m = np.ascontiguousarray(mass, dtype=np.float64)
+    pot = np.zeros(pts.shape[0])
+    rc = lib.ds_potential_at_points(pts.shape[0], _p(pts), float(h), x.size, _p(x), _p(y),
+                                    _p(z), _p(m), float(G), VARIANTS[variant], int(nthreads),
+                                    _p(pot))
+    if rc:
+        raise RuntimeError(f"ds_potential_at_points rc={rc}")
+    return pot
+
+
+def background_point_mass(pos, GM, center, acc=None, pot=None, G=1.0):
+    """a10: analytic point-mass field added to acc/pot in place (G*M = G*GM/G)."""
+    x, y, z = _soa(pos)
+    c = np.ascontiguousarray(center, dtype=np.float64)
+    lib.ds_background_point_mass(x.size, _p(x), _p(y), _p(z), float(G), float(GM / G), _p(c),
+                                 _p(acc), _p(pot))
+
+
+def energy(pos, vel, mass, pot=None):
+    """a14: (K, W, P, L) with W = 0.5 * sum(m * phi) as total_potential(sim)."""
+    x, y, z = _soa(pos)
+    v = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3)
+    m = np.ascontiguousarray(mass, dtype=np.float64)
+    p = None if pot is None else np.ascontiguousarray(pot, dtype=np.float64)
+    K, Ws = C.c_double(), C.c_double()
+    P, L = np.zeros(3), np.zeros(3)
+    rc = lib.ds_energy(x.size, _p(x), _p(y), _p(z), _p(v), _p(m), _p(p), C.byref(K), C.byref(Ws),
+                       _p(P), _p(L))
+    if rc:
+        raise RuntimeError(f"ds_energy rc={rc}")
+    return K.value, 0.5 * Ws.value, P, L
+
+
+# ---------------------------------------------------------------------------
+# Constant-timestep run loop: src/run.jl:46-184 with
+#   find_next_sync_point_and_drift (const)  src/base/Timestep.jl:197-215
+#   drift_particles_kernel (const)          src/base/Timestep.jl:181-191
+#   advance_and_find_timestep_local (const) src/base/Timestep.jl:84-93
+#   leapfrog_half_kick_local (const)        src/base/Timestep.jl:116-125
+#   TimeInfo initial values                 src/base/Config.jl:746-766
+# ---------------------------------------------------------------------------
+def count_steps(t_begin: float, t_end: float, dt: float) -> int:
+    """Number of loop iterations of run(): t += dt (IEEE FP64) until t >= End."""
+    t = float(t_begin)
+    k = 0
+    while True:
+        t = t + dt
+        k += 1
+        if t >= t_end:
+            return k
+
+
+def run_const_kdk(pos, vel, mass, soft, G, *, t_begin, t_end, dt, between_snapshots=None,
+                  collection=None, on_output=None, analysers=None, skip_self=False,
+                  variant="tuned", nthreads=0):
+    """Restatement of run(sim) for DirectSum + ConstantTimestep + Leapfrog.
+
+    Returns (pos, vel, acc, info) with info = {"steps", "outputs", "analysis"}.
+    on_output(t, pos, vel) is called at each snapshot event (snapshot 0 after
+    the first half kick, then whenever next_output_time <= t + dt)."""
+    pos = np.array(pos, dtype=np.float64).reshape(-1, 3)
+    vel = np.array(vel, dtype=np.float64).reshape(-1, 3)
+    m = np.ascontiguousarray(mass, dtype=np.float64)
+    n = pos.shape[0]
+    x, y, z = [np.ascontiguousarray(pos[:, k]) for k in range(3)]
+    v = np.ascontiguousarray(vel)
+    if between_snapshots is None:
+        between_snapshots = t_end - t_begin
+
+    def force():
+        return compute_force(np.stack([x, y, z], axis=1), m, soft, G, collection,
+                             variant=variant, skip_self=skip_self, nthreads=nthreads)
+
+    outputs = []
+    analysis = []
+
+    def output(t):
+        outputs.append(t)
+        if on_output is not None:
+            on_output(t, np.stack([x, y, z], axis=1), v.copy())
+
+    system_time = float(t_begin)
+    next_output = float(t_begin) + between_snapshots
+    acc = force()                                   # run.jl:76
+    lib.ds_half_kick(n, _p(v), _p(acc), +1, float(dt))   # run.jl:85-87
+    output(system_time)                             # run.jl:90
+    steps = 0
+    state = True
+    while state:                                    # run.jl:101
+        next_time = system_time + dt                # Timestep.jl:198
+        while next_output <= next_time:             # Timestep.jl:200-210
+            d = next_output - system_time
+            lib.ds_drift(n, _p(x), _p(y), _p(z), _p(v), float(d))
+            system_time = next_output
+            output(system_time)
+            next_output += between_snapshots
+        d = next_time - system_time                 # Timestep.jl:183
+        lib.ds_drift(n, _p(x), _p(y), _p(z), _p(v), float(d))
+        system_time = next_time
+        acc = force()                               # compute_force
+        lib.ds_kick(n, _p(v), _p(acc), float(dt))   # Timestep.jl:84-93
+        if analysers:
+            analysis.append([f(np.stack([x, y, z], axis=1), v, m) for f in analysers])
+        if system_time >= t_end:                    # run.jl:158-160
+            state = False
+        steps += 1
+    lib.ds_half_kick(n, _p(v), _p(acc), -1, float(dt))   # run.jl:171-173
+    return np.stack([x, y, z], axis=1), v.reshape(n, 3), acc, {
+        "steps": steps, "outputs": outputs, "analysis": analysis}
+
+
+def kdk_steps(pos, vel, mass, soft, G, *, nsteps, dt, t_begin=0.0, acc=None, collection=None,
+              skip_self=False, variant="tuned", nthreads=0):
+    """nsteps x {drift by (t+dt)-t, force, kick by dt} (no half kicks): the body
+    of step(sim, DirectSum, CPU)  src/directsum/timestep.jl:43-55."""
+    pos = np.array(pos, dtype=np.float64).reshape(-1, 3)
+    v = np.ascontiguousarray(np.array(vel, dtype=np.float64).reshape(-1, 3))
+    m = np.ascontiguousarray(mass, dtype=np.float64)
+    n = pos.shape[0]
+    x, y, z = [np.ascontiguousarray(pos[:, k]) for k in range(3)]
+    t = float(t_begin)
+    for _ in range(nsteps):
+        t_new = t + dt
+        lib.ds_drift(n, _p(x), _p(y), _p(z), _p(v), float(t_new - t))
+        t = t_new
+        acc = compute_force(np.stack([x, y, z], axis=1), m, soft, G, collection,
+                            variant=variant, skip_self=skip_self, nthreads=nthreads)
+        lib.ds_kick(n, _p(v), _p(acc), float(dt))
+    return np.stack([x, y, z], axis=1), v, acc
+
+
+def half_kick(vel, acc, forward, dt):
+    v = np.ascontiguousarray(np.array(vel, dtype=np.float64))
+    a = np.ascontiguousarray(acc, dtype=np.float64)
+    lib.ds_half_kick(v.shape[0], _p(v), _p(a), int(forward), float(dt))
+    return v
+
+
+def rel_err(a, b):
+    """Per-particle |a-b|/|b| for (n,3) vectors or (n,) scalars."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    if a.ndim == 2:
+        return np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
+    return np.abs(a - b) / np.abs(b)
+
+
+def err_stats(e):
+    e = np.asarray(e)
+    return {"median": float(np.median(e)), "p99": float(np.percentile(e, 99)),
+            "max": float(np.max(e))}

@@ -1,0 +1,346 @@
+"""Pins of the FP64 oracle (oracle/ds_oracle.c) -- CPU only.
+
+The reference ships no golden vectors for this path (its tests are closed
+forms and cross-layout self-consistency, test/StaticTest.jl), so each oracle
+function is pinned by something independent of its own text: closed forms,
+the published Gadget-2 kernel with exact fractions, continuity and symmetry
+invariants, an mpmath brute force, the reference's hand-written leapfrog
+integrator (test/Local-Timestep.jl:54-91), reversibility and dt^2 scaling.
+"""
+import math
+
+import mpmath as mp
+import numpy as np
+import pytest
+
+mp.mp.dps = 40
+
+
+# -- independent formulas -------------------------------------------------------
+def spline_fac_exact(r, m, h):
+    """Gadget-2 cubic-spline force factor with exact fractions (Springel 2005,
+    MNRAS 364, 1105, eq. A2 derivative), mpmath."""
+    r, m, h = mp.mpf(r), mp.mpf(m), mp.mpf(h)
+    if r >= h:
+        return m / r ** 3
+    u = r / h
+    if u < mp.mpf(1) / 2:
+        return m / h ** 3 * (mp.mpf(32) / 3 + u ** 2 * (32 * u - mp.mpf(192) / 5))
+    return m / h ** 3 * (mp.mpf(64) / 3 - 48 * u + mp.mpf(192) / 5 * u ** 2
+                         - mp.mpf(32) / 3 * u ** 3 - mp.mpf(1) / 15 / u ** 3)
+
+
+def spline_pot_exact(r, m, h):
+    r, m, h = mp.mpf(r), mp.mpf(m), mp.mpf(h)
+    if r == 0:
+        return mp.mpf(0)
+    if r >= h:
+        return -m / r
+    u = r / h
+    if u < mp.mpf(1) / 2:
+        wp = -mp.mpf(14) / 5 + u ** 2 * (mp.mpf(16) / 3 + u ** 2 * (mp.mpf(32) / 5 * u - mp.mpf(48) / 5))
+    else:
+        wp = (-mp.mpf(16) / 5 + mp.mpf(1) / 15 / u
+              + u ** 2 * (mp.mpf(32) / 3 + u * (-16 + u * (mp.mpf(48) / 5 - mp.mpf(32) / 15 * u))))
+    return m / h * wp
+
+
+def brute_force_acc(pos, mass, h, G=1.0, skip_self=False):
+    n = len(mass)
+    out = np.zeros((n, 3))
+    for i in range(n):
+        acc = [mp.mpf(0)] * 3
+        for j in range(n):
+            if skip_self and i == j:
+                continue
+            d = [mp.mpf(pos[j][k]) - mp.mpf(pos[i][k]) for k in range(3)]
+            r = mp.sqrt(d[0] ** 2 + d[1] ** 2 + d[2] ** 2)
+            if r == 0:
+                continue
+            f = spline_fac_exact(r, mass[j], h) * G
+            acc = [acc[k] + d[k] * f for k in range(3)]
+        out[i] = [float(a) for a in acc]
+    return out
+
+
+def brute_force_pot(pos, mass, h, G=1.0):
+    n = len(mass)
+    out = np.zeros(n)
+    for i in range(n):
+        s = mp.mpf(0)
+        for j in range(n):
+            d = [mp.mpf(pos[j][k]) - mp.mpf(pos[i][k]) for k in range(3)]
+            r = mp.sqrt(d[0] ** 2 + d[1] ** 2 + d[2] ** 2)
+            s += spline_pot_exact(r, mass[j], h) * G
+        out[i] = float(s)
+    return out
+
+
+# -- a1 / a2 pair functions ------------------------------------------------------
+@pytest.mark.parametrize("r", [0.011, 0.5, 3.0, 1e3])
+def test_pair_force_newtonian_outside_h(orc, r):
+    p = np.zeros(3)
+    s = np.array([r / math.sqrt(3)] * 3)
+    a = orc.pair_force(p, s, 2.5, 1.7, 0.01)
+    expect = 1.7 * 2.5 / r ** 2
+    assert np.linalg.norm(a) == pytest.approx(expect, rel=1e-14)
+    assert np.allclose(a / np.linalg.norm(a), s / np.linalg.norm(s), rtol=1e-14)
+
+
+@pytest.mark.parametrize("u", [1e-3, 0.1, 0.3, 0.4999, 0.5, 0.5001, 0.7, 0.9, 0.9999])
+def test_pair_force_spline_matches_exact_fractions(orc, u):
+    h = 0.02
+    r = u * h
+    s = np.array([r, 0.0, 0.0])
+    a = orc.pair_force(np.zeros(3), s, 1.3, 1.0, h)
+    exact = float(spline_fac_exact(r, 1.3, h)) * r
+    # the reference's 12-digit decimals differ from the exact fractions at ~1e-12
+    assert a[0] == pytest.approx(exact, rel=5e-11)
+    pot = orc.pair_potential(np.zeros(3), s, 1.3, 1.0, h)
+    assert pot == pytest.approx(float(spline_pot_exact(r, 1.3, h)), rel=5e-11)
+
+
+@pytest.mark.parametrize("edge", [0.5, 1.0])
+def test_spline_continuity(orc, edge):
+    h = 0.5
+    lo = orc.pair_force(np.zeros(3), np.array([edge * h * (1 - 1e-13), 0, 0]), 1.0, 1.0, h)[0]
+    hi = orc.pair_force(np.zeros(3), np.array([edge * h * (1 + 1e-13), 0, 0]), 1.0, 1.0, h)[0]
+    assert lo == pytest.approx(hi, rel=1e-10)
+    plo = orc.pair_potential(np.zeros(3), np.array([edge * h * (1 - 1e-13), 0, 0]), 1.0, 1.0, h)
+    phi = orc.pair_potential(np.zeros(3), np.array([edge * h * (1 + 1e-13), 0, 0]), 1.0, 1.0, h)
+    assert plo == pytest.approx(phi, rel=1e-10)
+
+
+def test_self_pair(orc):
+    p = np.array([0.3, -0.2, 0.1])
+    assert np.all(orc.pair_force(p, p, 1.0, 1.0, 0.01) == 0.0)
+    assert orc.pair_potential(p, p, 1.0, 1.0, 0.01) == 0.0
+    # h == 0: the reference computes 0*Inf (src/directsum/gravity.jl:16-27)
+    assert np.all(np.isnan(orc.pair_force(p, p, 1.0, 1.0, 0.0)))
+    assert np.isnan(orc.pair_potential(p, p, 1.0, 1.0, 0.0))
+
+
+def test_pair_antisymmetry(orc):
+    rng = np.random.default_rng(0)
+    for _ in range(50):
+        p, s = rng.normal(size=3), rng.normal(size=3)
+        mi, mj = rng.uniform(0.1, 2, 2)
+        h = rng.uniform(0.01, 3.0)
+        aij = orc.pair_force(p, s, mj, 1.0, h)
+        aji = orc.pair_force(s, p, mi, 1.0, h)
+        assert np.allclose(mi * aij, -mj * aji, rtol=1e-13, atol=0)
+
+
+# -- a3-a6 sums --------------------------------------------------------------------
+def test_sums_match_mpmath_bruteforce(orc):
+    rng = np.random.default_rng(1)
+    n = 24
+    pos = rng.normal(scale=0.05, size=(n, 3))
+    pos[5] = pos[4] + [0.003, 0.0, 0.0]     # a pair inside h
+    pos[7] = pos[6] + [0.0, 0.012, 0.0]     # a pair in the u > 0.5 band
+    mass = rng.uniform(0.5, 1.5, n)
+    h = 0.02
+    ref = brute_force_acc(pos, mass, h, G=1.3)
+    for variant in ("faithful", "tuned", "longdouble", "simd"):
+        acc = orc.compute_force(pos, mass, [h] * 6, 1.3, variant=variant)
+        assert np.allclose(acc, ref, rtol=1e-11, atol=1e-12 * np.abs(ref).max()), variant
+    pref = brute_force_pot(pos, mass, h, G=1.3)
+    for variant in ("faithful", "tuned", "longdouble"):
+        pot = orc.compute_potential(pos, mass, [h] * 6, 1.3, variant=variant)
+        assert np.allclose(pot, pref, rtol=1e-11), variant
+
+
+def test_target_side_softening(orc):
+    """h comes from the target's collection (src/directsum/gravity.jl:97)."""
+    pos = np.array([[0.0, 0.0, 0.0], [0.004, 0.0, 0.0]])
+    mass = np.array([1.0, 1.0])
+    coll = np.array([1, 5], dtype=np.uint8)
+    table = [0.01, 1e-4, 1e-4, 1e-4, 1e-5, 1e-4]
+    acc = orc.compute_force(pos, mass, table, 1.0, coll)
+    a0 = orc.pair_force(pos[0], pos[1], 1.0, 1.0, 0.01)
+    a1 = orc.pair_force(pos[1], pos[0], 1.0, 1.0, 1e-5)
+    assert np.array_equal(acc[0], a0) and np.array_equal(acc[1], a1)
+    assert abs(acc[1, 0]) == pytest.approx(1.0 / 0.004 ** 2, rel=1e-14)   # Newtonian for particle 2
+
+
+def test_variants_agree_plummer(pkg, orc):
+    pos, vel, mass = pkg.ics.plummer(2000, seed=3)
+    ref = orc.compute_force(pos, mass, [0.01] * 6, variant="longdouble")
+    for v in ("faithful", "tuned", "simd"):
+        a = orc.compute_force(pos, mass, [0.01] * 6, variant=v)
+        assert np.max(orc.rel_err(a, ref)) < 1e-12, v
+
+
+def test_staticcase_binary_closed_forms(pkg, orc):
+    """test/StaticTest.jl:18-52,163-207 (unitless: G = 1 here, m = 1e10, D = 2, h = 1e-4)."""
+    G, m, D, h = 1.0, 1.0e10, 2.0, 1e-4
+    pos, vel, mass = pkg.ics.binary(m, D, G)
+    acc = orc.compute_force(pos, mass, [h] * 6, G)
+    binary_force = G * m / (D ** 2 + h ** 2) / math.sqrt(D ** 2 + h ** 2) * D
+    assert acc[0, 0] + acc[1, 0] == pytest.approx(0.0, abs=1e-6)
+    assert abs(acc[0, 0]) == pytest.approx(binary_force, rel=1.5e-8)   # Julia's default rtol sqrt(eps)
+    pot = orc.compute_potential(pos, mass, [h] * 6, G)
+    K, W, P, L = orc.energy(pos, vel, mass, pot)
+    v = math.sqrt(G * m / D / 2.0)
+    assert W == pytest.approx(-G * m ** 2 / math.sqrt(D ** 2 + h ** 2), rel=1.5e-8)
+    assert K == pytest.approx(m * v ** 2, rel=1e-14)
+    assert np.allclose(P, 0.0, atol=1e-6 * m * v)
+    assert np.allclose(L, [0.0, 0.0, m * v * D], rtol=1e-14)
+
+
+def test_staticcase_box_probe(pkg, orc):
+    """test/StaticTest.jl:93-123,333-349: force/potential at the origin."""
+    pos, _, mass = pkg.ics.box10(1e9)
+    h = 1e-4
+    pt = np.zeros((1, 3))
+    a = orc.force_at_points(pt, h, pos, mass, 1.0)[0]
+    ref = np.zeros(3)
+    for j in range(10):
+        r = np.linalg.norm(pos[j])
+        ref += mass[j] * pos[j] / r ** 3
+    assert np.allclose(a, ref, rtol=1e-13)
+    p = orc.potential_at_points(pt, h, pos, mass, 1.0)[0]
+    assert p == pytest.approx(-sum(mass[j] / np.linalg.norm(pos[j]) for j in range(10)), rel=1e-13)
+
+
+def test_momentum_conservation(pkg, orc):
+    pos, vel, mass = pkg.ics.plummer(1500, seed=9, masses=pkg.ics.lognormal_masses(1500, 2))
+    acc = orc.compute_force(pos, mass, [0.02] * 6)
+    net = (mass[:, None] * acc).sum(0)
+    scale = (mass[:, None] * np.abs(acc)).sum(0)
+    assert np.all(np.abs(net) < 1e-13 * scale)
+
+
+def test_plummer_virial_and_potential(pkg, orc):
+    """Continuum Plummer: W = -3 pi G M^2 / (32 a); virial 2K/|W| = 1."""
+    pos, vel, mass = pkg.ics.plummer(4096, seed=1)
+    pot = orc.compute_potential(pos, mass, [0.01] * 6)
+    K, W, _, _ = orc.energy(pos, vel, mass, pot)
+    assert W == pytest.approx(-3 * math.pi / 32, rel=0.05)
+    assert 2 * K / abs(W) == pytest.approx(1.0, rel=0.06)
+
+
+def test_self_potential_term(pkg, orc):
+    """public compute_potential(sim) adds G m_i / (h_i/2.8) (src/base/Potential.jl:5-24)."""
+    pos, _, mass = pkg.ics.plummer(300, seed=4)
+    p0 = orc.compute_potential(pos, mass, [0.05] * 6, 2.0)
+    p1 = orc.compute_potential(pos, mass, [0.05] * 6, 2.0, self_term=True)
+    assert np.allclose(p1 - p0, 2.0 * mass / (0.05 / 2.8), rtol=1e-12)
+
+
+def test_active_filter_and_ranges(pkg, orc):
+    pos, _, mass = pkg.ics.plummer(400, seed=5)
+    full = orc.compute_force(pos, mass, [0.01] * 6)
+    part = orc.compute_force(pos, mass, [0.01] * 6, targets=(100, 250))
+    assert np.array_equal(part[100:250], full[100:250]) and np.all(part[:100] == 0)
+    active = np.zeros(400, dtype=np.uint8)
+    active[::3] = 1
+    sel = orc.compute_force(pos, mass, [0.01] * 6, active=active)
+    assert np.array_equal(sel[::3], full[::3]) and np.all(sel[1::3] == 0)
+
+
+def test_zero_softening_skip_self(pkg, orc):
+    pos, _, mass = pkg.ics.plummer(64, seed=8)
+    acc = orc.compute_force(pos, mass, [0.0] * 6, skip_self=True)
+    assert np.all(np.isfinite(acc))
+    ref = brute_force_acc(pos, mass, 0.0, skip_self=True)
+    assert np.allclose(acc, ref, rtol=1e-12)
+    assert np.all(np.isnan(orc.compute_force(pos, mass, [0.0] * 6, skip_self=False)))
+
+
+# -- a11-a13 integration -------------------------------------------------------------
+def test_step_count_float_accumulation(orc):
+    """run() stops when t += dt reaches End (src/base/Timestep.jl:198, src/run.jl:158)."""
+    assert orc.count_steps(0.0, 0.1, 1e-3) == 100
+    assert orc.count_steps(0.0, 50 * 5e-4, 5e-4) == 50
+    assert orc.count_steps(0.0, 20 * 1e-3, 1e-3) == 20
+    assert orc.count_steps(0.0, 0.2, 1e-3) == 200
+    assert orc.count_steps(0.0, 10.0, 1e-3) == 10001
+
+
+def integrate_const_leapfrog(N, x1, v1, dt, compute_acc):
+    """test/Local-Timestep.jl:54-91, restated (state vectors instead of a DataFrame)."""
+    x = [x1]
+    v = [v1]
+    a = [compute_acc(x1)]
+    v[0] = v[0] + 0.5 * a[0] * dt
+    for i in range(N - 1):
+        x.append(x[i] + v[i] * dt)
+        a.append(compute_acc(x[i + 1]))
+        v.append(v[i] + a[i + 1] * dt)
+    v[-1] = v[-1] - 0.5 * a[-1] * dt
+    return x[-1], v[-1]
+
+
+def test_run_loop_matches_hand_integrator(pkg, orc):
+    """test/Local-Timestep.jl:402-455: run(sim) vs the hand integrator on a
+    circular binary (two-body acceleration from the oracle's own pair
+    function for both)."""
+    G, m, D, h = 1.0, 1.0, 1.0, 1e-4
+    pos, vel, mass = pkg.ics.binary(m, D, G)
+    dt, steps = 1e-3, 500
+    t_end = steps * dt
+
+    def acc2(x):
+        a = np.zeros((2, 3))
+        a[0] = orc.pair_force(x[0], x[1], m, G, h)
+        a[1] = orc.pair_force(x[1], x[0], m, G, h)
+        return a
+
+    nsteps = orc.count_steps(0.0, t_end, dt)
+    xh, vh = integrate_const_leapfrog(nsteps + 1, pos.copy(), vel.copy(), dt, acc2)
+    xr, vr, _, info = orc.run_const_kdk(pos, vel, mass, [h] * 6, G, t_begin=0.0, t_end=t_end,
+                                        dt=dt, between_snapshots=t_end)
+    assert info["steps"] == nsteps
+    assert np.allclose(xr, xh, rtol=0, atol=1e-12)
+    assert np.allclose(vr, vh, rtol=0, atol=1e-12)
+
+
+def test_time_reversibility(pkg, orc):
+    pos, vel, mass = pkg.ics.plummer(200, seed=12)
+    soft = [0.05] * 6
+    dt = 1e-3
+    acc0 = orc.compute_force(pos, mass, soft)
+    v = orc.half_kick(vel, acc0, +1, dt)
+    p1, v1, a1 = orc.kdk_steps(pos, v, mass, soft, 1.0, nsteps=40, dt=dt)
+    v1 = orc.half_kick(v1, a1, -1, dt)
+    v1 = -v1
+    v = orc.half_kick(v1, a1, +1, dt)
+    p2, v2, a2 = orc.kdk_steps(p1, v, mass, soft, 1.0, nsteps=40, dt=dt)
+    v2 = orc.half_kick(v2, a2, -1, dt)
+    assert np.max(np.abs(p2 - pos)) < 1e-11
+    assert np.max(np.abs(-v2 - vel)) < 1e-10
+
+
+def test_energy_error_scales_dt2(pkg, orc):
+    pos, vel, mass = pkg.ics.binary(1.0, 1.0, 1.0)
+    vel = vel * 0.8   # eccentric orbit
+    soft = [1e-4] * 6
+
+    def energy_err(dt, t_end):
+        e0 = orc.energy(pos, vel, mass, orc.compute_potential(pos, mass, soft))
+        xr, vr, _, _ = orc.run_const_kdk(pos, vel, mass, soft, 1.0, t_begin=0.0, t_end=t_end,
+                                         dt=dt, between_snapshots=t_end)
+        e1 = orc.energy(xr, vr, mass, orc.compute_potential(xr, mass, soft))
+        E0, E1 = e0[0] + e0[1], e1[0] + e1[1]
+        return abs(E1 - E0) / abs(E0)
+
+    e1 = energy_err(4e-3, 0.6)
+    e2 = energy_err(2e-3, 0.6)
+    assert 2.5 < e1 / e2 < 6.0
+
+
+def test_snapshot_split_drift(pkg, orc):
+    """Snapshots inside a step split the drift; the total drift equals the
+    un-split one (CPU semantics, src/base/Timestep.jl:181-215)."""
+    pos, vel, mass = pkg.ics.plummer(100, seed=2)
+    soft = [0.05] * 6
+    times = []
+    a, b, _, info = orc.run_const_kdk(pos, vel, mass, soft, 1.0, t_begin=0.0, t_end=0.02,
+                                      dt=3e-3, between_snapshots=0.0045,
+                                      on_output=lambda t, p, v: times.append(t))
+    c, d, _, info2 = orc.run_const_kdk(pos, vel, mass, soft, 1.0, t_begin=0.0, t_end=0.02,
+                                       dt=3e-3, between_snapshots=1.0)
+    assert info["steps"] == info2["steps"] == orc.count_steps(0.0, 0.02, 3e-3)
+    assert times[0] == 0.0 and len(times) == 1 + 4
+    assert np.allclose(a, c, atol=1e-14) and np.allclose(b, d, atol=1e-13)
