@@ -12,6 +12,7 @@ from .simulation import (
     total_energy, total_kinetic, total_momentum, total_potential,
 )
 from . import ics
+from .ensemble import Ensemble
 
 __all__ = [
     "NBError", "device_info", "version", "AllPairs", "Tiled", "CPU", "GPU", "DirectSum",
@@ -20,5 +21,5 @@ __all__ = [
     "total_potential", "total_energy", "total_momentum", "total_angular_momentum",
     "get_all_data", "get_local_data", "softlen", "set_softlen", "set_softlen_",
     "init_timesteps", "leapfrog_half_kick", "find_next_sync_point_and_drift",
-    "advance_and_find_timestep", "ics",
+    "advance_and_find_timestep", "ics", "Ensemble",
 ]

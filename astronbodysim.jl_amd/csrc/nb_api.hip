@@ -152,14 +152,23 @@ __global__ void k_gather_central(const double *__restrict__ x, const double *__r
 }
 
 // The all-pairs evaluation for every particle (acc, and potential when asked).
-int run_force(nb_ctx *c, bool with_pot, bool allow_timing) {
+// Re-sort into Morton order when the order is stale.
+int maybe_sort(nb_ctx *c, int64_t upcoming) {
+    if (c->sort.interval <= 0 || c->n < 2) return NB_OK;
+    if (!c->sort.valid || c->calls_since_sort + upcoming > c->sort.interval) {
+        NB_TRY(morton_reorder(c));
+        c->calls_since_sort = 0;
+    }
+    return NB_OK;
+}
+
+int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = true) {
     hipStream_t s = c->stream;
     const int64_t n = c->n;
     if (n == 0) return NB_OK;
-    if (c->sort.interval > 0 && c->force_calls % c->sort.interval == 0 && c->mode != NB_MODE_FP64) {
-        NB_TRY(morton_reorder(c));
-    }
+    if (allow_sort) NB_TRY(maybe_sort(c, 1));
     c->force_calls++;
+    c->calls_since_sort++;
     NB_TRY(launch_tile_boxes(c->x, c->y, c->z, c->is_double, n, c->tile_box, s));
     const bool f64 = c->mode == NB_MODE_FP64;
     const int tpt = f64 ? 1 : c->tun.targets_per_thread;
@@ -364,7 +373,7 @@ int nb_create(nb_ctx **out, int64_t n, int32_t precision_mode) {
         }
         c->tun.targets_per_thread = default_tpt(n);
         c->tun.j_segments = default_segments(n, c->tun.targets_per_thread);
-        c->tun.sort_interval = 0;
+        c->tun.sort_interval = precision_mode == NB_MODE_FP64 ? 0 : 16;
         tuner_lookup(c->mode, n, &c->tun);
         c->sort.interval = c->tun.sort_interval;
     } while (0);
@@ -466,6 +475,7 @@ int nb_upload(nb_ctx *c, const double *pos, const double *vel, const double *mas
     }
     c->uploaded = true;
     c->force_calls = 0;
+    c->calls_since_sort = 0;
     NB_TRY(rebuild_central(c));
     NB_HIP(hipStreamSynchronize(s));
     return NB_OK;
@@ -512,12 +522,13 @@ int nb_compute_potential(nb_ctx *c, int32_t add_self_term) {
     // kernel; keep the caller's accelerations by saving them around it.
     hipStream_t s = c->stream;
     const size_t nb3 = sizeof(double) * (size_t)c->n;
+    NB_TRY(maybe_sort(c, 1));   // reorder first, so the saved accelerations stay aligned
     if (c->n > 0) {
         NB_HIP(hipMemcpyAsync(c->stage, c->ax, nb3, hipMemcpyDeviceToDevice, s));
         NB_HIP(hipMemcpyAsync(c->stage + c->n, c->ay, nb3, hipMemcpyDeviceToDevice, s));
         NB_HIP(hipMemcpyAsync(c->stage + 2 * c->n, c->az, nb3, hipMemcpyDeviceToDevice, s));
     }
-    NB_TRY(run_force(c, true, false));
+    NB_TRY(run_force(c, true, false, false));
     if (c->n > 0) {
         NB_HIP(hipMemcpyAsync(c->ax, c->stage, nb3, hipMemcpyDeviceToDevice, s));
         NB_HIP(hipMemcpyAsync(c->ay, c->stage + c->n, nb3, hipMemcpyDeviceToDevice, s));
@@ -648,11 +659,11 @@ int nb_kick(nb_ctx *c, double dt) {
     return NB_OK;
 }
 
-static int enqueue_step(nb_ctx *c, int64_t k, double dt, bool allow_timing) {
+static int enqueue_step(nb_ctx *c, int64_t k, double dt, bool allow_timing, bool allow_sort) {
     const int rd = (int)(k & 1), wr = 1 - rd;
     NB_TRY(drift(c->x, c->y, c->z, c->vx, c->vy, c->vz, c->n, dt, c->time, rd, wr, c->is_double,
                  c->stream));
-    NB_TRY(run_force(c, false, allow_timing));
+    NB_TRY(run_force(c, false, allow_timing, allow_sort));
     NB_TRY(kick(c->vx, c->vy, c->vz, c->ax, c->ay, c->az, c->n, dt, c->is_double, c->stream));
     return NB_OK;
 }
@@ -665,8 +676,9 @@ int nb_kdk_steps(nb_ctx *c, int64_t nsteps, double dt, double t_begin, int32_t u
     }
     hipStream_t s = c->stream;
     NB_TRY(set_time(c->time, 0, t_begin, s));
-    // Graph replay needs a static schedule: no re-sorting inside the graph.
-    const bool graph_ok = use_graph && c->sort.interval == 0 && nsteps >= 2;
+    // Graph replay needs a static schedule: re-sorting happens between
+    // graph launches, never inside one.
+    const bool graph_ok = use_graph && nsteps >= 2;
     int64_t k = 0;
     if (graph_ok) {
         const int64_t chunk = std::min<int64_t>(nsteps & ~(int64_t)1, 16);
@@ -681,7 +693,8 @@ int nb_kdk_steps(nb_ctx *c, int64_t nsteps, double dt, double t_begin, int32_t u
             NB_TRY(ensure_part(c, g.nseg));
             NB_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             int rc = NB_OK;
-            for (int64_t q = 0; q < chunk && rc == NB_OK; ++q) rc = enqueue_step(c, q, dt, false);
+            for (int64_t q = 0; q < chunk && rc == NB_OK; ++q)
+                rc = enqueue_step(c, q, dt, false, false);
             hipGraph_t graph = nullptr;
             hipError_t e = hipStreamEndCapture(s, &graph);
             if (rc != NB_OK) {
@@ -706,13 +719,16 @@ int nb_kdk_steps(nb_ctx *c, int64_t nsteps, double dt, double t_begin, int32_t u
             c->g_nbg = c->bg.size();
             // Force calls made inside the capture were counted; undo.
             c->force_calls -= chunk;
+            c->calls_since_sort -= chunk;
         }
         for (; k + c->g_steps <= nsteps; k += c->g_steps) {
+            NB_TRY(maybe_sort(c, c->g_steps));
             NB_HIP(hipGraphLaunch(c->gexec, s));
             c->force_calls += c->g_steps;
+            c->calls_since_sort += c->g_steps;
         }
     }
-    for (; k < nsteps; ++k) NB_TRY(enqueue_step(c, k, dt, true));
+    for (; k < nsteps; ++k) NB_TRY(enqueue_step(c, k, dt, true, true));
     return NB_OK;
 }
 

@@ -1,0 +1,1237 @@
+// nb_batch.hip -- many independent systems of uneven size on one GPU.
+//
+// Two execution paths, both following the one-system semantics per system
+// (src/directsum/gravity.jl:8-28 pair law with the system's softening and G;
+// constant-dt KDK as src/directsumgpu/timestep.jl:156-224):
+//
+//  * ragged path: one persistent launch of the all-pairs kernel serves every
+//    system.  The work items are (system, tile of 256*TPT targets), ordered by
+//    cost (largest systems first) and pulled from an atomic queue; each item
+//    is the same block routine as the one-system kernel (force_block), with
+//    the system's own source range, per-system Morton order, FP64 origin and
+//    hi/lo near tiles.  KDK steps are captured in HIP graphs.
+//  * resident path (every system <= 256 particles): one block per system
+//    keeps its system in registers/LDS and runs many KDK steps in a single
+//    launch; central bodies use FP64 pair terms, the other massive bodies FP32
+//    hi/lo terms relative to the system's first particle.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <string>
+#include <vector>
+
+#include "nb_batch.h"
+#include "nb_block.h"
+
+namespace nb {
+
+static int launch_ok(const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("launch of %s failed: %s", what, hipGetErrorString(e));
+        return NB_ERR_HIP;
+    }
+    return NB_OK;
+}
+
+static inline unsigned blocks_for(int64_t n, int bs = 256) {
+    return (unsigned)((n + bs - 1) / bs);
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BS) void k_batch_tile_boxes(const double *__restrict__ x,
+                                                        const double *__restrict__ y,
+                                                        const double *__restrict__ z,
+                                                        const int64_t *__restrict__ off,
+                                                        const int64_t *__restrict__ tile_base,
+                                                        const int32_t *__restrict__ tile_sys,
+                                                        double *__restrict__ box) {
+    __shared__ double red[6][NW];
+    const int64_t t = blockIdx.x;
+    const int s = tile_sys[t];
+    const int64_t start = off[s] + (t - tile_base[s]) * BS;
+    const int64_t end = off[s + 1];
+    const int64_t j = start + threadIdx.x;
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    if (j < end) {
+        lo[0] = hi[0] = x[j];
+        lo[1] = hi[1] = y[j];
+        lo[2] = hi[2] = z[j];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = wave_min(lo[d]);
+        hi[d] = wave_max(hi[d]);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) { red[d][w] = lo[d]; red[3 + d][w] = hi[d]; }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int q = threadIdx.x;
+        double r = red[q][0];
+        for (int ww = 1; ww < NW; ++ww) r = q < 3 ? fmin(r, red[q][ww]) : fmax(r, red[q][ww]);
+        box[t * 8 + q] = r;
+    }
+}
+
+struct BatchArgs {
+    const int64_t *off;
+    const int64_t *tile_base;
+    const double *tile_box;
+    const double *Gs;
+    const double *x, *y, *z, *m, *h;
+    const uint8_t *skip_src;
+    double *ax, *ay, *az, *pot;
+    const int64_t *items;
+    int64_t nitems;
+    unsigned *counter;
+};
+
+// Persistent ragged all-pairs kernel: blocks pull (system, target tile) items
+// from an atomic queue until it is empty; the last block to finish resets the
+// queue for the next launch (so the launch can be replayed from a graph).
+template <int TPT, bool POT>
+__global__ __launch_bounds__(BS) void k_batch_force(BatchArgs a) {
+    __shared__ BlockShared<double> sh;
+    __shared__ int64_t item_sh;
+    const int tid = threadIdx.x;
+    for (;;) {
+        if (tid == 0) {
+            const unsigned q = atomicAdd(a.counter, 1u);
+            item_sh = (int64_t)q < a.nitems ? a.items[q] : -1;
+        }
+        __syncthreads();
+        const int64_t item = item_sh;
+        if (item < 0) break;
+        const int s = (int)(item >> 32);
+        const int64_t k = item & 0xffffffffll;
+        const int64_t o0 = a.off[s], o1 = a.off[s + 1];
+        const int64_t i0 = o0 + k * (BS * TPT);
+        double Ax[TPT], Ay[TPT], Az[TPT], Aq[TPT];
+        bool valid[TPT];
+        // force_block starts with a barrier, so item_sh is read by every
+        // thread before thread 0 overwrites it in the next iteration.
+        force_block<double, TPT, POT>(sh, a.x, a.y, a.z, a.h, 0.0, i0, o1, a.x, a.y, a.z, a.m, o0,
+                                      o1, a.tile_box + a.tile_base[s] * 8, a.skip_src, true, Ax,
+                                      Ay, Az, Aq, valid);
+        const double G = a.Gs[s];
+#pragma unroll
+        for (int q = 0; q < TPT; ++q) {
+            if (!valid[q]) continue;
+            const int64_t i = i0 + q * BS + tid;
+            a.ax[i] = G * Ax[q];
+            a.ay[i] = G * Ay[q];
+            a.az[i] = G * Az[q];
+            if (POT) a.pot[i] = -G * Aq[q];
+        }
+    }
+    if (tid == 0) {
+        __threadfence();
+        const unsigned done = atomicAdd(a.counter + 1, 1u);
+        if (done == gridDim.x - 1) {
+            atomicExch(a.counter, 0u);
+            atomicExch(a.counter + 1, 0u);
+        }
+    }
+}
+
+// FP64 pair arithmetic, one item per block (correctness mode).
+template <bool POT>
+__global__ __launch_bounds__(BS) void k_batch_force_f64(BatchArgs a) {
+    __shared__ double4 sj[BS];
+    const int64_t item = a.items[blockIdx.x];
+    const int s = (int)(item >> 32);
+    const int64_t k = item & 0xffffffffll;
+    const int64_t o0 = a.off[s], o1 = a.off[s + 1];
+    const int64_t i = o0 + k * BS + threadIdx.x;
+    const bool valid = i < o1;
+    const double xi = valid ? a.x[i] : 0.0, yi = valid ? a.y[i] : 0.0, zi = valid ? a.z[i] : 0.0;
+    const double h = valid ? a.h[i] : 1.0;
+    const double hinv = 1.0 / h, hinv3 = hinv * hinv * hinv;
+    double Ax = 0.0, Ay = 0.0, Az = 0.0, Aq = 0.0;
+    for (int64_t base = o0; base < o1; base += BS) {
+        const int64_t j = base + threadIdx.x;
+        __syncthreads();
+        sj[threadIdx.x] = j < o1 ? make_double4(a.x[j], a.y[j], a.z[j], a.m[j])
+                                 : make_double4(1e150, 1e150, 1e150, 0.0);
+        __syncthreads();
+        const int cnt = (o1 - base < BS) ? (int)(o1 - base) : BS;
+        for (int jj = 0; jj < cnt; ++jj) {
+            const double4 v = sj[jj];
+            pair_f64<POT>(v.x, v.y, v.z, v.w, xi, yi, zi, h, hinv, hinv3, Ax, Ay, Az, Aq);
+        }
+    }
+    if (!valid) return;
+    const double G = a.Gs[s];
+    a.ax[i] = G * Ax;
+    a.ay[i] = G * Ay;
+    a.az[i] = G * Az;
+    if (POT) a.pot[i] = -G * Aq;
+}
+
+// FP64 terms of the central bodies of each system, added to acc (and pot).
+template <bool POT>
+__global__ void k_batch_central(const int32_t *__restrict__ sys_of, const int64_t *__restrict__ cidx,
+                                int ncentral, const double *__restrict__ Gs,
+                                const double *__restrict__ x, const double *__restrict__ y,
+                                const double *__restrict__ z, const double *__restrict__ m,
+                                const double *__restrict__ h, int64_t n, double *__restrict__ ax,
+                                double *__restrict__ ay, double *__restrict__ az,
+                                double *__restrict__ pot) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = sys_of[i];
+    const double xi = x[i], yi = y[i], zi = z[i], hi = h[i];
+    const double hinv = 1.0 / hi, hinv3 = hinv * hinv * hinv;
+    double Ax = 0.0, Ay = 0.0, Az = 0.0, Aq = 0.0;
+    for (int c = 0; c < ncentral; ++c) {
+        const int64_t j = cidx[(int64_t)s * ncentral + c];
+        if (j == i) continue;
+        pair_f64<POT>(x[j], y[j], z[j], m[j], xi, yi, zi, hi, hinv, hinv3, Ax, Ay, Az, Aq);
+    }
+    const double G = Gs[s];
+    ax[i] += G * Ax;
+    ay[i] += G * Ay;
+    az[i] += G * Az;
+    if (POT) pot[i] -= G * Aq;
+}
+
+// v += a * coef * dt_s ; optionally then x += v * dt_s (the next step's drift)
+__global__ void k_batch_kick_drift(double *__restrict__ x, double *__restrict__ y,
+                                   double *__restrict__ z, double *__restrict__ vx,
+                                   double *__restrict__ vy, double *__restrict__ vz,
+                                   const double *__restrict__ ax, const double *__restrict__ ay,
+                                   const double *__restrict__ az,
+                                   const int32_t *__restrict__ sys_of,
+                                   const double *__restrict__ dts, int64_t n, double kick_coef,
+                                   int do_drift) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double dt = dts[sys_of[i]];
+    double VX = vx[i], VY = vy[i], VZ = vz[i];
+    if (kick_coef != 0.0) {
+        const double c = kick_coef * dt;
+        VX += ax[i] * c;
+        VY += ay[i] * c;
+        VZ += az[i] * c;
+        vx[i] = VX;
+        vy[i] = VY;
+        vz[i] = VZ;
+    }
+    if (do_drift) {
+        x[i] += VX * dt;
+        y[i] += VY * dt;
+        z[i] += VZ * dt;
+    }
+}
+
+// Per-system FP64 reductions, one block per system: K, sum m phi, P, L, and
+// (sync != 0) the kinetic energy of velocities synchronised to the step,
+// v - a dt/2, in slot 0 instead.
+__global__ __launch_bounds__(256) void k_batch_energy(
+    const int64_t *__restrict__ off, const double *__restrict__ x, const double *__restrict__ y,
+    const double *__restrict__ z, const double *__restrict__ vx, const double *__restrict__ vy,
+    const double *__restrict__ vz, const double *__restrict__ ax, const double *__restrict__ ay,
+    const double *__restrict__ az, const double *__restrict__ m, const double *__restrict__ pot,
+    const double *__restrict__ dts, int sync, double *__restrict__ out) {
+    __shared__ double red[8][4];
+    const int s = blockIdx.x;
+    const int64_t o0 = off[s], o1 = off[s + 1];
+    const double half = sync ? 0.5 * dts[s] : 0.0;
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t i = o0 + threadIdx.x; i < o1; i += 256) {
+        const double mi = m[i];
+        const double VX = vx[i] - half * ax[i], VY = vy[i] - half * ay[i], VZ = vz[i] - half * az[i];
+        v[0] += 0.5 * mi * (VX * VX + VY * VY + VZ * VZ);
+        if (pot) v[1] += pot[i] * mi;
+        v[2] += VX * mi;
+        v[3] += VY * mi;
+        v[4] += VZ * mi;
+        v[5] += mi * (y[i] * VZ - z[i] * VY);
+        v[6] += mi * (z[i] * VX - x[i] * VZ);
+        v[7] += mi * (x[i] * VY - y[i] * VX);
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = wave_sum(v[q]);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) red[q][w] = v[q];
+    }
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        const int q = threadIdx.x;
+        out[(int64_t)s * 8 + q] = red[q][0] + red[q][1] + red[q][2] + red[q][3];
+    }
+}
+
+// ---- resident multi-step kernel (systems of <= 256 particles) ------------
+struct ResArgs {
+    const int64_t *off;
+    const double *Gs, *hs, *dts;
+    double *x, *y, *z, *vx, *vy, *vz;
+    const double *m;
+    double *ax, *ay, *az;
+    int ncentral;
+    double *energy;      // [nsys][slots] or nullptr
+    int energy_every;
+    int energy_slots;
+};
+
+template <bool ALL_F64>
+__global__ __launch_bounds__(256) void k_batch_resident(ResArgs a, int nsteps) {
+    __shared__ double sx[256], sy[256], sz[256], sm[256];
+    __shared__ float4 shi[256], slo[256];
+    __shared__ int sidx[256];
+    __shared__ int wave_cnt[4];
+    __shared__ double red[4];
+    const int s = blockIdx.x;
+    const int t = threadIdx.x;
+    const int64_t o0 = a.off[s], o1 = a.off[s + 1];
+    const int ns = (int)(o1 - o0);
+    const bool act = t < ns;
+    const int64_t i = o0 + t;
+    double X = 0, Y = 0, Z = 0, VX = 0, VY = 0, VZ = 0, AX = 0, AY = 0, AZ = 0, M = 0;
+    if (act) {
+        X = a.x[i]; Y = a.y[i]; Z = a.z[i];
+        VX = a.vx[i]; VY = a.vy[i]; VZ = a.vz[i];
+        AX = a.ax[i]; AY = a.ay[i]; AZ = a.az[i];
+        M = a.m[i];
+    }
+    const double G = a.Gs[s], dt = a.dts[s], hsys = a.hs[s];
+    const double hinv = 1.0 / hsys, hinv3 = hinv * hinv * hinv;
+    const float hf = (float)hsys, h2f = hf * hf, h2cf = fmaxf(h2f, FLT_MIN);
+    const float hinvf = 1.0f / hf, hinv3f = hinvf * hinvf * hinvf;
+    // compact the massive sources (deterministic: local index order)
+    const bool massive = act && M != 0.0;
+    const unsigned long long bal = __ballot(massive);
+    const int lane = t & 63, w = t >> 6;
+    const int before = __popcll(bal & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+    if (lane == 0) wave_cnt[w] = __popcll(bal);
+    __syncthreads();
+    int wbase = 0;
+    for (int q = 0; q < w; ++q) wbase += wave_cnt[q];
+    const int nmass = wave_cnt[0] + wave_cnt[1] + wave_cnt[2] + wave_cnt[3];
+    const int slot = wbase + before;
+    if (massive) {
+        sidx[slot] = t;
+        sm[slot] = M;
+    }
+    int e_slot = 0;
+    for (int step = 0; step < nsteps; ++step) {
+        // drift (FP64)
+        X += VX * dt; Y += VY * dt; Z += VZ * dt;
+        __syncthreads();   // previous step's readers of the source arrays are done
+        if (massive) { sx[slot] = X; sy[slot] = Y; sz[slot] = Z; }
+        __syncthreads();
+        // origin: the system's first particle (the central body of a planetary system)
+        const double ox = nmass > 0 ? sx[0] : 0.0;
+        const double oy = nmass > 0 ? sy[0] : 0.0, oz = nmass > 0 ? sz[0] : 0.0;
+        if (!ALL_F64 && massive) {
+            const double dx = X - ox, dy = Y - oy, dz = Z - oz;
+            const float hx = (float)dx, hy = (float)dy, hz = (float)dz;
+            shi[slot] = make_float4(hx, hy, hz, (float)M);
+            slo[slot] = make_float4((float)(dx - (double)hx), (float)(dy - (double)hy),
+                                    (float)(dz - (double)hz), 0.0f);
+        }
+        __syncthreads();
+        double Ax = 0.0, Ay = 0.0, Az = 0.0, Aq = 0.0;
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f, fq = 0.0f;
+        if (act) {
+            const double dxi = X - ox, dyi = Y - oy, dzi = Z - oz;
+            const float xih = (float)dxi, yih = (float)dyi, zih = (float)dzi;
+            const float xil = (float)(dxi - (double)xih), yil = (float)(dyi - (double)yih),
+                        zil = (float)(dzi - (double)zih);
+            for (int k = 0; k < nmass; ++k) {
+                const int j = sidx[k];
+                if (j == t) continue;
+                if (ALL_F64 || j < a.ncentral) {
+                    pair_f64<false>(sx[k], sy[k], sz[k], sm[k], X, Y, Z, hsys, hinv, hinv3, Ax, Ay,
+                                    Az, Aq);
+                } else {
+                    const float4 v = shi[k], l = slo[k];
+                    pair_f32_hilo<false>(v.x, v.y, v.z, v.w, l.x, l.y, l.z, xih, yih, zih, xil,
+                                         yil, zil, h2cf, h2f, hinvf, hinv3f, fx, fy, fz, fq);
+                }
+            }
+        }
+        AX = G * (Ax + (double)fx);
+        AY = G * (Ay + (double)fy);
+        AZ = G * (Az + (double)fz);
+        // kick (FP64)
+        VX += AX * dt; VY += AY * dt; VZ += AZ * dt;
+        if (a.energy && (step + 1) % a.energy_every == 0) {
+            // total energy with velocities synchronised to the step
+            double e = 0.0;
+            if (act) {
+                const double ux = VX - 0.5 * AX * dt, uy = VY - 0.5 * AY * dt, uz = VZ - 0.5 * AZ * dt;
+                e = 0.5 * M * (ux * ux + uy * uy + uz * uz);
+                if (M != 0.0) {
+                    double bx = 0, by = 0, bz = 0, q = 0.0;
+                    for (int k = 0; k < nmass; ++k) {
+                        if (sidx[k] == t) continue;
+                        pair_f64<true>(sx[k], sy[k], sz[k], sm[k], X, Y, Z, hsys, hinv, hinv3, bx,
+                                       by, bz, q);
+                    }
+                    e -= 0.5 * G * M * q;
+                }
+            }
+            e = wave_sum(e);
+            if (lane == 0) red[w] = e;
+            __syncthreads();
+            if (t == 0 && e_slot < a.energy_slots)
+                a.energy[(int64_t)s * a.energy_slots + e_slot] = red[0] + red[1] + red[2] + red[3];
+            ++e_slot;
+        }
+    }
+    if (act) {
+        a.x[i] = X; a.y[i] = Y; a.z[i] = Z;
+        a.vx[i] = VX; a.vy[i] = VY; a.vz[i] = VZ;
+        a.ax[i] = AX; a.ay[i] = AY; a.az[i] = AZ;
+    }
+}
+
+// ---- Morton order within systems -----------------------------------------
+__global__ __launch_bounds__(256) void k_sys_box(const int64_t *__restrict__ off,
+                                                 const double *__restrict__ x,
+                                                 const double *__restrict__ y,
+                                                 const double *__restrict__ z,
+                                                 double *__restrict__ box) {
+    __shared__ double red[6][4];
+    const int s = blockIdx.x;
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int64_t i = off[s] + threadIdx.x; i < off[s + 1]; i += 256) {
+        lo[0] = fmin(lo[0], x[i]); hi[0] = fmax(hi[0], x[i]);
+        lo[1] = fmin(lo[1], y[i]); hi[1] = fmax(hi[1], y[i]);
+        lo[2] = fmin(lo[2], z[i]); hi[2] = fmax(hi[2], z[i]);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = wave_min(lo[d]);
+        hi[d] = wave_max(hi[d]);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) { red[d][w] = lo[d]; red[3 + d][w] = hi[d]; }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int q = threadIdx.x;
+        double r = red[q][0];
+        for (int ww = 1; ww < 4; ++ww) r = q < 3 ? fmin(r, red[q][ww]) : fmax(r, red[q][ww]);
+        box[(int64_t)s * 8 + q] = r;
+    }
+}
+
+__device__ __forceinline__ uint64_t spread17(uint64_t v) {
+    v &= 0x1ffff;
+    v = (v | (v << 32)) & 0x1f00000000ffffULL;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffULL;
+    v = (v | (v << 8)) & 0x100f00f00f00f00fULL;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ULL;
+    v = (v | (v << 2)) & 0x1249249249249249ULL;
+    return v;
+}
+
+__global__ void k_batch_morton(const int32_t *__restrict__ sys_of, const double *__restrict__ x,
+                               const double *__restrict__ y, const double *__restrict__ z,
+                               const double *__restrict__ box, int64_t n,
+                               uint64_t *__restrict__ key, int64_t *__restrict__ idx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = sys_of[i];
+    const double *b = box + (int64_t)s * 8;
+    const double p[3] = {x[i], y[i], z[i]};
+    uint64_t c[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double ext = b[3 + d] - b[d];
+        double u = ext > 0.0 ? (p[d] - b[d]) / ext : 0.0;
+        u = fmin(fmax(u, 0.0), 1.0);
+        c[d] = (uint64_t)(u * 131071.0);
+    }
+    key[i] = ((uint64_t)s << 51) | spread17(c[0]) | (spread17(c[1]) << 1) | (spread17(c[2]) << 2);
+    idx[i] = i;
+}
+
+template <typename V>
+__global__ void k_bgather(const V *__restrict__ src, const int64_t *__restrict__ idx, int64_t n,
+                          V *__restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dst[i] = src[idx[i]];
+}
+
+__global__ void k_bcompose(const int64_t *__restrict__ old_perm, const int64_t *__restrict__ idx,
+                           int64_t n, int64_t *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = old_perm ? old_perm[idx[i]] : idx[i];
+}
+
+__global__ void k_central_list(const int64_t *__restrict__ perm, const int32_t *__restrict__ sys_of,
+                               const int64_t *__restrict__ off, int64_t n, int ncentral,
+                               uint8_t *__restrict__ flags, int64_t *__restrict__ cidx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t u = perm ? perm[i] : i;   // caller index
+    const int s = sys_of[i];
+    const int64_t local = u - off[s];
+    const bool c = local < ncentral;
+    flags[i] = c ? 1 : 0;
+    if (c) cidx[(int64_t)s * ncentral + local] = i;
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+static int bcheck(nb_batch *b, bool need_upload = true) {
+    clear_error();
+    if (!b) {
+        set_error("null batch");
+        return NB_ERR_INVALID;
+    }
+    hipError_t e = hipSetDevice(b->dev);
+    if (e != hipSuccess) {
+        set_error("hipSetDevice failed: %s", hipGetErrorString(e));
+        return NB_ERR_HIP;
+    }
+    if (need_upload && !b->uploaded) {
+        set_error("no particle data uploaded (call nb_batch_upload first)");
+        return NB_ERR_STATE;
+    }
+    return NB_OK;
+}
+
+static void bdrop_graph(nb_batch *b) {
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    b->gexec = nullptr;
+    b->g_steps = 0;
+}
+
+static bool resident_ok(const nb_batch *b) {
+    return b->tun.resident && b->max_ns <= 256 && b->nsys > 0;
+}
+
+// (system, target tile) items, the most expensive (largest system) first.
+static int build_items(nb_batch *b, int tpt) {
+    if (b->items && b->items_tpt == tpt) return NB_OK;
+    const int64_t ti = (int64_t)BS * tpt;
+    std::vector<int32_t> order(b->nsys);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t p, int32_t q) {
+        return b->h_off[p + 1] - b->h_off[p] > b->h_off[q + 1] - b->h_off[q];
+    });
+    std::vector<int64_t> items;
+    for (int32_t s : order) {
+        const int64_t ns = b->h_off[s + 1] - b->h_off[s];
+        for (int64_t k = 0; k * ti < ns; ++k) items.push_back(((int64_t)s << 32) | k);
+    }
+    dfree(b->items);
+    b->nitems = (int64_t)items.size();
+    NB_TRY(dalloc(&b->items, sizeof(int64_t) * std::max<size_t>(items.size(), 1)));
+    if (!items.empty())
+        NB_HIP(hipMemcpy(b->items, items.data(), sizeof(int64_t) * items.size(),
+                         hipMemcpyHostToDevice));
+    b->items_tpt = tpt;
+    return NB_OK;
+}
+
+static int bpermute(double *arr, const int64_t *idx, int64_t n, void *spare, hipStream_t s) {
+    k_bgather<double><<<blocks_for(n), 256, 0, s>>>(arr, idx, n, (double *)spare);
+    NB_TRY(launch_ok("k_bgather"));
+    NB_HIP(hipMemcpyAsync(arr, spare, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
+    return NB_OK;
+}
+
+static int rebuild_central(nb_batch *b) {
+    if (b->ncentral <= 0 || b->n == 0) return NB_OK;
+    k_central_list<<<blocks_for(b->n), 256, 0, b->stream>>>(b->perm, b->sys_of, b->off, b->n,
+                                                            b->ncentral, b->central, b->cidx);
+    return launch_ok("k_central_list");
+}
+
+static int batch_sort(nb_batch *b) {
+    const int64_t n = b->n;
+    if (n < 2) return NB_OK;
+    hipStream_t s = b->stream;
+    if (!b->keys) {
+        NB_TRY(dalloc(&b->keys, sizeof(uint64_t) * 2 * (size_t)n));
+        NB_TRY(dalloc(&b->idx, sizeof(int64_t) * 2 * (size_t)n));
+        NB_TRY(dalloc(&b->sysbox, sizeof(double) * 8 * (size_t)b->nsys));
+        NB_TRY(dalloc(&b->spare, sizeof(double) * (size_t)n));
+        size_t bytes = 0;
+        NB_HIP(rocprim::radix_sort_pairs(nullptr, bytes, b->keys, b->keys + n, b->idx, b->idx + n,
+                                         (size_t)n, 0, 64, s));
+        NB_TRY(dalloc(&b->sort_tmp, bytes));
+        b->sort_tmp_bytes = bytes;
+    }
+    k_sys_box<<<b->nsys, 256, 0, s>>>(b->off, b->x, b->y, b->z, b->sysbox);
+    NB_TRY(launch_ok("k_sys_box"));
+    k_batch_morton<<<blocks_for(n), 256, 0, s>>>(b->sys_of, b->x, b->y, b->z, b->sysbox, n,
+                                                 b->keys, b->idx);
+    NB_TRY(launch_ok("k_batch_morton"));
+    size_t bytes = b->sort_tmp_bytes;
+    NB_HIP(rocprim::radix_sort_pairs(b->sort_tmp, bytes, b->keys, b->keys + n, b->idx, b->idx + n,
+                                     (size_t)n, 0, 64, s));
+    const int64_t *idx = b->idx + n;
+    double *arrs[12] = {b->x, b->y, b->z, b->vx, b->vy, b->vz, b->m, b->h, b->ax, b->ay, b->az, b->pot};
+    for (double *p : arrs) NB_TRY(bpermute(p, idx, n, b->spare, s));
+    int64_t *np = (int64_t *)b->keys;
+    k_bcompose<<<blocks_for(n), 256, 0, s>>>(b->perm, idx, n, np);
+    NB_TRY(launch_ok("k_bcompose"));
+    if (!b->perm) NB_TRY(dalloc(&b->perm, sizeof(int64_t) * (size_t)n));
+    NB_HIP(hipMemcpyAsync(b->perm, np, sizeof(int64_t) * (size_t)n, hipMemcpyDeviceToDevice, s));
+    NB_TRY(rebuild_central(b));
+    b->sorted = true;
+    b->calls_since_sort = 0;
+    return NB_OK;
+}
+
+// Systems small enough for the resident kernel stay in caller order (the
+// resident kernel identifies central bodies and the origin by position in
+// the system); only the ragged path of larger systems is re-sorted.
+static int bmaybe_sort(nb_batch *b, int64_t upcoming) {
+    if (b->mode == NB_MODE_FP64 || b->tun.sort_interval <= 0 || resident_ok(b)) return NB_OK;
+    if (!b->sorted || b->calls_since_sort + upcoming > b->tun.sort_interval) return batch_sort(b);
+    return NB_OK;
+}
+
+static int bforce(nb_batch *b, bool with_pot, bool allow_timing) {
+    if (b->n == 0) return NB_OK;
+    hipStream_t s = b->stream;
+    const bool f64 = b->mode == NB_MODE_FP64;
+    const int tpt = f64 ? 1 : b->tun.targets_per_thread;
+    NB_TRY(build_items(b, tpt));
+    b->calls_since_sort++;
+    BatchArgs a{};
+    a.off = b->off; a.tile_base = b->tile_base; a.tile_box = b->tile_box; a.Gs = b->Gs;
+    a.x = b->x; a.y = b->y; a.z = b->z; a.m = b->m; a.h = b->h;
+    a.skip_src = (b->mode == NB_MODE_SPLIT_CENTRAL && b->ncentral > 0) ? b->central : nullptr;
+    a.ax = b->ax; a.ay = b->ay; a.az = b->az; a.pot = b->pot;
+    a.items = b->items; a.nitems = b->nitems; a.counter = b->counter;
+    hipEvent_t e0 = nullptr;
+    if (allow_timing && b->timing) {
+        NB_HIP(hipEventCreate(&e0));
+        NB_HIP(hipEventRecord(e0, s));
+    }
+    if (f64) {
+        if (with_pot) k_batch_force_f64<true><<<(unsigned)b->nitems, BS, 0, s>>>(a);
+        else k_batch_force_f64<false><<<(unsigned)b->nitems, BS, 0, s>>>(a);
+        NB_TRY(launch_ok("k_batch_force_f64"));
+    } else {
+        k_batch_tile_boxes<<<(unsigned)b->ntiles, BS, 0, s>>>(b->x, b->y, b->z, b->off,
+                                                              b->tile_base, b->tile_sys, b->tile_box);
+        NB_TRY(launch_ok("k_batch_tile_boxes"));
+        int nblk = b->tun.blocks > 0 ? b->tun.blocks : 8 * b->cu_count;
+        nblk = (int)std::max<int64_t>(1, std::min<int64_t>(nblk, b->nitems));
+#define NB_BL(T_, P_) k_batch_force<T_, P_><<<nblk, BS, 0, s>>>(a)
+        if (tpt == 1) { if (with_pot) NB_BL(1, true); else NB_BL(1, false); }
+        else if (tpt == 2) { if (with_pot) NB_BL(2, true); else NB_BL(2, false); }
+        else { if (with_pot) NB_BL(4, true); else NB_BL(4, false); }
+#undef NB_BL
+        NB_TRY(launch_ok("k_batch_force"));
+    }
+    if (e0) {
+        hipEvent_t e1;
+        NB_HIP(hipEventCreate(&e1));
+        NB_HIP(hipEventRecord(e1, s));
+        b->pending.emplace_back(e0, e1);
+    }
+    if (b->mode == NB_MODE_SPLIT_CENTRAL && b->ncentral > 0) {
+        if (with_pot)
+            k_batch_central<true><<<blocks_for(b->n), 256, 0, s>>>(
+                b->sys_of, b->cidx, b->ncentral, b->Gs, b->x, b->y, b->z, b->m, b->h, b->n, b->ax,
+                b->ay, b->az, b->pot);
+        else
+            k_batch_central<false><<<blocks_for(b->n), 256, 0, s>>>(
+                b->sys_of, b->cidx, b->ncentral, b->Gs, b->x, b->y, b->z, b->m, b->h, b->n, b->ax,
+                b->ay, b->az, b->pot);
+        NB_TRY(launch_ok("k_batch_central"));
+    }
+    return NB_OK;
+}
+
+static int bkick_drift(nb_batch *b, double kick_coef, bool drift) {
+    if (b->n == 0) return NB_OK;
+    k_batch_kick_drift<<<blocks_for(b->n), 256, 0, b->stream>>>(
+        b->x, b->y, b->z, b->vx, b->vy, b->vz, b->ax, b->ay, b->az, b->sys_of, b->dts, b->n,
+        kick_coef, drift ? 1 : 0);
+    return launch_ok("k_batch_kick_drift");
+}
+
+// S steps as one schedule: D, F, (KD, F) x (S-1), K
+static int enqueue_steps(nb_batch *b, int64_t S, bool timing) {
+    NB_TRY(bkick_drift(b, 0.0, true));
+    for (int64_t k = 0; k < S; ++k) {
+        if (k > 0) NB_TRY(bkick_drift(b, 1.0, true));
+        NB_TRY(bforce(b, false, timing));
+    }
+    NB_TRY(bkick_drift(b, 1.0, false));
+    return NB_OK;
+}
+
+static int resident_steps(nb_batch *b, int64_t nsteps, int every, double *energy_dev, int slots) {
+    ResArgs a{};
+    a.off = b->off; a.Gs = b->Gs; a.hs = b->hs; a.dts = b->dts;
+    a.x = b->x; a.y = b->y; a.z = b->z; a.vx = b->vx; a.vy = b->vy; a.vz = b->vz;
+    a.m = b->m; a.ax = b->ax; a.ay = b->ay; a.az = b->az;
+    a.ncentral = b->mode == NB_MODE_SPLIT_CENTRAL ? b->ncentral : 0;
+    a.energy = energy_dev; a.energy_every = every > 0 ? every : 1; a.energy_slots = slots;
+    if (nsteps > INT32_MAX) {
+        set_error("too many steps for one resident launch");
+        return NB_ERR_INVALID;
+    }
+    if (b->mode == NB_MODE_FP64)
+        k_batch_resident<true><<<b->nsys, 256, 0, b->stream>>>(a, (int)nsteps);
+    else
+        k_batch_resident<false><<<b->nsys, 256, 0, b->stream>>>(a, (int)nsteps);
+    return launch_ok("k_batch_resident");
+}
+
+static int batch_steps(nb_batch *b, int64_t nsteps, bool use_graph) {
+    if (nsteps <= 0 || b->n == 0) return NB_OK;
+    if (resident_ok(b) && use_graph) {
+        // the resident kernel runs unsorted (caller order); restore it if needed
+        return resident_steps(b, nsteps, 0, nullptr, 0);
+    }
+    int64_t k = 0;
+    const int64_t S = std::max(2, b->tun.steps_per_graph & ~1);
+    if (use_graph && nsteps >= S) {
+        if (!b->gexec || b->g_steps != S) {
+            bdrop_graph(b);
+            NB_TRY(build_items(b, b->mode == NB_MODE_FP64 ? 1 : b->tun.targets_per_thread));
+            NB_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            const int64_t keep = b->calls_since_sort;
+            int rc = enqueue_steps(b, S, false);
+            b->calls_since_sort = keep;
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(b->stream, &g);
+            if (rc != NB_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) {
+                set_error("hipStreamEndCapture failed: %s", hipGetErrorString(e));
+                if (g) (void)hipGraphDestroy(g);
+                return NB_ERR_HIP;
+            }
+            e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                b->gexec = nullptr;
+                set_error("hipGraphInstantiate failed: %s", hipGetErrorString(e));
+                return NB_ERR_HIP;
+            }
+            b->g_steps = S;
+        }
+        for (; k + S <= nsteps; k += S) {
+            NB_TRY(bmaybe_sort(b, S));
+            NB_HIP(hipGraphLaunch(b->gexec, b->stream));
+            b->calls_since_sort += S;
+        }
+    }
+    for (; k < nsteps; ++k) {
+        NB_TRY(bmaybe_sort(b, 1));
+        NB_TRY(enqueue_steps(b, 1, true));
+    }
+    return NB_OK;
+}
+
+static int collect(nb_batch *b) {
+    for (auto &p : b->pending) {
+        NB_HIP(hipEventSynchronize(p.second));
+        float ms = 0.f;
+        NB_HIP(hipEventElapsedTime(&ms, p.first, p.second));
+        b->tms += ms;
+        b->tcount++;
+        (void)hipEventDestroy(p.first);
+        (void)hipEventDestroy(p.second);
+    }
+    b->pending.clear();
+    return NB_OK;
+}
+
+}  // namespace nb
+
+using namespace nb;
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+int nb_batch_create(nb_batch **out, int32_t nsys, const int64_t *offsets, int32_t precision_mode) {
+    clear_error();
+    if (!out || !offsets || nsys < 0) {
+        set_error("invalid batch arguments");
+        return NB_ERR_INVALID;
+    }
+    *out = nullptr;
+    if (precision_mode != NB_MODE_FP64 && precision_mode != NB_MODE_SPLIT &&
+        precision_mode != NB_MODE_SPLIT_CENTRAL) {
+        set_error("batch precision mode must be FP64, SPLIT or SPLIT_CENTRAL (got %d)",
+                  precision_mode);
+        return NB_ERR_INVALID;
+    }
+    if (offsets[0] != 0) {
+        set_error("offsets[0] must be 0");
+        return NB_ERR_INVALID;
+    }
+    int64_t max_ns = 0;
+    for (int32_t s = 0; s < nsys; ++s) {
+        const int64_t ns = offsets[s + 1] - offsets[s];
+        if (ns < 0) {
+            set_error("offsets must be non-decreasing (system %d)", s);
+            return NB_ERR_INVALID;
+        }
+        max_ns = std::max(max_ns, ns);
+    }
+    if (offsets[nsys] > ((int64_t)1 << 31)) {
+        set_error("too many particles");
+        return NB_ERR_INVALID;
+    }
+    nb_batch *b = new (std::nothrow) nb_batch();
+    if (!b) {
+        set_error("host allocation failed");
+        return NB_ERR_NOMEM;
+    }
+    b->nsys = nsys;
+    b->mode = precision_mode;
+    b->n = offsets[nsys];
+    b->max_ns = max_ns;
+    b->h_off.assign(offsets, offsets + nsys + 1);
+    b->hG.assign(nsys, 1.0);
+    b->hh.assign(nsys, 1e-4);
+    b->hdt.assign(nsys, 1e-3);
+    b->tun.targets_per_thread = 1;
+    b->tun.blocks = 0;
+    b->tun.steps_per_graph = 16;
+    b->tun.sort_interval = precision_mode == NB_MODE_FP64 ? 0 : 16;
+    b->tun.resident = 1;
+    int rc = NB_OK;
+    do {
+        hipError_t e = hipGetDevice(&b->dev);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            set_error("device/stream setup failed: %s", hipGetErrorString(e));
+            rc = NB_ERR_HIP;
+            break;
+        }
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, b->dev) == hipSuccess) b->cu_count = p.multiProcessorCount;
+        const size_t n = (size_t)b->n, ns = (size_t)nsys;
+        // tiles
+        std::vector<int64_t> tb(ns + 1, 0);
+        for (size_t s = 0; s < ns; ++s) tb[s + 1] = tb[s] + (b->h_off[s + 1] - b->h_off[s] + BS - 1) / BS;
+        b->ntiles = tb[ns];
+        std::vector<int32_t> tsys((size_t)b->ntiles);
+        for (size_t s = 0; s < ns; ++s)
+            for (int64_t t = tb[s]; t < tb[s + 1]; ++t) tsys[(size_t)t] = (int32_t)s;
+        std::vector<int32_t> sys_of(n);
+        for (size_t s = 0; s < ns; ++s)
+            for (int64_t i = b->h_off[s]; i < b->h_off[s + 1]; ++i) sys_of[(size_t)i] = (int32_t)s;
+        if ((rc = dalloc(&b->off, sizeof(int64_t) * (ns + 1))) ||
+            (rc = dalloc(&b->sys_of, sizeof(int32_t) * n)) ||
+            (rc = dalloc(&b->Gs, sizeof(double) * ns)) || (rc = dalloc(&b->hs, sizeof(double) * ns)) ||
+            (rc = dalloc(&b->dts, sizeof(double) * ns)) || (rc = dalloc(&b->x, sizeof(double) * n)) ||
+            (rc = dalloc(&b->y, sizeof(double) * n)) || (rc = dalloc(&b->z, sizeof(double) * n)) ||
+            (rc = dalloc(&b->vx, sizeof(double) * n)) || (rc = dalloc(&b->vy, sizeof(double) * n)) ||
+            (rc = dalloc(&b->vz, sizeof(double) * n)) || (rc = dalloc(&b->m, sizeof(double) * n)) ||
+            (rc = dalloc(&b->h, sizeof(double) * n)) || (rc = dalloc(&b->ax, sizeof(double) * n)) ||
+            (rc = dalloc(&b->ay, sizeof(double) * n)) || (rc = dalloc(&b->az, sizeof(double) * n)) ||
+            (rc = dalloc(&b->pot, sizeof(double) * n)) || (rc = dalloc(&b->central, n)) ||
+            (rc = dalloc(&b->tile_base, sizeof(int64_t) * (ns + 1))) ||
+            (rc = dalloc(&b->tile_sys, sizeof(int32_t) * (size_t)std::max<int64_t>(b->ntiles, 1))) ||
+            (rc = dalloc(&b->tile_box, sizeof(double) * 8 * (size_t)std::max<int64_t>(b->ntiles, 1))) ||
+            (rc = dalloc(&b->counter, sizeof(unsigned) * 2)) ||
+            (rc = dalloc(&b->stage, sizeof(double) * 3 * n)) ||
+            (rc = dalloc(&b->red, sizeof(double) * 8 * std::max<size_t>(ns, 1))))
+            break;
+        e = hipMemcpy(b->off, offsets, sizeof(int64_t) * (ns + 1), hipMemcpyHostToDevice);
+        if (e == hipSuccess && n) e = hipMemcpy(b->sys_of, sys_of.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(b->tile_base, tb.data(), sizeof(int64_t) * (ns + 1), hipMemcpyHostToDevice);
+        if (e == hipSuccess && b->ntiles) e = hipMemcpy(b->tile_sys, tsys.data(), sizeof(int32_t) * tsys.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemset(b->counter, 0, sizeof(unsigned) * 2);
+        if (e == hipSuccess && n) e = hipMemset(b->central, 0, n);
+        if (e == hipSuccess && n) e = hipMemset(b->ax, 0, sizeof(double) * n);
+        if (e == hipSuccess && n) e = hipMemset(b->ay, 0, sizeof(double) * n);
+        if (e == hipSuccess && n) e = hipMemset(b->az, 0, sizeof(double) * n);
+        if (e == hipSuccess && n) e = hipMemset(b->pot, 0, sizeof(double) * n);
+        if (e != hipSuccess) {
+            set_error("batch initialisation failed: %s", hipGetErrorString(e));
+            rc = NB_ERR_HIP;
+            break;
+        }
+        rc = nb_batch_set_params(b, b->hG.data(), b->hh.data(), b->hdt.data());
+    } while (0);
+    if (rc != NB_OK) {
+        std::string keep = nb_last_error();
+        nb_batch_destroy(b);
+        set_error("%s", keep.c_str());
+        return rc;
+    }
+    *out = b;
+    return NB_OK;
+}
+
+int nb_batch_destroy(nb_batch *b) {
+    clear_error();
+    if (!b) return NB_OK;
+    (void)hipSetDevice(b->dev);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (auto &p : b->pending) {
+        (void)hipEventDestroy(p.first);
+        (void)hipEventDestroy(p.second);
+    }
+    bdrop_graph(b);
+    dfree(b->off); dfree(b->sys_of); dfree(b->Gs); dfree(b->hs); dfree(b->dts);
+    dfree(b->x); dfree(b->y); dfree(b->z); dfree(b->vx); dfree(b->vy); dfree(b->vz);
+    dfree(b->m); dfree(b->h); dfree(b->ax); dfree(b->ay); dfree(b->az); dfree(b->pot);
+    dfree(b->perm); dfree(b->central); dfree(b->cidx);
+    dfree(b->tile_base); dfree(b->tile_sys); dfree(b->tile_box);
+    dfree(b->items); dfree(b->counter); dfree(b->stage); dfree(b->red);
+    dfree(b->sort_tmp); dfree(b->keys); dfree(b->idx); dfree(b->sysbox); dfree(b->spare);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return NB_OK;
+}
+
+int nb_batch_set_params(nb_batch *b, const double *G, const double *soft, const double *dt) {
+    NB_TRY(bcheck(b, false));
+    if (!G || !soft || !dt) {
+        set_error("null parameter array");
+        return NB_ERR_INVALID;
+    }
+    for (int32_t s = 0; s < b->nsys; ++s) {
+        if (!isfinite(G[s]) || !(soft[s] >= 0.0) || !isfinite(soft[s]) || !isfinite(dt[s])) {
+            set_error("invalid parameters of system %d", s);
+            return NB_ERR_INVALID;
+        }
+    }
+    if (G != b->hG.data()) b->hG.assign(G, G + b->nsys);
+    if (soft != b->hh.data()) b->hh.assign(soft, soft + b->nsys);
+    if (dt != b->hdt.data()) b->hdt.assign(dt, dt + b->nsys);
+    bdrop_graph(b);
+    const size_t ns = (size_t)b->nsys;
+    if (ns) {
+        NB_HIP(hipMemcpy(b->Gs, b->hG.data(), sizeof(double) * ns, hipMemcpyHostToDevice));
+        NB_HIP(hipMemcpy(b->hs, b->hh.data(), sizeof(double) * ns, hipMemcpyHostToDevice));
+        NB_HIP(hipMemcpy(b->dts, b->hdt.data(), sizeof(double) * ns, hipMemcpyHostToDevice));
+    }
+    if (b->uploaded && b->n) {
+        // per-particle h from the system table, in the current internal order
+        std::vector<double> hh((size_t)b->n);
+        std::vector<int32_t> so((size_t)b->n);
+        NB_HIP(hipMemcpy(so.data(), b->sys_of, sizeof(int32_t) * so.size(), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < hh.size(); ++i) hh[i] = b->hh[(size_t)so[i]];
+        NB_HIP(hipMemcpy(b->h, hh.data(), sizeof(double) * hh.size(), hipMemcpyHostToDevice));
+    }
+    return NB_OK;
+}
+
+int nb_batch_upload(nb_batch *b, const double *pos, const double *vel, const double *mass) {
+    NB_TRY(bcheck(b, false));
+    const int64_t n = b->n;
+    if (n > 0 && (!pos || !mass)) {
+        set_error("pos and mass are required");
+        return NB_ERR_INVALID;
+    }
+    bdrop_graph(b);
+    dfree(b->perm);
+    b->sorted = false;
+    hipStream_t s = b->stream;
+    if (n > 0) {
+        NB_HIP(hipMemcpyAsync(b->stage, pos, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, s));
+        NB_TRY(to_soa(b->stage, nullptr, n, b->x, b->y, b->z, true, s));
+        NB_HIP(hipStreamSynchronize(s));
+        if (vel) {
+            NB_HIP(hipMemcpyAsync(b->stage, vel, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, s));
+            NB_TRY(to_soa(b->stage, nullptr, n, b->vx, b->vy, b->vz, true, s));
+        } else {
+            NB_HIP(hipMemsetAsync(b->vx, 0, sizeof(double) * (size_t)n, s));
+            NB_HIP(hipMemsetAsync(b->vy, 0, sizeof(double) * (size_t)n, s));
+            NB_HIP(hipMemsetAsync(b->vz, 0, sizeof(double) * (size_t)n, s));
+        }
+        NB_HIP(hipMemcpyAsync(b->m, mass, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, s));
+        NB_HIP(hipStreamSynchronize(s));
+    }
+    b->uploaded = true;
+    NB_TRY(nb_batch_set_params(b, b->hG.data(), b->hh.data(), b->hdt.data()));
+    NB_TRY(rebuild_central(b));
+    NB_HIP(hipStreamSynchronize(s));
+    return NB_OK;
+}
+
+int nb_batch_download(nb_batch *b, double *pos, double *vel, double *acc, double *pot) {
+    NB_TRY(bcheck(b));
+    const int64_t n = b->n;
+    if (n == 0) return NB_OK;
+    hipStream_t s = b->stream;
+    const size_t b3 = sizeof(double) * 3 * (size_t)n;
+    if (pos) {
+        NB_TRY(to_aos(b->x, b->y, b->z, b->perm, n, true, b->stage, s));
+        NB_HIP(hipMemcpyAsync(pos, b->stage, b3, hipMemcpyDeviceToHost, s));
+        NB_HIP(hipStreamSynchronize(s));
+    }
+    if (vel) {
+        NB_TRY(to_aos(b->vx, b->vy, b->vz, b->perm, n, true, b->stage, s));
+        NB_HIP(hipMemcpyAsync(vel, b->stage, b3, hipMemcpyDeviceToHost, s));
+        NB_HIP(hipStreamSynchronize(s));
+    }
+    if (acc) {
+        NB_TRY(to_aos(b->ax, b->ay, b->az, b->perm, n, true, b->stage, s));
+        NB_HIP(hipMemcpyAsync(acc, b->stage, b3, hipMemcpyDeviceToHost, s));
+        NB_HIP(hipStreamSynchronize(s));
+    }
+    if (pot) {
+        NB_TRY(scalar_out(b->pot, b->perm, n, b->stage, s));
+        NB_HIP(hipMemcpyAsync(pot, b->stage, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, s));
+        NB_HIP(hipStreamSynchronize(s));
+    }
+    return NB_OK;
+}
+
+int nb_batch_set_central(nb_batch *b, int32_t count) {
+    NB_TRY(bcheck(b, false));
+    if (count < 0 || (count > 0 && b->mode != NB_MODE_SPLIT_CENTRAL)) {
+        set_error("central bodies need NB_MODE_SPLIT_CENTRAL and count >= 0");
+        return NB_ERR_INVALID;
+    }
+    for (int32_t s = 0; s < b->nsys; ++s) {
+        if (b->h_off[s + 1] - b->h_off[s] < count) {
+            set_error("system %d has fewer than %d particles", s, count);
+            return NB_ERR_INVALID;
+        }
+    }
+    bdrop_graph(b);
+    dfree(b->cidx);
+    b->ncentral = count;
+    NB_HIP(hipMemset(b->central, 0, (size_t)std::max<int64_t>(b->n, 1)));
+    if (count > 0) {
+        NB_TRY(dalloc(&b->cidx, sizeof(int64_t) * (size_t)b->nsys * (size_t)count));
+        NB_TRY(rebuild_central(b));
+        NB_HIP(hipStreamSynchronize(b->stream));
+    }
+    return NB_OK;
+}
+
+int nb_batch_compute_force(nb_batch *b) {
+    NB_TRY(bcheck(b));
+    NB_TRY(bmaybe_sort(b, 1));
+    NB_TRY(bforce(b, false, true));
+    return NB_OK;
+}
+
+int nb_batch_compute_potential(nb_batch *b) {
+    NB_TRY(bcheck(b));
+    NB_TRY(bmaybe_sort(b, 1));
+    NB_TRY(bforce(b, true, false));
+    NB_HIP(hipStreamSynchronize(b->stream));
+    return NB_OK;
+}
+
+int nb_batch_half_kick(nb_batch *b, int32_t forward) {
+    NB_TRY(bcheck(b));
+    if (forward != 1 && forward != -1) {
+        set_error("forward must be +1 or -1");
+        return NB_ERR_INVALID;
+    }
+    return bkick_drift(b, (double)forward * 0.5, false);
+}
+
+int nb_batch_kdk_steps(nb_batch *b, int64_t nsteps, int32_t use_graph) {
+    NB_TRY(bcheck(b));
+    if (nsteps < 0) {
+        set_error("nsteps must be >= 0");
+        return NB_ERR_INVALID;
+    }
+    return batch_steps(b, nsteps, use_graph != 0);
+}
+
+int nb_batch_kdk_steps_tracked(nb_batch *b, int64_t nsteps, int32_t every, double *energy) {
+    NB_TRY(bcheck(b));
+    if (nsteps < 0 || every <= 0 || !energy) {
+        set_error("invalid tracked-step arguments");
+        return NB_ERR_INVALID;
+    }
+    const int64_t slots = nsteps / every;
+    const size_t ns = (size_t)b->nsys;
+    if (slots == 0) return batch_steps(b, nsteps, true);
+    double *dev = nullptr;
+    NB_TRY(dalloc(&dev, sizeof(double) * ns * (size_t)slots));
+    int rc = NB_OK;
+    if (resident_ok(b)) {
+        rc = resident_steps(b, nsteps, every, dev, (int)slots);
+    } else {
+        std::vector<double> host(ns * 8);
+        for (int64_t k = 0; k < slots && rc == NB_OK; ++k) {
+            rc = batch_steps(b, every, true);
+            if (rc == NB_OK) rc = bforce(b, true, false);   // potential (acc unchanged)
+            if (rc == NB_OK) {
+                k_batch_energy<<<b->nsys, 256, 0, b->stream>>>(
+                    b->off, b->x, b->y, b->z, b->vx, b->vy, b->vz, b->ax, b->ay, b->az, b->m,
+                    b->pot, b->dts, 1, b->red);
+                rc = launch_ok("k_batch_energy");
+            }
+            if (rc == NB_OK) {
+                hipError_t e = hipMemcpyAsync(host.data(), b->red, sizeof(double) * 8 * ns,
+                                              hipMemcpyDeviceToHost, b->stream);
+                if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+                if (e != hipSuccess) {
+                    set_error("energy readback: %s", hipGetErrorString(e));
+                    rc = NB_ERR_HIP;
+                }
+            }
+            if (rc == NB_OK)
+                for (size_t s = 0; s < ns; ++s)
+                    energy[s * (size_t)slots + (size_t)k] = host[s * 8] + 0.5 * host[s * 8 + 1];
+        }
+        if (rc == NB_OK && nsteps % every) rc = batch_steps(b, nsteps % every, true);
+        dfree(dev);
+        return rc;
+    }
+    if (rc == NB_OK) {
+        hipError_t e = hipMemcpyAsync(energy, dev, sizeof(double) * ns * (size_t)slots,
+                                      hipMemcpyDeviceToHost, b->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+        if (e != hipSuccess) {
+            set_error("energy readback: %s", hipGetErrorString(e));
+            rc = NB_ERR_HIP;
+        }
+    }
+    if (rc == NB_OK && nsteps % every) rc = resident_steps(b, nsteps % every, 0, nullptr, 0);
+    (void)hipStreamSynchronize(b->stream);
+    dfree(dev);
+    return rc;
+}
+
+int nb_batch_energy(nb_batch *b, double *K, double *W, double *P, double *L) {
+    NB_TRY(bcheck(b));
+    if (W) {
+        NB_TRY(bmaybe_sort(b, 1));
+        NB_TRY(bforce(b, true, false));
+    }
+    k_batch_energy<<<b->nsys, 256, 0, b->stream>>>(b->off, b->x, b->y, b->z, b->vx, b->vy, b->vz,
+                                                   b->ax, b->ay, b->az, b->m, W ? b->pot : nullptr,
+                                                   b->dts, 0, b->red);
+    NB_TRY(launch_ok("k_batch_energy"));
+    std::vector<double> host((size_t)b->nsys * 8);
+    NB_HIP(hipMemcpyAsync(host.data(), b->red, sizeof(double) * host.size(), hipMemcpyDeviceToHost,
+                          b->stream));
+    NB_HIP(hipStreamSynchronize(b->stream));
+    for (int32_t s = 0; s < b->nsys; ++s) {
+        const double *r = host.data() + (size_t)s * 8;
+        if (K) K[s] = r[0];
+        if (W) W[s] = 0.5 * r[1];
+        if (P) { P[3 * s] = r[2]; P[3 * s + 1] = r[3]; P[3 * s + 2] = r[4]; }
+        if (L) { L[3 * s] = r[5]; L[3 * s + 1] = r[6]; L[3 * s + 2] = r[7]; }
+    }
+    return NB_OK;
+}
+
+int nb_batch_sync(nb_batch *b) {
+    NB_TRY(bcheck(b, false));
+    NB_HIP(hipStreamSynchronize(b->stream));
+    return NB_OK;
+}
+
+int nb_batch_set_tuning(nb_batch *b, const nb_batch_tuning *t) {
+    NB_TRY(bcheck(b, false));
+    if (!t || (t->targets_per_thread != 1 && t->targets_per_thread != 2 &&
+               t->targets_per_thread != 4) ||
+        t->blocks < 0 || t->blocks > (1 << 20) || t->steps_per_graph < 2 ||
+        t->steps_per_graph > 4096 || t->sort_interval < 0) {
+        set_error("invalid batch tuning");
+        return NB_ERR_INVALID;
+    }
+    bdrop_graph(b);
+    b->tun = *t;
+    return NB_OK;
+}
+
+int nb_batch_get_tuning(nb_batch *b, nb_batch_tuning *t) {
+    NB_TRY(bcheck(b, false));
+    if (!t) {
+        set_error("null output");
+        return NB_ERR_INVALID;
+    }
+    *t = b->tun;
+    return NB_OK;
+}
+
+int nb_batch_autotune(nb_batch *b) {
+    NB_TRY(bcheck(b));
+    if (b->mode == NB_MODE_FP64 || b->n == 0) return NB_OK;
+    NB_TRY(bmaybe_sort(b, 1));
+    const nb_batch_tuning keep = b->tun;
+    nb_batch_tuning best = keep;
+    float best_ms = 1e30f;
+    const int tpts[3] = {1, 2, 4};
+    const int mult[4] = {2, 4, 8, 16};
+    hipEvent_t e0, e1;
+    NB_HIP(hipEventCreate(&e0));
+    NB_HIP(hipEventCreate(&e1));
+    int rc = NB_OK;
+    for (int ti = 0; ti < 3 && rc == NB_OK; ++ti) {
+        for (int mi = 0; mi < 4 && rc == NB_OK; ++mi) {
+            b->tun.targets_per_thread = tpts[ti];
+            b->tun.blocks = mult[mi] * b->cu_count;
+            rc = bforce(b, false, false);
+            if (rc) break;
+            (void)hipEventRecord(e0, b->stream);
+            for (int r = 0; r < 3 && rc == NB_OK; ++r) rc = bforce(b, false, false);
+            (void)hipEventRecord(e1, b->stream);
+            (void)hipEventSynchronize(e1);
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = b->tun;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->tun = best;
+    bdrop_graph(b);
+    if (rc == NB_OK) rc = bforce(b, false, false);
+    if (rc == NB_OK) {
+        hipError_t e = hipStreamSynchronize(b->stream);
+        if (e != hipSuccess) {
+            set_error("autotune sync: %s", hipGetErrorString(e));
+            rc = NB_ERR_HIP;
+        }
+    }
+    return rc;
+}
+
+int nb_batch_set_timing(nb_batch *b, int32_t enable) {
+    NB_TRY(bcheck(b, false));
+    NB_TRY(collect(b));
+    b->timing = enable != 0;
+    b->tcount = 0;
+    b->tms = 0.0;
+    return NB_OK;
+}
+
+int nb_batch_get_timing(nb_batch *b, int64_t *count, double *total_ms) {
+    NB_TRY(bcheck(b, false));
+    NB_TRY(collect(b));
+    if (count) *count = b->tcount;
+    if (total_ms) *total_ms = b->tms;
+    return NB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,282 @@
+// nb_block.h -- the block-level all-pairs routine shared by the one-system
+// kernel (nb_force.hip) and the ragged ensemble kernel (nb_batch.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <stdint.h>
+
+#include "nb_internal.h"
+#include "nb_pair.h"
+
+namespace nb {
+
+// Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).
+// Map hardware block ids so that each XCD receives a contiguous range of
+// logical blocks; logical blocks of one source segment then share an L2.
+__device__ __forceinline__ int xcd_remap(int b, int nblocks) {
+    constexpr int X = 8;
+    int per = nblocks / X, rem = nblocks % X;
+    int xcd = b % X, slot = b / X;
+    return xcd * per + (xcd < rem ? xcd : rem) + slot;
+}
+
+template <typename V>
+__device__ __forceinline__ V wave_sum(V v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+template <typename V>
+__device__ __forceinline__ V wave_min(V v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+    return v;
+}
+template <typename V>
+__device__ __forceinline__ V wave_max(V v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Block statistics of the targets: centroid, bounding box, largest h.
+struct BlockStats {
+    double ox, oy, oz;        // origin (centroid, or 0)
+    double lo[3], hi[3];      // bounding box of the valid targets
+    double hmax;
+};
+
+constexpr int NW = BS / 64;
+
+template <int TPT>
+__device__ void block_stats(const double (&px)[TPT], const double (&py)[TPT], const double (&pz)[TPT],
+                            const double (&ph)[TPT], const bool (&valid)[TPT], bool centroid,
+                            double (*red)[NW], BlockStats &st) {
+    double v[11];
+    v[0] = v[1] = v[2] = v[3] = 0.0;
+    v[4] = v[5] = v[6] = DBL_MAX;
+    v[7] = v[8] = v[9] = -DBL_MAX;
+    v[10] = 0.0;
+#pragma unroll
+    for (int k = 0; k < TPT; ++k) {
+        if (valid[k]) {
+            v[0] += px[k]; v[1] += py[k]; v[2] += pz[k]; v[3] += 1.0;
+            v[4] = fmin(v[4], px[k]); v[5] = fmin(v[5], py[k]); v[6] = fmin(v[6], pz[k]);
+            v[7] = fmax(v[7], px[k]); v[8] = fmax(v[8], py[k]); v[9] = fmax(v[9], pz[k]);
+            v[10] = fmax(v[10], ph[k]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = wave_sum(v[q]);
+#pragma unroll
+    for (int q = 4; q < 7; ++q) v[q] = wave_min(v[q]);
+#pragma unroll
+    for (int q = 7; q < 11; ++q) v[q] = wave_max(v[q]);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < 11; ++q) red[q][w] = v[q];
+    }
+    __syncthreads();
+    double r[11];
+#pragma unroll
+    for (int q = 0; q < 11; ++q) r[q] = red[q][0];
+#pragma unroll
+    for (int ww = 1; ww < NW; ++ww) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) r[q] += red[q][ww];
+#pragma unroll
+        for (int q = 4; q < 7; ++q) r[q] = fmin(r[q], red[q][ww]);
+#pragma unroll
+        for (int q = 7; q < 11; ++q) r[q] = fmax(r[q], red[q][ww]);
+    }
+    __syncthreads();
+    if (centroid && r[3] > 0.0) {
+        st.ox = r[0] / r[3]; st.oy = r[1] / r[3]; st.oz = r[2] / r[3];
+    } else {
+        st.ox = st.oy = st.oz = 0.0;
+    }
+    st.lo[0] = r[4]; st.lo[1] = r[5]; st.lo[2] = r[6];
+    st.hi[0] = r[7]; st.hi[1] = r[8]; st.hi[2] = r[9];
+    st.hmax = r[10];
+}
+
+// Source staged for LDS: position relative to the block origin, FP32.
+// Sources past the end of the range (or excluded) become massless particles
+// placed far away, so both tile paths can run a fixed trip count.
+constexpr float PAD_COORD = 1.0e18f;
+
+template <int TPT, bool CHECKED, bool POT>
+__device__ __forceinline__ void tile_f32(const float4 *__restrict__ s, const float (&xi)[TPT],
+                                         const float (&yi)[TPT], const float (&zi)[TPT],
+                                         const float (&h2c)[TPT], const float (&h2)[TPT],
+                                         const float (&hinv)[TPT], const float (&hinv3)[TPT],
+                                         float (&ax)[TPT], float (&ay)[TPT], float (&az)[TPT],
+                                         float (&aq)[TPT]) {
+#pragma unroll 4
+    for (int jj = 0; jj < BS; ++jj) {
+        const float4 v = s[jj];
+#pragma unroll
+        for (int k = 0; k < TPT; ++k)
+            pair_f32<CHECKED, POT>(v.x, v.y, v.z, v.w, xi[k], yi[k], zi[k], h2c[k], h2[k], hinv[k],
+                                   hinv3[k], ax[k], ay[k], az[k], aq[k]);
+    }
+}
+
+template <int TPT, bool POT>
+__device__ __forceinline__ void tile_f32_hilo(const float4 *__restrict__ s,
+                                              const float4 *__restrict__ sl,
+                                              const float (&xi)[TPT], const float (&yi)[TPT],
+                                              const float (&zi)[TPT], const float (&xil)[TPT],
+                                              const float (&yil)[TPT], const float (&zil)[TPT],
+                                              const float (&h2c)[TPT], const float (&h2)[TPT],
+                                              const float (&hinv)[TPT], const float (&hinv3)[TPT],
+                                              float (&ax)[TPT], float (&ay)[TPT],
+                                              float (&az)[TPT], float (&aq)[TPT]) {
+#pragma unroll 2
+    for (int jj = 0; jj < BS; ++jj) {
+        const float4 v = s[jj];
+        const float4 w = sl[jj];
+#pragma unroll
+        for (int k = 0; k < TPT; ++k)
+            pair_f32_hilo<POT>(v.x, v.y, v.z, v.w, w.x, w.y, w.z, xi[k], yi[k], zi[k], xil[k],
+                               yil[k], zil[k], h2c[k], h2[k], hinv[k], hinv3[k], ax[k], ay[k],
+                               az[k], aq[k]);
+    }
+}
+
+// FP32-pair kernel.  T = double (SPLIT: FP64 state, per-block origin, hi/lo
+// coordinates on near tiles) or float (FP32 state, absolute coordinates).
+//
+// Tile classification (block-uniform): a source tile is "far" when its box is
+// separated from the block's target box by more than G = max(h_max, R) along
+// some axis, where R is the block's radius about its origin (SPLIT) or 0
+// (FP32).  Far tiles run the 12-VALU Newtonian loop on hi coordinates: every
+// pair there has r > h (no softening) and r > R, so the separation carries a
+// relative rounding error below 4 FP32 epsilons.  Near tiles run the
+// softening-checked loop (on hi/lo coordinates in SPLIT).
+
+// Shared memory of one all-pairs block.
+template <typename T>
+struct BlockShared {
+    static constexpr bool HILO = sizeof(T) == 8;
+    float4 sj[2][BS];
+    float4 sjl[HILO ? 2 : 1][HILO ? BS : 1];
+    double red[11][NW];
+};
+
+// One block's share of an all-pairs sum: targets i0 + k*BS + tid (k < TPT,
+// i < i_end) against the sources [j0, j1).  tile_box points at the box of
+// the BS-source tile that starts at j0 (tiles are BS-aligned from j0).
+// Returns FP64 sums (without G) in Ax..Aq and the target validity.
+template <typename T, int TPT, bool POT>
+__device__ __forceinline__ void force_block(
+    BlockShared<T> &sh, const T *__restrict__ tx, const T *__restrict__ ty,
+    const T *__restrict__ tz, const double *__restrict__ th, double th_const, int64_t i0,
+    int64_t i_end, const T *__restrict__ sx, const T *__restrict__ sy, const T *__restrict__ sz,
+    const T *__restrict__ sm, int64_t j0, int64_t j1, const double *__restrict__ tile_box,
+    const uint8_t *__restrict__ skip_src, bool use_origin, double (&Ax)[TPT], double (&Ay)[TPT],
+    double (&Az)[TPT], double (&Aq)[TPT], bool (&valid)[TPT]) {
+    constexpr bool HILO = BlockShared<T>::HILO;
+    const int tid = threadIdx.x;
+    __syncthreads();  // the previous call's readers of sh are done
+
+    double px[TPT], py[TPT], pz[TPT], ph[TPT];
+#pragma unroll
+    for (int k = 0; k < TPT; ++k) {
+        const int64_t i = i0 + k * BS + tid;
+        valid[k] = i < i_end;
+        px[k] = valid[k] ? (double)tx[i] : 0.0;
+        py[k] = valid[k] ? (double)ty[i] : 0.0;
+        pz[k] = valid[k] ? (double)tz[i] : 0.0;
+        ph[k] = valid[k] ? (th ? th[i] : th_const) : 0.0;
+    }
+    BlockStats st;
+    block_stats<TPT>(px, py, pz, ph, valid, use_origin, sh.red, st);
+    const double ox = st.ox, oy = st.oy, oz = st.oz;
+
+    float xi[TPT], yi[TPT], zi[TPT], xil[TPT], yil[TPT], zil[TPT];
+    float h2c[TPT], h2[TPT], hinv[TPT], hinv3[TPT];
+    float ax[TPT], ay[TPT], az[TPT], aq[TPT];
+#pragma unroll
+    for (int k = 0; k < TPT; ++k) {
+        const double dx = px[k] - ox, dy = py[k] - oy, dz = pz[k] - oz;
+        xi[k] = (float)dx;
+        yi[k] = (float)dy;
+        zi[k] = (float)dz;
+        xil[k] = (float)(dx - (double)xi[k]);
+        yil[k] = (float)(dy - (double)yi[k]);
+        zil[k] = (float)(dz - (double)zi[k]);
+        const float hf = (float)ph[k];
+        h2[k] = hf * hf;
+        h2c[k] = fmaxf(h2[k], FLT_MIN);
+        hinv[k] = 1.0f / hf;
+        hinv3[k] = hinv[k] * hinv[k] * hinv[k];
+        ax[k] = ay[k] = az[k] = aq[k] = 0.0f;
+        Ax[k] = Ay[k] = Az[k] = Aq[k] = 0.0;
+    }
+    // Target box grown by the classification distance plus a margin that
+    // keeps every far pair at FP32-resolvable, nonzero separation.
+    double rb = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double o = d == 0 ? ox : (d == 1 ? oy : oz);
+        rb = fmax(rb, fmax(fabs(st.lo[d] - o), fabs(st.hi[d] - o)));
+    }
+    const double reach = HILO ? fmax(st.hmax, rb) : st.hmax;
+    const double grow = reach * (1.0 + 1e-6) + 1e-6 * rb + 1e-300;
+    const double glo0 = st.lo[0] - grow, glo1 = st.lo[1] - grow, glo2 = st.lo[2] - grow;
+    const double ghi0 = st.hi[0] + grow, ghi1 = st.hi[1] + grow, ghi2 = st.hi[2] + grow;
+
+    // Register prefetch of the first tile.
+    T rx = 0, ry = 0, rz = 0, rm = 0;
+    bool rv = false;
+    {
+        const int64_t j = j0 + tid;
+        rv = j < j1 && !(skip_src && skip_src[j]);
+        if (rv) { rx = sx[j]; ry = sy[j]; rz = sz[j]; rm = sm[j]; }
+    }
+    int it = 0;
+    for (int64_t base = j0; base < j1; base += BS, ++it) {
+        const int buf = it & 1;
+        if (rv) {
+            const double dx = (double)rx - ox, dy = (double)ry - oy, dz = (double)rz - oz;
+            const float hx = (float)dx, hy = (float)dy, hz = (float)dz;
+            sh.sj[buf][tid] = make_float4(hx, hy, hz, (float)rm);
+            if (HILO)
+                sh.sjl[HILO ? buf : 0][HILO ? tid : 0] =
+                    make_float4((float)(dx - (double)hx), (float)(dy - (double)hy),
+                                (float)(dz - (double)hz), 0.0f);
+        } else {
+            sh.sj[buf][tid] = make_float4(PAD_COORD, PAD_COORD, PAD_COORD, 0.0f);
+            if (HILO) sh.sjl[HILO ? buf : 0][HILO ? tid : 0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        __syncthreads();
+        {
+            const int64_t j = base + BS + tid;
+            rv = j < j1 && !(skip_src && skip_src[j]);
+            if (rv) { rx = sx[j]; ry = sy[j]; rz = sz[j]; rm = sm[j]; }
+        }
+        const double *tb = tile_box + ((base - j0) / BS) * 8;
+        const bool near = !(tb[0] > ghi0 || tb[3] < glo0 || tb[1] > ghi1 || tb[4] < glo1 ||
+                            tb[2] > ghi2 || tb[5] < glo2);
+        if (near) {
+            if (HILO)
+                tile_f32_hilo<TPT, POT>(sh.sj[buf], sh.sjl[HILO ? buf : 0], xi, yi, zi, xil, yil,
+                                        zil, h2c, h2, hinv, hinv3, ax, ay, az, aq);
+            else
+                tile_f32<TPT, true, POT>(sh.sj[buf], xi, yi, zi, h2c, h2, hinv, hinv3, ax, ay, az,
+                                         aq);
+        } else {
+            tile_f32<TPT, false, POT>(sh.sj[buf], xi, yi, zi, h2c, h2, hinv, hinv3, ax, ay, az, aq);
+        }
+#pragma unroll
+        for (int k = 0; k < TPT; ++k) {
+            Ax[k] += (double)ax[k]; Ay[k] += (double)ay[k]; Az[k] += (double)az[k];
+            ax[k] = ay[k] = az[k] = 0.0f;
+            if (POT) { Aq[k] += (double)aq[k]; aq[k] = 0.0f; }
+        }
+    }
+}
+
+}  // namespace nb

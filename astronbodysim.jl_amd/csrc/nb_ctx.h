@@ -53,6 +53,7 @@ struct nb_ctx {
     int64_t *cidx = nullptr;      // internal indices of the central sources
     nb::SortState sort;
     int64_t force_calls = 0;
+    int64_t calls_since_sort = 0;
     nb_tuning tun{2, 1, 0, 0};
     bool uploaded = false;
     std::vector<nb::BgField> bg;

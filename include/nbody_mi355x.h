@@ -183,6 +183,55 @@ NB_API int nb_tuner_save(const char *path);
 NB_API int nb_tuner_load(const char *path);
 NB_API int nb_tuner_clear(void);
 
+
+/* ---- batched ensembles: many independent systems in one launch ----------- */
+/* No reference counterpart (the reference runs one system per Simulation);
+ * each system follows the same per-system semantics as the one-system API. */
+
+typedef struct nb_batch_tuning {
+    int32_t targets_per_thread; /* 1, 2 or 4 (targets of a work item = 256 x this)   */
+    int32_t blocks;             /* persistent blocks of the ragged kernel (0 = 8 per CU) */
+    int32_t steps_per_graph;    /* KDK steps per captured HIP graph (even, >= 2)     */
+    int32_t sort_interval;      /* re-sort every system into Morton order every k force calls (0 = never) */
+    int32_t resident;           /* 1: multi-step resident kernel when every system has <= 256 particles */
+    int32_t reserved[3];
+} nb_batch_tuning;
+
+/* nsys systems; system s owns particles [offsets[s], offsets[s+1]) of the
+ * packed arrays.  precision_mode: NB_MODE_FP64, NB_MODE_SPLIT or
+ * NB_MODE_SPLIT_CENTRAL. */
+NB_API int nb_batch_create(nb_batch **out, int32_t nsys, const int64_t *offsets,
+                           int32_t precision_mode);
+NB_API int nb_batch_destroy(nb_batch *b);
+/* Per-system G, softening length (all collections) and timestep. */
+NB_API int nb_batch_set_params(nb_batch *b, const double *G, const double *soft, const double *dt);
+NB_API int nb_batch_upload(nb_batch *b, const double *pos, const double *vel, const double *mass);
+NB_API int nb_batch_download(nb_batch *b, double *pos, double *vel, double *acc, double *pot);
+/* First `count` particles of every system are central bodies (FP64 terms in
+ * NB_MODE_SPLIT_CENTRAL). */
+NB_API int nb_batch_set_central(nb_batch *b, int32_t count);
+NB_API int nb_batch_compute_force(nb_batch *b);
+NB_API int nb_batch_compute_potential(nb_batch *b);
+/* Vel += Acc * forward * 0.5 * dt_s for every system s. */
+NB_API int nb_batch_half_kick(nb_batch *b, int32_t forward);
+/* nsteps x (drift dt_s, force, kick dt_s) for every system.  use_graph != 0
+ * replays captured graphs (or the resident kernel, see nb_batch_tuning). */
+NB_API int nb_batch_kdk_steps(nb_batch *b, int64_t nsteps, int32_t use_graph);
+/* As nb_batch_kdk_steps, recording every system's total energy (FP64, with
+ * velocities synchronised to the step: v - a dt/2) after every
+ * `every`-th step into energy[s * (nsteps / every) + k]. */
+NB_API int nb_batch_kdk_steps_tracked(nb_batch *b, int64_t nsteps, int32_t every, double *energy);
+/* Per-system K, W (= 1/2 sum m phi), P[3], L[3]; arrays of nsys / 3*nsys. */
+NB_API int nb_batch_energy(nb_batch *b, double *K, double *W, double *P, double *L);
+NB_API int nb_batch_sync(nb_batch *b);
+NB_API int nb_batch_set_tuning(nb_batch *b, const nb_batch_tuning *t);
+NB_API int nb_batch_get_tuning(nb_batch *b, nb_batch_tuning *t);
+/* Time candidate launch configurations of the ragged force kernel on this
+ * batch and keep the fastest. */
+NB_API int nb_batch_autotune(nb_batch *b);
+NB_API int nb_batch_set_timing(nb_batch *b, int32_t enable);
+NB_API int nb_batch_get_timing(nb_batch *b, int64_t *count, double *total_ms);
+
 #ifdef __cplusplus
 }
 #endif

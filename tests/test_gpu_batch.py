@@ -1,0 +1,171 @@
+"""GPU parity of the batched ensemble engine against the FP64 oracle, system
+by system (needs an MI355X).  Gate: median per-particle force relative error
+<= 1e-6 in the FP32-pair modes, <= 1e-12 in FP64 mode."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GATE = 1e-6
+
+
+@pytest.fixture(scope="module")
+def clusters(pkg):
+    return pkg.ics.cluster_ensemble(nsys=8, seed=5, nmin=200, nmax=3000)
+
+
+def oracle_forces(orc, ens, G=1.0, skip_self=False, pot=False):
+    off = ens["offsets"]
+    acc = np.zeros_like(ens["pos"])
+    phi = np.zeros(len(ens["mass"]))
+    for s in range(len(off) - 1):
+        sl = slice(off[s], off[s + 1])
+        h = ens["h"][s] if "h" in ens else 0.0
+        acc[sl] = orc.compute_force(ens["pos"][sl], ens["mass"][sl], [h] * 6, G, skip_self=skip_self)
+        if pot:
+            phi[sl] = orc.compute_potential(ens["pos"][sl], ens["mass"][sl], [h] * 6, G,
+                                            skip_self=skip_self)
+    return acc, phi
+
+
+def make(pkg, ens, mode="split", dt=1e-3, **tun):
+    e = pkg.Ensemble(ens["offsets"], mode)
+    e.set_params(1.0, ens.get("h", 0.0), dt)
+    e.upload(ens["pos"], ens["vel"], ens["mass"])
+    if tun:
+        e.set_tuning(**tun)
+    return e
+
+
+@pytest.mark.parametrize("mode,tol,tpt", [("split", GATE, 1), ("split", GATE, 2), ("split", GATE, 4),
+                                          ("fp64", 1e-12, 1)])
+def test_ragged_force(pkg, orc, clusters, mode, tol, tpt):
+    ref, _ = oracle_forces(orc, clusters)
+    with make(pkg, clusters, mode, targets_per_thread=tpt, blocks=37, resident=0) as e:
+        e.compute_force()
+        acc = e.download(pos=False, vel=False)["acc"]
+    err = orc.rel_err(acc, ref)
+    off = clusters["offsets"]
+    for s in range(len(off) - 1):
+        assert np.median(err[off[s]:off[s + 1]]) <= tol, s
+
+
+def test_ragged_queue_reused_across_launches(pkg, orc, clusters):
+    ref, _ = oracle_forces(orc, clusters)
+    with make(pkg, clusters, "split", blocks=5, resident=0) as e:
+        for _ in range(4):     # the queue is reset by the last block of each launch
+            e.compute_force()
+        acc = e.download(pos=False, vel=False)["acc"]
+    assert np.median(orc.rel_err(acc, ref)) <= GATE
+
+
+def test_batch_matches_plain_contexts(pkg, orc, clusters):
+    nat = pkg._native
+    with make(pkg, clusters, "split", resident=0) as e:
+        e.compute_force()
+        e.half_kick(+1)
+        e.kdk_steps(8, use_graph=False)
+        d = e.download()
+    off = clusters["offsets"]
+    for s in (0, 3, 7):
+        sl = slice(off[s], off[s + 1])
+        with nat.Context(off[s + 1] - off[s], nat.NB_MODE_SPLIT) as c:
+            c.set_params(1.0, [clusters["h"][s]] * 6)
+            c.upload(clusters["pos"][sl], clusters["vel"][sl], clusters["mass"][sl])
+            c.compute_force()
+            c.half_kick(+1, 1e-3)
+            c.kdk_steps(8, 1e-3, 0.0)
+            p = c.download()
+        assert np.max(np.abs(p["pos"] - d["pos"][sl])) < 1e-9
+        assert np.median(orc.rel_err(p["acc"], d["acc"][sl])) < 1e-6
+
+
+def test_graph_and_eager_steps_identical(pkg, clusters):
+    outs = []
+    for use_graph in (False, True):
+        with make(pkg, clusters, "split", steps_per_graph=4, resident=0, sort_interval=0) as e:
+            e.compute_force()
+            e.half_kick(+1)
+            for _ in range(3):
+                e.kdk_steps(8, use_graph=use_graph)
+                e.sync()
+            outs.append(e.download())
+    assert np.array_equal(outs[0]["pos"], outs[1]["pos"])
+    assert np.array_equal(outs[0]["vel"], outs[1]["vel"])
+
+
+def test_batch_energy(pkg, orc, clusters):
+    ref, phi = oracle_forces(orc, clusters, pot=True)
+    with make(pkg, clusters, "split", resident=0) as e:
+        en = e.energy()
+    off = clusters["offsets"]
+    for s in range(len(off) - 1):
+        sl = slice(off[s], off[s + 1])
+        K, W, P, L = orc.energy(clusters["pos"][sl], clusters["vel"][sl], clusters["mass"][sl],
+                                phi[sl])
+        assert en["K"][s] == pytest.approx(K, rel=1e-12)
+        assert en["W"][s] == pytest.approx(W, rel=1e-6)
+
+
+@pytest.fixture(scope="module")
+def planets(pkg):
+    return pkg.ics.planetary_ensemble(nsys=6, seed=6)
+
+
+def test_planetary_force_central_fp64(pkg, orc, planets):
+    ref, _ = oracle_forces(orc, planets, skip_self=True)
+    for resident in (0, 1):
+        with make(pkg, planets, "split_central", resident=resident) as e:
+            e.set_central(1)
+            e.compute_force()
+            acc = e.download(pos=False, vel=False)["acc"]
+        err = orc.rel_err(acc, ref)
+        assert np.max(err) < 1e-6 and np.median(err) < 1e-9
+
+
+def test_resident_matches_oracle_run(pkg, orc, planets):
+    """Resident multi-step kernel vs the oracle's KDK run loop, 300 steps."""
+    steps, dt = 300, 1e-3
+    with make(pkg, planets, "split_central", dt=dt, resident=1) as e:
+        e.set_central(1)
+        e.compute_force()
+        e.half_kick(+1)
+        e.kdk_steps(steps, use_graph=True)
+        e.half_kick(-1)
+        d = e.download()
+    off = planets["offsets"]
+    for s in range(len(off) - 1):
+        sl = slice(off[s], off[s + 1])
+        xo, vo, _, info = orc.run_const_kdk(planets["pos"][sl], planets["vel"][sl],
+                                            planets["mass"][sl], [0.0] * 6, 1.0, t_begin=0.0,
+                                            t_end=steps * dt, dt=dt, between_snapshots=1.0,
+                                            skip_self=True)
+        assert info["steps"] == steps
+        assert np.max(np.abs(d["pos"][sl] - xo)) < 1e-9
+        assert np.max(np.abs(d["vel"][sl] - vo)) < 1e-8
+
+
+def test_resident_and_ragged_paths_agree(pkg, planets):
+    outs = []
+    for resident in (1, 0):
+        with make(pkg, planets, "split_central", resident=resident) as e:
+            e.set_central(1)
+            e.compute_force()
+            e.half_kick(+1)
+            e.kdk_steps(40, use_graph=True)
+            outs.append(e.download())
+    assert np.max(np.abs(outs[0]["pos"] - outs[1]["pos"])) < 1e-10
+
+
+def test_tracked_energy(pkg, orc, planets):
+    steps, every, dt = 400, 100, 1e-3
+    with make(pkg, planets, "split_central", dt=dt, resident=1) as e:
+        e.set_central(1)
+        e.compute_force()
+        e0 = e.energy()
+        E0 = e0["K"] + e0["W"]
+        e.half_kick(+1)
+        E = e.kdk_steps_tracked(steps, every)
+    assert E.shape == (6, 4)
+    rel = np.abs(E - E0[:, None]) / np.abs(E0[:, None])
+    assert np.all(rel < 1e-4)
