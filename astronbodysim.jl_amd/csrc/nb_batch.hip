@@ -360,7 +360,7 @@ __global__ __launch_bounds__(256) void k_batch_resident(ResArgs a, int nsteps) {
                                     Az, Aq);
                 } else {
                     const float4 v = shi[k], l = slo[k];
-                    pair_f32_hilo<false>(v.x, v.y, v.z, v.w, l.x, l.y, l.z, xih, yih, zih, xil,
+                    pair_f32_hilo<true, false>(v.x, v.y, v.z, v.w, l.x, l.y, l.z, xih, yih, zih, xil,
                                          yil, zil, h2cf, h2f, hinvf, hinv3f, fx, fy, fz, fq);
                 }
             }

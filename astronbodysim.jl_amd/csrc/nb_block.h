@@ -123,7 +123,7 @@ __device__ __forceinline__ void tile_f32(const float4 *__restrict__ s, const flo
     }
 }
 
-template <int TPT, bool POT>
+template <int TPT, bool CHECKED, bool POT>
 __device__ __forceinline__ void tile_f32_hilo(const float4 *__restrict__ s,
                                               const float4 *__restrict__ sl,
                                               const float (&xi)[TPT], const float (&yi)[TPT],
@@ -139,23 +139,23 @@ __device__ __forceinline__ void tile_f32_hilo(const float4 *__restrict__ s,
         const float4 w = sl[jj];
 #pragma unroll
         for (int k = 0; k < TPT; ++k)
-            pair_f32_hilo<POT>(v.x, v.y, v.z, v.w, w.x, w.y, w.z, xi[k], yi[k], zi[k], xil[k],
-                               yil[k], zil[k], h2c[k], h2[k], hinv[k], hinv3[k], ax[k], ay[k],
-                               az[k], aq[k]);
+            pair_f32_hilo<CHECKED, POT>(v.x, v.y, v.z, v.w, w.x, w.y, w.z, xi[k], yi[k], zi[k],
+                                        xil[k], yil[k], zil[k], h2c[k], h2[k], hinv[k], hinv3[k],
+                                        ax[k], ay[k], az[k], aq[k]);
     }
 }
 
-// FP32-pair kernel.  T = double (SPLIT: FP64 state, per-block origin, hi/lo
-// coordinates on near tiles) or float (FP32 state, absolute coordinates).
-//
-// Tile classification (block-uniform): a source tile is "far" when its box is
-// separated from the block's target box by more than G = max(h_max, R) along
-// some axis, where R is the block's radius about its origin (SPLIT) or 0
-// (FP32).  Far tiles run the 12-VALU Newtonian loop on hi coordinates: every
-// pair there has r > h (no softening) and r > R, so the separation carries a
-// relative rounding error below 4 FP32 epsilons.  Near tiles run the
-// softening-checked loop (on hi/lo coordinates in SPLIT).
-
+// Tile classification (block-uniform), from the Euclidean distance d between
+// a source tile's box and the block's target box:
+//   d <= h_max (+ a resolution margin)      -> "soft": softening-checked pairs
+//                                              (hi/lo coordinates in SPLIT);
+//   d <= R/2 (SPLIT only; R = block radius   -> "close": hi/lo coordinates,
+//     about its FP64 origin)                    Newtonian (every pair has r > h);
+//   otherwise                               -> "far": 12-VALU Newtonian loop on
+//                                              hi coordinates.
+// For a far pair the separation x_j - x_i, formed from two FP32 roundings of
+// coordinates of size <= R + r, carries a relative error <= 2 eps (1 + R/r)
+// <= 6 eps since r > R/2.
 // Shared memory of one all-pairs block.
 template <typename T>
 struct BlockShared {
@@ -223,10 +223,9 @@ __device__ __forceinline__ void force_block(
         const double o = d == 0 ? ox : (d == 1 ? oy : oz);
         rb = fmax(rb, fmax(fabs(st.lo[d] - o), fabs(st.hi[d] - o)));
     }
-    const double reach = HILO ? fmax(st.hmax, rb) : st.hmax;
-    const double grow = reach * (1.0 + 1e-6) + 1e-6 * rb + 1e-300;
-    const double glo0 = st.lo[0] - grow, glo1 = st.lo[1] - grow, glo2 = st.lo[2] - grow;
-    const double ghi0 = st.hi[0] + grow, ghi1 = st.hi[1] + grow, ghi2 = st.hi[2] + grow;
+    const double soft_d = st.hmax * (1.0 + 1e-6) + 1e-6 * rb + 1e-300;
+    const double close_d = HILO ? fmax(soft_d, 0.5 * rb) : soft_d;
+    const double soft_d2 = soft_d * soft_d, close_d2 = close_d * close_d;
 
     // Register prefetch of the first tile.
     T rx = 0, ry = 0, rz = 0, rm = 0;
@@ -258,15 +257,20 @@ __device__ __forceinline__ void force_block(
             if (rv) { rx = sx[j]; ry = sy[j]; rz = sz[j]; rm = sm[j]; }
         }
         const double *tb = tile_box + ((base - j0) / BS) * 8;
-        const bool near = !(tb[0] > ghi0 || tb[3] < glo0 || tb[1] > ghi1 || tb[4] < glo1 ||
-                            tb[2] > ghi2 || tb[5] < glo2);
-        if (near) {
+        const double gx = fmax(0.0, fmax(tb[0] - st.hi[0], st.lo[0] - tb[3]));
+        const double gy = fmax(0.0, fmax(tb[1] - st.hi[1], st.lo[1] - tb[4]));
+        const double gz = fmax(0.0, fmax(tb[2] - st.hi[2], st.lo[2] - tb[5]));
+        const double d2 = gx * gx + gy * gy + gz * gz;
+        if (d2 <= soft_d2) {
             if (HILO)
-                tile_f32_hilo<TPT, POT>(sh.sj[buf], sh.sjl[HILO ? buf : 0], xi, yi, zi, xil, yil,
-                                        zil, h2c, h2, hinv, hinv3, ax, ay, az, aq);
+                tile_f32_hilo<TPT, true, POT>(sh.sj[buf], sh.sjl[HILO ? buf : 0], xi, yi, zi, xil,
+                                              yil, zil, h2c, h2, hinv, hinv3, ax, ay, az, aq);
             else
                 tile_f32<TPT, true, POT>(sh.sj[buf], xi, yi, zi, h2c, h2, hinv, hinv3, ax, ay, az,
                                          aq);
+        } else if (HILO && d2 <= close_d2) {
+            tile_f32_hilo<TPT, false, POT>(sh.sj[buf], sh.sjl[HILO ? buf : 0], xi, yi, zi, xil,
+                                           yil, zil, h2c, h2, hinv, hinv3, ax, ay, az, aq);
         } else {
             tile_f32<TPT, false, POT>(sh.sj[buf], xi, yi, zi, h2c, h2, hinv, hinv3, ax, ay, az, aq);
         }

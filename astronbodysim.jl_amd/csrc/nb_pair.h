@@ -89,8 +89,8 @@ __device__ __forceinline__ void pair_f32(float sx, float sy, float sz, float sm,
 // One FP32 pair with hi/lo coordinates: x = hi + lo (both FP32, relative to
 // the block origin).  dx = (xj_hi - xi_hi) + (xj_lo - xi_lo) keeps the
 // relative error of a close pair's separation at ~FP32 epsilon whatever the
-// distance of the pair from the origin.  Always softening-checked.
-template <bool POT>
+// distance of the pair from the origin.
+template <bool CHECKED, bool POT>
 __device__ __forceinline__ void pair_f32_hilo(float sx, float sy, float sz, float sm, float sxl,
                                               float syl, float szl, float xi, float yi, float zi,
                                               float xil, float yil, float zil, float h2c,
@@ -103,7 +103,9 @@ __device__ __forceinline__ void pair_f32_hilo(float sx, float sy, float sz, floa
     float ri = __builtin_amdgcn_rsqf(r2);
     float q = sm * ri;
     float fac = q * (ri * ri);
-    if (__builtin_expect(r2 < h2c, 0)) spline_f32(r2, sm, h2, hinv, hinv3, fac, q);
+    if (CHECKED) {
+        if (__builtin_expect(r2 < h2c, 0)) spline_f32(r2, sm, h2, hinv, hinv3, fac, q);
+    }
     ax = __builtin_fmaf(fac, dx, ax);
     ay = __builtin_fmaf(fac, dy, ay);
     az = __builtin_fmaf(fac, dz, az);

@@ -54,6 +54,165 @@ def traffic_from_profile():
     return None
 
 
+def progress(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
+def bench_clusters(pkg, nat, args):
+    """512 Plummer clusters, N log-uniform in [256, 16384], a ~ U[0.5, 2],
+    h = 0.01 a, 200 KDK steps at dt = 1e-3: one ragged persistent launch per
+    force, graph-captured steps, versus sequential one-system launches."""
+    steps, dt = 200, 1e-3
+    progress("cluster ensemble: generating 512 clusters")
+    ens = pkg.ics.cluster_ensemble(nsys=512, seed=5)
+    off = ens["offsets"]
+    ns = np.diff(off)
+    pairs = float((ns.astype(np.float64) ** 2).sum())
+    e = pkg.Ensemble(off, "split")
+    e.set_params(1.0, ens["h"], dt)
+    e.upload(ens["pos"], ens["vel"], ens["mass"])
+    e.autotune()
+    e.compute_force()
+    acc = e.download(pos=False, vel=False)["acc"]
+    e.half_kick(+1)
+    e.kdk_steps(16)
+    e.sync()
+    t0 = time.perf_counter()
+    e.kdk_steps(steps)
+    e.sync()
+    t_batch = time.perf_counter() - t0
+    tun = e.get_tuning()
+    e.close()
+    # accuracy on a few clusters (oracle, FP64)
+    err = None
+    try:
+        from oracle import oracle as orc
+        es = []
+        for s in (0, 100, 200, 300, 400, 511):
+            sl = slice(off[s], off[s + 1])
+            ref = orc.compute_force(ens["pos"][sl], ens["mass"][sl], [ens["h"][s]] * 6)
+            es.append(orc.rel_err(acc[sl], ref))
+        es = np.concatenate(es)
+        err = {"median": float(np.median(es)), "max": float(es.max()), "clusters": 6}
+    except Exception as exc:  # pragma: no cover - reported, not fatal
+        err = {"error": str(exc)}
+    # sequential plain launches (one context per cluster), measured on a
+    # seeded subset and scaled by the subset's share of the pairs
+    rng = np.random.default_rng(11)
+    subset = np.sort(rng.choice(512, 64, replace=False))
+    sub_steps = 20
+    ctxs = []
+    for s in subset:
+        sl = slice(off[s], off[s + 1])
+        c = nat.Context(int(ns[s]), nat.NB_MODE_SPLIT)
+        c.set_params(1.0, [ens["h"][s]] * 6)
+        c.upload(ens["pos"][sl], ens["vel"][sl], ens["mass"][sl])
+        c.compute_force()
+        c.half_kick(+1, dt)
+        c.kdk_steps(2, dt, 0.0, use_graph=True)
+        c.sync()
+        ctxs.append(c)
+    t0 = time.perf_counter()
+    for c in ctxs:
+        c.kdk_steps(sub_steps, dt, 0.0, use_graph=True)
+        c.sync()
+    t_sub = time.perf_counter() - t0
+    for c in ctxs:
+        c.close()
+    sub_pairs = float((ns[subset].astype(np.float64) ** 2).sum())
+    t_plain = t_sub / sub_steps * steps * (pairs / sub_pairs)
+    return {"workload": "512 Plummer clusters, N logU[256,16384], 200 KDK steps",
+            "n_total": int(off[-1]), "pairs_per_step": pairs, "steps": steps,
+            "batched_s": t_batch, "batched_pairs_per_s": pairs * steps / t_batch,
+            "plain_s_estimate": t_plain, "plain_pairs_per_s": pairs * steps / t_plain,
+            "speedup": t_plain / t_batch, "force_rel_err": err, "tuning": tun,
+            "plain_sample": f"{len(subset)} seeded clusters x {sub_steps} steps, "
+                            "scaled by their share of the pairs"}
+
+
+def bench_planetary(pkg, nat, args):
+    """2048 planetary systems (central mass + 8 planets + 64 test particles),
+    eps = 0, 10^4 KDK steps at dt = 1e-3, energy tracked every 100 steps: the
+    resident multi-step kernel versus sequential one-system launches."""
+    steps, every, dt = 10000, 100, 1e-3
+    progress("planetary ensemble: generating 2048 systems")
+    ens = pkg.ics.planetary_ensemble(nsys=2048, seed=6)
+    off = ens["offsets"]
+    per = ens["per_system"]
+    nmass = 9
+    e = pkg.Ensemble(off, "split_central")
+    e.set_params(1.0, 0.0, dt)
+    e.upload(ens["pos"], ens["vel"], ens["mass"])
+    e.set_central(1)
+    e.compute_force()
+    e0 = e.energy()
+    E0 = e0["K"] + e0["W"]
+    e.half_kick(+1)
+    e.sync()
+    t0 = time.perf_counter()
+    E = e.kdk_steps_tracked(steps, every)
+    e.sync()
+    t_batch = time.perf_counter() - t0
+    e.close()
+    rel = np.abs(E - E0[:, None]) / np.abs(E0[:, None])
+    # sequential plain launches on a seeded subset of systems
+    rng = np.random.default_rng(12)
+    subset = rng.choice(2048, 8, replace=False)
+    sub_steps = 200
+    ctxs = []
+    for s in subset:
+        sl = slice(off[s], off[s + 1])
+        c = nat.Context(per, nat.NB_MODE_SPLIT_CENTRAL)
+        c.set_params(1.0, [0.0] * 6)
+        c.upload(ens["pos"][sl], ens["vel"][sl], ens["mass"][sl])
+        c.set_central(1)
+        c.compute_force()
+        c.half_kick(+1, dt)
+        c.kdk_steps(16, dt, 0.0, use_graph=True)
+        c.sync()
+        ctxs.append(c)
+    t0 = time.perf_counter()
+    for c in ctxs:
+        c.kdk_steps(sub_steps, dt, 0.0, use_graph=True)
+        c.sync()
+    t_sub = time.perf_counter() - t0
+    for c in ctxs:
+        c.close()
+    t_plain = t_sub / (len(subset) * sub_steps) * 2048 * steps
+    allpairs = 2048.0 * per * per
+    evaluated = 2048.0 * per * nmass
+    out = {"workload": "2048 planetary systems x 73 bodies, eps=0, 1e4 KDK steps, "
+                       "energy every 100 steps",
+           "steps": steps, "batched_s": t_batch,
+           "batched_pairs_per_s_allpairs": allpairs * steps / t_batch,
+           "batched_pairs_per_s_evaluated": evaluated * steps / t_batch,
+           "plain_s_estimate": t_plain, "speedup": t_plain / t_batch,
+           "rel_energy_err": {"median": float(np.median(rel)),
+                              "p99": float(np.percentile(rel.max(1), 99)),
+                              "max": float(rel.max()),
+                              "systems_above_1e-3": int((rel.max(1) > 1e-3).sum())},
+           "plain_sample": f"{len(subset)} seeded systems x {sub_steps} steps, scaled"}
+    if not args.no_cpu:
+        # the FP64 oracle integrates the same ensemble: CPU baseline and a
+        # system-by-system check of the tracked energy errors
+        progress("planetary ensemble: FP64 oracle run")
+        from oracle import oracle as orc
+        t0 = time.perf_counter()
+        _, _, E0o, Eo = orc.run_ensemble_kdk(off, ens["pos"], ens["vel"], ens["mass"], 1.0, 0.0,
+                                             dt, nsteps=steps, every=every)
+        t_cpu = time.perf_counter() - t0
+        relo = np.abs(Eo - E0o[:, None]) / np.abs(E0o[:, None])
+        calm = relo.max(1) < 1e-3
+        out["oracle"] = {"cpu_s": t_cpu, "cores": orc.num_threads(),
+                         "rel_energy_err_median": float(np.median(relo)),
+                         "rel_energy_err_max": float(relo.max()),
+                         "systems_above_1e-3": int((~calm).sum()),
+                         "gpu_minus_oracle_energy_err_calm_systems_max":
+                             float(np.max(np.abs(rel[calm] - relo[calm]))),
+                         "evaluated_pairs_per_s": evaluated * steps / t_cpu}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -82,6 +241,7 @@ def main():
         t += dt
     ctx.sync()
 
+    progress(f"N={n}: {args.warmup} warm-up steps done, timing {args.steps} steps")
     ctx.set_timing(True)
     ctx.sync()
     t0 = time.perf_counter()
@@ -101,7 +261,9 @@ def main():
     # accuracy + CPU baseline on a seeded sample of targets against all sources
     cpu = None
     err = None
+    progress(f"timed steps done ({elapsed:.2f} s)")
     if not args.no_cpu:
+        progress("CPU oracle sample")
         from oracle import oracle as orc
         threads = orc.num_threads()
         rng = np.random.default_rng(7)
@@ -125,11 +287,8 @@ def main():
 
     batch = None
     if not args.no_batch:
-        try:
-            from bench_batch import run_batch_comparison  # noqa: WPS433
-            batch = run_batch_comparison(pkg)
-        except ImportError:
-            batch = None
+        batch = {"clusters": bench_clusters(pkg, nat, args),
+                 "planetary": bench_planetary(pkg, nat, args)}
 
     line = {
         "metric": METRIC,

@@ -502,3 +502,82 @@ DS_EXPORT int ds_num_threads(void) {
     return 1;
 #endif
 }
+
+/* ------------------------------------------------------------------------ */
+/* Ensemble of independent systems, each integrated with the constant-dt    */
+/* KDK sequence of run(sim) (src/run.jl:76-112; src/base/Timestep.jl:84-93, */
+/* :116-125, :181-191): force, +1/2 kick, then nsteps x {drift dt, force,   */
+/* kick dt}.  No final -1/2 kick.  Per system s: G[s], softening soft[s]    */
+/* for every particle, timestep dt[s]; the self pair is skipped (h may be   */
+/* 0).  Massless sources are skipped: their terms are exactly +0.0.         */
+/* E0[s] = K + W before the first kick; energy[s*slots + k] = K + W after   */
+/* step (k+1)*every with velocities synchronised to the step (v - a dt/2).  */
+/* Pinned against run_const_kdk (the run-loop restatement) in the tests.    */
+/* ------------------------------------------------------------------------ */
+static void sys_force(int64_t n, const double *x, const double *y, const double *z,
+                      const double *m, double G, double h, double *acc, double *pot) {
+    for (int64_t i = 0; i < n; ++i) {
+        const double p[3] = {x[i], y[i], z[i]};
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, ph = 0.0, t[3];
+        for (int64_t j = 0; j < n; ++j) {
+            if (j == i || m[j] == 0.0) continue;
+            pair_force(p, x[j], y[j], z[j], m[j], G, h, t);
+            a0 += t[0]; a1 += t[1]; a2 += t[2];
+            if (pot) ph += pair_potential(p, x[j], y[j], z[j], m[j], G, h);
+        }
+        acc[3 * i] = a0; acc[3 * i + 1] = a1; acc[3 * i + 2] = a2;
+        if (pot) pot[i] = ph;
+    }
+}
+
+static double sys_energy(int64_t n, const double *x, const double *y, const double *z,
+                         const double *vel, const double *acc, const double *m, double G,
+                         double h, double half_dt, double *scratch) {
+    sys_force(n, x, y, z, m, G, h, scratch, scratch + 3 * n);
+    double K = 0.0, W = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double ux = vel[3 * i] - half_dt * acc[3 * i];
+        const double uy = vel[3 * i + 1] - half_dt * acc[3 * i + 1];
+        const double uz = vel[3 * i + 2] - half_dt * acc[3 * i + 2];
+        K += 0.5 * m[i] * (ux * ux + uy * uy + uz * uz);
+        W += scratch[3 * n + i] * m[i];
+    }
+    return K + 0.5 * W;
+}
+
+DS_EXPORT int ds_run_ensemble_kdk(int32_t nsys, const int64_t *off, double *x, double *y,
+                                  double *z, double *vel, const double *m, const double *G,
+                                  const double *soft, const double *dt, int64_t nsteps,
+                                  int32_t every, double *energy, double *E0, int nthreads) {
+    int nt = set_threads(nthreads);
+    const int64_t slots = every > 0 ? nsteps / every : 0;
+    int err = 0;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+    for (int32_t s = 0; s < nsys; ++s) {
+        const int64_t o = off[s], n = off[s + 1] - off[s];
+        double *X = x + o, *Y = y + o, *Z = z + o, *V = vel + 3 * o;
+        const double *M = m + o;
+        double *acc = (double *)malloc(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
+        double *scr = (double *)malloc(sizeof(double) * 4 * (size_t)(n > 0 ? n : 1));
+        if (!acc || !scr) {
+            free(acc); free(scr);
+#pragma omp atomic write
+            err = -3;
+            continue;
+        }
+        sys_force(n, X, Y, Z, M, G[s], soft[s], acc, NULL);
+        if (E0) E0[s] = sys_energy(n, X, Y, Z, V, acc, M, G[s], soft[s], 0.0, scr);
+        ds_half_kick(n, V, acc, +1, dt[s]);
+        for (int64_t k = 1; k <= nsteps; ++k) {
+            ds_drift(n, X, Y, Z, V, dt[s]);
+            sys_force(n, X, Y, Z, M, G[s], soft[s], acc, NULL);
+            ds_kick(n, V, acc, dt[s]);
+            if (energy && every > 0 && k % every == 0 && k / every <= slots)
+                energy[(int64_t)s * slots + (k / every - 1)] =
+                    sys_energy(n, X, Y, Z, V, acc, M, G[s], soft[s], 0.5 * dt[s], scr);
+        }
+        free(acc);
+        free(scr);
+    }
+    return err;
+}

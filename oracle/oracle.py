@@ -64,6 +64,9 @@ def _lib():
     lib.ds_drift.restype = None
     lib.ds_energy.argtypes = [I64, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D]
     lib.ds_energy.restype = I
+    lib.ds_run_ensemble_kdk.argtypes = [C.c_int32, C.POINTER(C.c_int64), _D, _D, _D, _D, _D, _D,
+                                        _D, _D, C.c_int64, C.c_int32, _D, _D, I]
+    lib.ds_run_ensemble_kdk.restype = I
     lib.ds_num_threads.argtypes = []
     lib.ds_num_threads.restype = I
     return lib
@@ -294,6 +297,29 @@ def kdk_steps(pos, vel, mass, soft, G, *, nsteps, dt, t_begin=0.0, acc=None, col
                             variant=variant, skip_self=skip_self, nthreads=nthreads)
         lib.ds_kick(n, _p(v), _p(acc), float(dt))
     return np.stack([x, y, z], axis=1), v, acc
+
+
+def run_ensemble_kdk(offsets, pos, vel, mass, G, soft, dt, *, nsteps, every=0, nthreads=0):
+    """Independent systems: force, +1/2 kick, nsteps x (drift, force, kick),
+    per-system energies every `every` steps (velocities synchronised to the
+    step).  Returns (pos, vel, E0 (nsys,), energy (nsys, nsteps // every))."""
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    nsys = off.size - 1
+    x, y, z = [np.ascontiguousarray(np.asarray(pos, dtype=np.float64)[:, k]) for k in range(3)]
+    v = np.ascontiguousarray(np.array(vel, dtype=np.float64).reshape(-1, 3))
+    m = np.ascontiguousarray(mass, dtype=np.float64)
+    vec = lambda a: np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), (nsys,)))
+    Gs, hs, dts = vec(G), vec(soft), vec(dt)
+    slots = nsteps // every if every > 0 else 0
+    energy = np.zeros(max(slots, 1) * nsys)
+    E0 = np.zeros(nsys)
+    rc = lib.ds_run_ensemble_kdk(nsys, off.ctypes.data_as(C.POINTER(C.c_int64)), _p(x), _p(y),
+                                 _p(z), _p(v), _p(m), _p(Gs), _p(hs), _p(dts), int(nsteps),
+                                 int(every), _p(energy), _p(E0), int(nthreads))
+    if rc:
+        raise RuntimeError(f"ds_run_ensemble_kdk rc={rc}")
+    return (np.stack([x, y, z], axis=1), v, E0,
+            energy[: slots * nsys].reshape(nsys, slots) if slots else np.zeros((nsys, 0)))
 
 
 def half_kick(vel, acc, forward, dt):

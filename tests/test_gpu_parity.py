@@ -108,12 +108,19 @@ def test_box_probe_points(pkg, orc):
     h = 1e-4
     ref_a = orc.force_at_points(pts, h, pos, mass)
     ref_p = orc.potential_at_points(pts, h, pos, mass)
+    # FP32 pair arithmetic: the error of a sum is bounded relative to the sum
+    # of the magnitudes of its terms (the box centre cancels ~1000:1)
+    mag = np.array([sum(np.linalg.norm(orc.pair_force(p, pos[j], mass[j], 1.0, h))
+                        for j in range(10)) for p in pts])
+    pmag = np.array([sum(abs(orc.pair_potential(p, pos[j], mass[j], 1.0, h))
+                         for j in range(10)) for p in pts])
     for mode, tol in (("fp64", 1e-12), ("split", 1e-6)):
         with make_ctx(pkg, pos, vel, mass, [h] * 6, mode) as ctx:
             a = ctx.force_at_points(pts, h)
             p = ctx.potential_at_points(pts, h)
-        assert np.max(orc.rel_err(a, ref_a)) <= tol, mode
-        assert np.max(orc.rel_err(p, ref_p)) <= tol, mode
+        assert np.all(np.linalg.norm(a - ref_a, axis=1) <= tol * mag), mode
+        assert np.all(np.abs(p - ref_p) <= tol * pmag), mode
+        assert np.median(orc.rel_err(a, ref_a)) <= tol, mode
 
 
 def test_zero_softening_no_nan(pkg, orc):
