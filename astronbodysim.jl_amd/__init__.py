@@ -13,6 +13,7 @@ from .simulation import (
 )
 from . import ics
 from .ensemble import Ensemble
+from . import adaptive  # noqa: F401  (registers the nb_adaptive_* entry points)
 
 __all__ = [
     "NBError", "device_info", "version", "AllPairs", "Tiled", "CPU", "GPU", "DirectSum",

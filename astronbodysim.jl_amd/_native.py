@@ -66,16 +66,7 @@ class Tuning(C.Structure):
         ("targets_per_thread", C.c_int32),
         ("j_segments", C.c_int32),
         ("sort_interval", C.c_int32),
-        ("reserved", C.c_int32),
-    ]
-
-
-class BatchTuning(C.Structure):
-    _fields_ = [
-        ("targets_per_item", C.c_int32),
-        ("persistent_blocks", C.c_int32),
-        ("steps_per_graph", C.c_int32),
-        ("reserved", C.c_int32),
+        ("mfma", C.c_int32),
     ]
 
 
@@ -133,6 +124,12 @@ def _load():
 
 
 lib = _load()
+
+# Launch configurations measured by the autotuner on MI355X, keyed by
+# (precision mode, ceil(log2 N)); nb_create starts from the matching entry.
+DEFAULT_TUNING = Path(__file__).resolve().parent / "tuning" / "mi355x.json"
+if DEFAULT_TUNING.exists():
+    lib.nb_tuner_load(os.fsencode(str(DEFAULT_TUNING)))
 
 
 def exported_symbols():
@@ -325,15 +322,16 @@ class Context:
         call("nb_sync", self.handle)
 
     # -- tuning / timing -----------------------------------------------------
-    def set_tuning(self, targets_per_thread: int, j_segments: int, sort_interval: int = 0):
-        t = Tuning(int(targets_per_thread), int(j_segments), int(sort_interval), 0)
+    def set_tuning(self, targets_per_thread: int, j_segments: int, sort_interval: int = 0,
+                   mfma: int = 0):
+        t = Tuning(int(targets_per_thread), int(j_segments), int(sort_interval), int(mfma))
         call("nb_set_tuning", self.handle, C.byref(t))
 
     def get_tuning(self) -> dict:
         t = Tuning()
         call("nb_get_tuning", self.handle, C.byref(t))
         return {"targets_per_thread": t.targets_per_thread, "j_segments": t.j_segments,
-                "sort_interval": t.sort_interval}
+                "sort_interval": t.sort_interval, "mfma": t.mfma}
 
     def autotune(self):
         call("nb_autotune", self.handle)

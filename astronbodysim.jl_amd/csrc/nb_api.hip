@@ -151,6 +151,16 @@ __global__ void k_gather_central(const double *__restrict__ x, const double *__r
     cm[t] = m[i];
 }
 
+int gather_central(nb_ctx *c) {
+    const int64_t k = c->ncentral;
+    if (k <= 0) return NB_OK;
+    k_gather_central<<<(unsigned)((k + 255) / 256), 256, 0, c->stream>>>(
+        (const double *)c->x, (const double *)c->y, (const double *)c->z, (const double *)c->m,
+        c->cidx, k, c->cx, c->cy, c->cz, c->cm);
+    NB_HIP(hipGetLastError());
+    return NB_OK;
+}
+
 // The all-pairs evaluation for every particle (acc, and potential when asked).
 // Re-sort into Morton order when the order is stale.
 int maybe_sort(nb_ctx *c, int64_t upcoming) {
@@ -191,7 +201,7 @@ int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = tru
         a.skip_src = (c->mode == NB_MODE_SPLIT_CENTRAL && c->ncentral > 0) ? c->central : nullptr;
         a.accumulate = 0;
         if (f64) NB_TRY(launch_force_f64(a, with_pot, s));
-        else NB_TRY(launch_force_split(a, tpt, with_pot, s));
+        else NB_TRY(launch_force_split(a, tpt, with_pot, s, c->tun.mfma != 0));
     } else {
         ForceArgs<float> a{};
         a.tx = (const float *)c->x; a.ty = (const float *)c->y; a.tz = (const float *)c->z;
@@ -214,10 +224,7 @@ int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = tru
     // FP64 central-body terms: every target, sources = the central particles.
     if (c->mode == NB_MODE_SPLIT_CENTRAL && c->ncentral > 0) {
         const int64_t k = c->ncentral;
-        k_gather_central<<<(unsigned)((k + 255) / 256), 256, 0, s>>>(
-            (const double *)c->x, (const double *)c->y, (const double *)c->z,
-            (const double *)c->m, c->cidx, k, c->cx, c->cy, c->cz, c->cm);
-        NB_HIP(hipGetLastError());
+        NB_TRY(gather_central(c));
         Geometry gc = geometry(n, k, 1, 1);
         ForceArgs<double> a{};
         a.tx = (const double *)c->x; a.ty = (const double *)c->y; a.tz = (const double *)c->z;
@@ -269,6 +276,7 @@ int ctx_run_force(nb_ctx *c, bool with_pot, bool allow_timing) {
     return run_force(c, with_pot, allow_timing);
 }
 int ctx_rebuild_central(nb_ctx *c) { return rebuild_central(c); }
+int ctx_gather_central(nb_ctx *c) { return gather_central(c); }
 void ctx_drop_graph(nb_ctx *c) { drop_graph(c); }
 }  // namespace nb
 
@@ -405,6 +413,8 @@ int nb_destroy(nb_ctx *c) {
     dfree(c->tile_box); dfree(c->part); dfree(c->stage); dfree(c->red);
     dfree(c->time); dfree(c->flag); dfree(c->central);
     dfree(c->cx); dfree(c->cy); dfree(c->cz); dfree(c->cm); dfree(c->cidx);
+    dfree(c->ti_end); dfree(c->ti_beg);
+    dfree(c->ad.flags); dfree(c->ad.idx); dfree(c->ad.cnt); dfree(c->ad.buf); dfree(c->ad.tmp);
     sort_free(c);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -815,7 +825,8 @@ int nb_set_tuning(nb_ctx *c, const nb_tuning *t) {
     NB_TRY(check_ctx(c, false));
     if (!t || (t->targets_per_thread != 1 && t->targets_per_thread != 2 &&
                t->targets_per_thread != 4) ||
-        t->j_segments < 1 || t->j_segments > 256 || t->sort_interval < 0) {
+        t->j_segments < 1 || t->j_segments > 256 || t->sort_interval < 0 || t->mfma < 0 ||
+        t->mfma > 1) {
         set_error("invalid tuning");
         return NB_ERR_INVALID;
     }

@@ -156,30 +156,72 @@ __device__ __forceinline__ void tile_f32_hilo(const float4 *__restrict__ s,
 // For a far pair the separation x_j - x_i, formed from two FP32 roundings of
 // coordinates of size <= R + r, carries a relative error <= 2 eps (1 + R/r)
 // <= 6 eps since r > R/2.
+// With MF (SPLIT only) the far tiles use the matrix cores for the sum:
+//   a_i = sum_j m_j r_ij^-3 (x'_j - x'_i) = S_xyz - x'_i S_m,
+//   S = sum_j r_ij^-3 (m_j x'_j, m_j y'_j, m_j z'_j, m_j),
+// one v_mfma_f32_4x4x1f32 per source and wave: 16 blocks of 4 targets x 1
+// source x 4 columns.  The VALU keeps the 8-op r^-3 part of every pair.  In
+// a far tile |x'_i| <= R < 2r and |x'_j| <= R + r < 3r, so each term of S is
+// bounded by a few times the pair's force and the subtraction costs a few eps.
+
 // Shared memory of one all-pairs block.
-template <typename T>
+template <typename T, bool MF = false>
 struct BlockShared {
     static constexpr bool HILO = sizeof(T) == 8;
     float4 sj[2][BS];
     float4 sjl[HILO ? 2 : 1][HILO ? BS : 1];
+    float sB[MF ? 2 : 1][MF ? 4 : 1][MF ? BS : 1];   // MFMA B columns, component-major
     double red[11][NW];
 };
+
+typedef float nb_f4v __attribute__((ext_vector_type(4)));
+
+// Far tile on the matrix cores.  Lane l = 4b + q supplies A = r^-3 of its own
+// target and B = component q of the source; D[v] of lane 4b + q accumulates
+// target 4b + v, component q.  Sources are taken four at a time: the VALU
+// part of the four pairs first (independent, so it interleaves), then four
+// MFMAs alternating between two accumulators (no back-to-back dependence).
+__device__ __forceinline__ void tile_mfma(const float4 *__restrict__ s, const float *__restrict__ sB,
+                                          int q, float xi, float yi, float zi, nb_f4v &D0,
+                                          nb_f4v &D1) {
+    const float *bp = sB + q * BS;
+#pragma unroll 2
+    for (int jj = 0; jj < BS; jj += 4) {
+        const float4 bq = *reinterpret_cast<const float4 *>(bp + jj);
+        float r3[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const float4 v = s[jj + u];
+            const float dx = v.x - xi, dy = v.y - yi, dz = v.z - zi;
+            const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+            const float ri = __builtin_amdgcn_rsqf(r2);
+            r3[u] = (ri * ri) * ri;
+        }
+        D0 = __builtin_amdgcn_mfma_f32_4x4x1f32(r3[0], bq.x, D0, 0, 0, 0);
+        D1 = __builtin_amdgcn_mfma_f32_4x4x1f32(r3[1], bq.y, D1, 0, 0, 0);
+        D0 = __builtin_amdgcn_mfma_f32_4x4x1f32(r3[2], bq.z, D0, 0, 0, 0);
+        D1 = __builtin_amdgcn_mfma_f32_4x4x1f32(r3[3], bq.w, D1, 0, 0, 0);
+    }
+}
 
 // One block's share of an all-pairs sum: targets i0 + k*BS + tid (k < TPT,
 // i < i_end) against the sources [j0, j1).  tile_box points at the box of
 // the BS-source tile that starts at j0 (tiles are BS-aligned from j0).
 // Returns FP64 sums (without G) in Ax..Aq and the target validity.
-template <typename T, int TPT, bool POT>
+template <typename T, int TPT, bool POT, bool MF = false>
 __device__ __forceinline__ void force_block(
-    BlockShared<T> &sh, const T *__restrict__ tx, const T *__restrict__ ty,
+    BlockShared<T, MF> &sh, const T *__restrict__ tx, const T *__restrict__ ty,
     const T *__restrict__ tz, const double *__restrict__ th, double th_const, int64_t i0,
     int64_t i_end, const T *__restrict__ sx, const T *__restrict__ sy, const T *__restrict__ sz,
     const T *__restrict__ sm, int64_t j0, int64_t j1, const double *__restrict__ tile_box,
     const uint8_t *__restrict__ skip_src, bool use_origin, double (&Ax)[TPT], double (&Ay)[TPT],
     double (&Az)[TPT], double (&Aq)[TPT], bool (&valid)[TPT]) {
-    constexpr bool HILO = BlockShared<T>::HILO;
+    constexpr bool HILO = BlockShared<T, MF>::HILO;
+    static_assert(!MF || (HILO && TPT == 1 && !POT), "MFMA far tiles: SPLIT, 1 target/lane, acc only");
     const int tid = threadIdx.x;
     __syncthreads();  // the previous call's readers of sh are done
+    nb_f4v D0 = {0.0f, 0.0f, 0.0f, 0.0f}, D1 = {0.0f, 0.0f, 0.0f, 0.0f};
+    double S[4] = {0.0, 0.0, 0.0, 0.0};
 
     double px[TPT], py[TPT], pz[TPT], ph[TPT];
 #pragma unroll
@@ -246,9 +288,22 @@ __device__ __forceinline__ void force_block(
                 sh.sjl[HILO ? buf : 0][HILO ? tid : 0] =
                     make_float4((float)(dx - (double)hx), (float)(dy - (double)hy),
                                 (float)(dz - (double)hz), 0.0f);
+            if constexpr (MF) {
+                const double md = (double)rm;
+                sh.sB[buf][0][tid] = (float)(md * dx);
+                sh.sB[buf][1][tid] = (float)(md * dy);
+                sh.sB[buf][2][tid] = (float)(md * dz);
+                sh.sB[buf][3][tid] = (float)md;
+            }
         } else {
             sh.sj[buf][tid] = make_float4(PAD_COORD, PAD_COORD, PAD_COORD, 0.0f);
             if (HILO) sh.sjl[HILO ? buf : 0][HILO ? tid : 0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if constexpr (MF) {
+                sh.sB[buf][0][tid] = 0.0f;
+                sh.sB[buf][1][tid] = 0.0f;
+                sh.sB[buf][2][tid] = 0.0f;
+                sh.sB[buf][3][tid] = 0.0f;
+            }
         }
         __syncthreads();
         {
@@ -271,6 +326,14 @@ __device__ __forceinline__ void force_block(
         } else if (HILO && d2 <= close_d2) {
             tile_f32_hilo<TPT, false, POT>(sh.sj[buf], sh.sjl[HILO ? buf : 0], xi, yi, zi, xil,
                                            yil, zil, h2c, h2, hinv, hinv3, ax, ay, az, aq);
+        } else if constexpr (MF) {
+            tile_mfma(sh.sj[buf], &sh.sB[buf][0][0], tid & 3, xi[0], yi[0], zi[0], D0, D1);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                S[v] += (double)D0[v] + (double)D1[v];
+                D0[v] = 0.0f;
+                D1[v] = 0.0f;
+            }
         } else {
             tile_f32<TPT, false, POT>(sh.sj[buf], xi, yi, zi, h2c, h2, hinv, hinv3, ax, ay, az, aq);
         }
@@ -280,6 +343,24 @@ __device__ __forceinline__ void force_block(
             ax[k] = ay[k] = az[k] = 0.0f;
             if (POT) { Aq[k] += (double)aq[k]; aq[k] = 0.0f; }
         }
+    }
+    if constexpr (MF) {
+        // Lane t = 4b + i collects target t's four columns: column c sits in
+        // S[i] of lane 4b + c.
+        const int lane = tid & 63, qb = lane & ~3, i = lane & 3;
+        double col[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const double t = __shfl(S[v], qb + c, 64);
+                if (v == i) col[c] = t;
+            }
+        }
+        const double xd = px[0] - ox, yd = py[0] - oy, zd = pz[0] - oz;
+        Ax[0] += col[0] - xd * col[3];
+        Ay[0] += col[1] - yd * col[3];
+        Az[0] += col[2] - zd * col[3];
     }
 }
 

@@ -23,6 +23,19 @@ struct SortState {
     void *spare = nullptr;     // n * 8 bytes gather target
 };
 
+// Adaptive-timestep state (nb_adaptive.hip).
+struct AdaptState {
+    double time_interval = 0.0;
+    int64_t time_base = 0;
+    double err_tol = 0.0, min_step = 0.0, max_step = 0.0;
+    uint8_t *flags = nullptr;  // active flags
+    int64_t *idx = nullptr;    // compacted active indices
+    int64_t *cnt = nullptr;    // [0] active count, [1] min end step
+    double *buf = nullptr;     // gathered targets and their accelerations (7n doubles)
+    void *tmp = nullptr;       // rocPRIM select storage
+    size_t tmp_bytes = 0;
+};
+
 }  // namespace nb
 
 struct nb_ctx {
@@ -52,6 +65,8 @@ struct nb_ctx {
     double *cx = nullptr, *cy = nullptr, *cz = nullptr, *cm = nullptr;  // compacted central sources
     int64_t *cidx = nullptr;      // internal indices of the central sources
     nb::SortState sort;
+    int64_t *ti_end = nullptr, *ti_beg = nullptr;   // adaptive integer timeline (internal order)
+    nb::AdaptState ad;
     int64_t force_calls = 0;
     int64_t calls_since_sort = 0;
     nb_tuning tun{2, 1, 0, 0};
@@ -99,5 +114,6 @@ int autotune_ctx(nb_ctx *c);
 // nb_api.hip
 int ctx_run_force(nb_ctx *c, bool with_pot, bool allow_timing);
 int ctx_rebuild_central(nb_ctx *c);
+int ctx_gather_central(nb_ctx *c);
 void ctx_drop_graph(nb_ctx *c);
 }  // namespace nb

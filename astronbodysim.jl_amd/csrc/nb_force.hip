@@ -36,9 +36,9 @@ namespace nb {
 // pair there has r > h (no softening) and r > R, so the separation carries a
 // relative rounding error below 4 FP32 epsilons.  Near tiles run the
 // softening-checked loop (on hi/lo coordinates in SPLIT).
-template <typename T, int TPT, bool POT>
+template <typename T, int TPT, bool POT, bool MF = false>
 __global__ __launch_bounds__(BS) void k_force_f32(ForceArgs<T> a) {
-    __shared__ BlockShared<T> sh;
+    __shared__ BlockShared<T, MF> sh;
     const int tid = threadIdx.x;
     const int logical = xcd_remap(blockIdx.x, gridDim.x);
     const int seg = logical / a.nblk_i;
@@ -48,7 +48,7 @@ __global__ __launch_bounds__(BS) void k_force_f32(ForceArgs<T> a) {
     const int64_t j1 = (j0 + a.seg_len < a.ns) ? j0 + a.seg_len : a.ns;
     double Ax[TPT], Ay[TPT], Az[TPT], Aq[TPT];
     bool valid[TPT];
-    force_block<T, TPT, POT>(sh, a.tx, a.ty, a.tz, a.th, a.th_const, i0, a.nt, a.sx, a.sy, a.sz,
+    force_block<T, TPT, POT, MF>(sh, a.tx, a.ty, a.tz, a.th, a.th_const, i0, a.nt, a.sx, a.sy, a.sz,
                              a.sm, j0, j1, a.tile_box + (j0 / BS) * 8, a.skip_src,
                              a.use_origin != 0, Ax, Ay, Az, Aq, valid);
 #pragma unroll
@@ -226,7 +226,13 @@ static int launch_f32_impl(const ForceArgs<T> &a, int tpt, bool pot, hipStream_t
 int launch_force_f32(const ForceArgs<float> &a, int tpt, bool pot, hipStream_t s) {
     return launch_f32_impl<float>(a, tpt, pot, s);
 }
-int launch_force_split(const ForceArgs<double> &a, int tpt, bool pot, hipStream_t s) {
+int launch_force_split(const ForceArgs<double> &a, int tpt, bool pot, hipStream_t s, bool mfma) {
+    if (mfma && tpt == 1 && !pot) {
+        const int64_t nb = (int64_t)a.nblk_i * a.nseg;
+        if (nb <= 0) return NB_OK;
+        k_force_f32<double, 1, false, true><<<(unsigned)nb, BS, 0, s>>>(a);
+        return check_launch("k_force_f32<mfma>");
+    }
     return launch_f32_impl<double>(a, tpt, pot, s);
 }
 

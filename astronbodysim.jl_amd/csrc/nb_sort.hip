@@ -160,6 +160,10 @@ int morton_reorder(nb_ctx *c) {
     NB_TRY(permute(c->pot, idx, n, S.spare, s));
     NB_TRY(permute(c->coll, idx, n, S.spare, s));
     NB_TRY(permute(c->central, idx, n, S.spare, s));
+    if (c->ti_end) {
+        NB_TRY(permute(c->ti_end, idx, n, S.spare, s));
+        NB_TRY(permute(c->ti_beg, idx, n, S.spare, s));
+    }
     // perm_new[i] = perm_old[idx[i]]  (written to the free key buffer half)
     int64_t *np = (int64_t *)S.keys;
     k_compose_perm<<<g, 256, 0, s>>>(c->perm, idx, n, np);

@@ -93,8 +93,11 @@ int autotune_ctx(nb_ctx *c) {
     nb_tuning best = keep;
     float best_ms = 1e30f;
     // A single evaluation is timed per candidate when it is long.
-    for (int ti = 0; ti < 3; ++ti) {
-        const int tpt = tpts[ti];
+    // candidates: (tpt, mfma) in {(1,0), (2,0), (4,0)} plus (1,1) in the SPLIT modes
+    const bool split = c->mode == NB_MODE_SPLIT || c->mode == NB_MODE_SPLIT_CENTRAL;
+    for (int ti = 0; ti < (split ? 4 : 3); ++ti) {
+        const int tpt = ti < 3 ? tpts[ti] : 1;
+        const int mf = ti == 3 ? 1 : 0;
         if (c->mode == NB_MODE_FP64 && tpt != 1) continue;
         const int64_t nblk = (n + (int64_t)BS * tpt - 1) / ((int64_t)BS * tpt);
         for (int si = 0; si < 8; ++si) {
@@ -102,9 +105,11 @@ int autotune_ctx(nb_ctx *c) {
             if (nseg > ntiles) break;
             if (nblk * nseg > (1 << 20)) break;
             if ((double)n * nseg * 32.0 > 4.0e9) break;  // partial-sum buffer bound
+            if (nseg > 1 && nblk * (nseg / 2) >= 8192) break;  // the grid was already full
             if (nblk * nseg < 64 && nseg < ntiles && si + 1 < 8) continue;  // far too few blocks
             c->tun.targets_per_thread = tpt;
             c->tun.j_segments = nseg;
+            c->tun.mfma = mf;
             const int reps = (double)n * (double)n > 1e11 ? 1 : 3;
             float ms = 0.f;
             int rc = time_force(c, reps, &ms);
@@ -152,9 +157,9 @@ extern "C" int nb_tuner_save(const char *path) {
         const nb::TuneEntry &e = nb::g_table[k];
         fprintf(f,
                 "%s\n  {\"mode\": %d, \"log2n\": %d, \"targets_per_thread\": %d, "
-                "\"j_segments\": %d, \"sort_interval\": %d, \"ms\": %.6g}",
+                "\"j_segments\": %d, \"sort_interval\": %d, \"mfma\": %d, \"ms\": %.6g}",
                 k ? "," : "", e.mode, e.log2n, e.t.targets_per_thread, e.t.j_segments,
-                e.t.sort_interval, e.ms);
+                e.t.sort_interval, e.t.mfma, e.ms);
     }
     fprintf(f, "\n]}\n");
     fclose(f);
@@ -202,19 +207,21 @@ extern "C" int nb_tuner_load(const char *path) {
         size_t b = text.find('}', a);
         if (b == std::string::npos) break;
         std::string obj = text.substr(a, b - a + 1);
-        long mode, log2n, tpt, nseg, si = 0;
+        long mode, log2n, tpt, nseg, si = 0, mf = 0;
         if (!json_int(obj, "mode", &mode) || !json_int(obj, "log2n", &log2n) ||
             !json_int(obj, "targets_per_thread", &tpt) || !json_int(obj, "j_segments", &nseg)) {
             nb::set_error("%s: malformed entry", path);
             return NB_ERR_INVALID;
         }
         json_int(obj, "sort_interval", &si);
+        json_int(obj, "mfma", &mf);
         if ((tpt != 1 && tpt != 2 && tpt != 4) || nseg < 1 || nseg > 256 || mode < 0 || mode > 3 ||
-            si < 0) {
+            si < 0 || mf < 0 || mf > 1 || (mf == 1 && tpt != 1)) {
             nb::set_error("%s: entry out of range", path);
             return NB_ERR_INVALID;
         }
-        nb::TuneEntry e{(int)mode, (int)log2n, {(int32_t)tpt, (int32_t)nseg, (int32_t)si, 0}, 0.0};
+        nb::TuneEntry e{(int)mode, (int)log2n, {(int32_t)tpt, (int32_t)nseg, (int32_t)si, (int32_t)mf},
+                        0.0};
         entries.push_back(e);
         pos = b + 1;
     }

@@ -41,6 +41,9 @@ def parse():
                    help="target duration of the CPU-oracle sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-batch", action="store_true")
+    p.add_argument("--no-tune", action="store_true")
+    p.add_argument("--save-tuning", default=None, help="write the autotuner's table here")
+    p.add_argument("--tuning-file", default=None, help="load a tuning table before creating contexts")
     return p.parse_args()
 
 
@@ -222,6 +225,8 @@ def main():
     import __graft_entry__ as ge
     pkg = ge.load_package()
     nat = pkg._native
+    if args.tuning_file:
+        nat.tuner_load(args.tuning_file)
     n = args.n
     h, dt = 0.002, 1e-3
     pos, vel, mass = pkg.ics.collision(n // 2, seeds=(3, 4))
@@ -231,6 +236,12 @@ def main():
     ctx = nat.Context(n, nat.NB_MODE_SPLIT)
     ctx.set_params(1.0, [h] * 6)
     ctx.upload(pos, vel, mass)
+    if not args.no_tune:
+        progress("autotuning the all-pairs launch configuration")
+        ctx.autotune()
+        if args.save_tuning:
+            nat.tuner_save(args.save_tuning)
+    progress(f"tuning {ctx.get_tuning()}")
     # initial force (also the accuracy sample) and the first half kick
     ctx.compute_force()
     acc0 = ctx.download(pos=False, vel=False, acc=True)["acc"]
@@ -312,7 +323,9 @@ def main():
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "frac_unpacked": achieved / PEAK_FP32_UNPACKED_TFLOPS,
                      "traffic": traffic_from_profile(),
-                     "kernel": "k_force_f32<double,TPT>", "flops_per_pair": FLOPS_PER_PAIR,
+                     "kernel": "k_force_f32<double,%d,false,%s>" % (
+                         tuning["targets_per_thread"], "true" if tuning["mfma"] else "false"),
+                     "flops_per_pair": FLOPS_PER_PAIR,
                      "avg_kernel_ms": avg_force_s * 1e3, "launches": nforce},
         "cpu_baseline": cpu,
         "batched": batch,

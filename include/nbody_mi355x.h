@@ -76,7 +76,8 @@ typedef struct nb_tuning {
     int32_t targets_per_thread; /* 1, 2 or 4                                   */
     int32_t j_segments;         /* source-range split (>=1; partials reduced in fixed order) */
     int32_t sort_interval;      /* re-sort into Morton order every k force calls (0 = never) */
-    int32_t reserved;
+    int32_t mfma;               /* 1: far tiles accumulate on the matrix cores (SPLIT modes,
+                                   targets_per_thread 1, acceleration-only evaluations) */
 } nb_tuning;
 
 /* ---- library / device ------------------------------------------------- */
@@ -184,6 +185,37 @@ NB_API int nb_tuner_load(const char *path);
 NB_API int nb_tuner_clear(void);
 
 
+/* ---- adaptive (block, power-of-two) timesteps ----------------------------- */
+/* The integer timeline of the reference's AdaptiveTimestep: TimeInterval =
+ * (End - Begin) / TimeBase; each particle's step is TimeBase / 2^k. */
+
+/* Ti_endstep = get_timestep(Acc, h), Ti_begstep = 0 for every particle (needs
+ * a force evaluation first); replaces init_timesteps  src/base/Timestep.jl:29-44,
+ * src/directsumgpu/timestep.jl:32-47.  NB_ERR_TIMESTEP when dt < MinStep or an
+ * integer step is <= 0 (the reference's error(), src/base/Timestep.jl:9-16). */
+NB_API int nb_adaptive_init(nb_ctx *ctx, double time_interval, int64_t time_base,
+                            double err_tol, double min_step, double max_step);
+/* Vel += Acc * forward * 0.5 * (Ti_endstep - Ti_begstep) * TimeInterval;
+ * replaces leapfrog_half_kick (AdaptiveTimestep)  src/base/Timestep.jl:105-114. */
+NB_API int nb_adaptive_half_kick(nb_ctx *ctx, int32_t forward);
+/* min(TimeBase, min_i Ti_endstep); replaces find_min_endstep
+ * src/base/Timestep.jl:135-144, src/directsumgpu/timestep.jl:103-114. */
+NB_API int nb_adaptive_min_endstep(nb_ctx *ctx, int64_t *min_endstep);
+/* Every particle: Pos += Vel * (to - from) * TimeInterval; replaces
+ * drift_particles_kernel (AdaptiveTimestep)  src/base/Timestep.jl:169-179. */
+NB_API int nb_adaptive_drift(nb_ctx *ctx, int64_t from_int, int64_t to_int);
+/* Accelerations of the active particles (Ti_endstep == t_int) from every
+ * source; others keep theirs.  Replaces compute_local_force's active filter
+ * src/directsum/gravity.jl:94-98. */
+NB_API int nb_adaptive_compute_force(nb_ctx *ctx, int64_t t_int, int64_t *nactive);
+/* Active particles: new integer step, kick from the old mid-step to the new
+ * one, Ti_begstep = Ti_endstep, Ti_endstep += step; replaces
+ * advance_and_find_timestep (AdaptiveTimestep)  src/base/Timestep.jl:47-82,
+ * src/directsumgpu/timestep.jl:122-154. */
+NB_API int nb_adaptive_kick(nb_ctx *ctx, int64_t t_int);
+/* Ti_endstep / Ti_begstep in the caller's order (either may be NULL). */
+NB_API int nb_adaptive_get_steps(nb_ctx *ctx, int64_t *endstep, int64_t *begstep);
+
 /* ---- batched ensembles: many independent systems in one launch ----------- */
 /* No reference counterpart (the reference runs one system per Simulation);
  * each system follows the same per-system semantics as the one-system API. */
@@ -194,7 +226,8 @@ typedef struct nb_batch_tuning {
     int32_t steps_per_graph;    /* KDK steps per captured HIP graph (even, >= 2)     */
     int32_t sort_interval;      /* re-sort every system into Morton order every k force calls (0 = never) */
     int32_t resident;           /* 1: multi-step resident kernel when every system has <= 256 particles */
-    int32_t reserved[3];
+    int32_t mfma;               /* 1: far tiles accumulate on the matrix cores (targets_per_thread 1) */
+    int32_t reserved[2];
 } nb_batch_tuning;
 
 /* nsys systems; system s owns particles [offsets[s], offsets[s+1]) of the

@@ -329,6 +329,115 @@ def half_kick(vel, acc, forward, dt):
     return v
 
 
+# ---------------------------------------------------------------------------
+# Adaptive (block) timestep run loop: src/run.jl:46-184 with AdaptiveTimestep
+#   get_timestep                      src/base/Timestep.jl:1-23
+#   init_timesteps_local              :29-39
+#   advance_and_find_timestep_local   :47-82
+#   leapfrog_half_kick_local          :105-114
+#   find_min_endstep_local            :135-144
+#   drift_particles_kernel            :169-179
+#   find_next_sync_point_and_drift    :224-245
+# ---------------------------------------------------------------------------
+def get_timestep(acc, h, *, err_tol, min_step, max_step, time_interval, time_base):
+    """Integer steps for accelerations acc (n,3) and softening h (n,)."""
+    acc = np.atleast_2d(acc)
+    out = np.empty(acc.shape[0], dtype=np.int64)
+    for i in range(acc.shape[0]):
+        ac = math.sqrt(acc[i, 0] ** 2 + acc[i, 1] ** 2 + acc[i, 2] ** 2)
+        with np.errstate(divide="ignore"):
+            dt = math.sqrt(2 * err_tol * h[i] / ac) if ac > 0 else math.inf
+        if dt > max_step:
+            dt = max_step
+        if dt < min_step:
+            raise RuntimeError(f"Timestep < MinStep = {min_step}, dt = {dt}")
+        ti_step = dt / time_interval
+        if ti_step <= 0:
+            raise RuntimeError(f"Wrong integer timestep: {ti_step}")
+        ti_min = time_base
+        while ti_min > ti_step:
+            ti_min >>= 1
+        out[i] = ti_min
+    return out
+
+
+def run_adaptive(pos, vel, mass, soft, G, *, t_begin, t_end, time_base=1 << 30, err_tol=0.025,
+                 min_step=1e-8, max_step=math.inf, between_snapshots=None, collection=None,
+                 on_output=None, nthreads=0, max_steps=10 ** 7):
+    """Restatement of run(sim) for DirectSum + AdaptiveTimestep + Leapfrog.
+    Returns (pos, vel, acc, info) with info = {"steps", "nactive", "outputs",
+    "ti_end", "ti_beg"}."""
+    pos = np.array(pos, dtype=np.float64).reshape(-1, 3)
+    v = np.ascontiguousarray(np.array(vel, dtype=np.float64).reshape(-1, 3))
+    m = np.ascontiguousarray(mass, dtype=np.float64)
+    n = pos.shape[0]
+    coll = np.ones(n, dtype=np.uint8) if collection is None else np.asarray(collection, np.uint8)
+    table = _table(soft)
+    h = table[coll - 1]
+    interval = (t_end - t_begin) / time_base
+    if between_snapshots is None:
+        between_snapshots = t_end - t_begin
+    bsi = int(math.floor(between_snapshots / interval))
+    x, y, z = [np.ascontiguousarray(pos[:, k]) for k in range(3)]
+    kw = dict(err_tol=err_tol, min_step=min_step, max_step=max_step, time_interval=interval,
+              time_base=time_base)
+    te = np.zeros(n, dtype=np.int64)
+    tb = np.zeros(n, dtype=np.int64)
+    sys_int, next_out = 0, bsi
+    sys_float = float(t_begin)
+    outputs = []
+
+    def force(active):
+        return compute_force(np.stack([x, y, z], axis=1), m, soft, G, coll,
+                             active=active.astype(np.uint8), nthreads=nthreads)
+
+    def drift(dt):
+        lib.ds_drift(n, _p(x), _p(y), _p(z), _p(v), float(dt))
+
+    acc = force(te == sys_int)                               # run.jl:76
+    te = get_timestep(acc, h, **kw)                          # run.jl:78
+    tb[:] = 0
+    v += acc * 1.0 * 0.5 * ((te - tb) * interval)[:, None]   # run.jl:85-87
+    outputs.append(sys_float)
+    steps = 0
+    nactive = []
+    while steps < max_steps:
+        min_end = min(int(te.min()), time_base)
+        while next_out <= min_end:
+            drift((next_out - sys_int) * interval)
+            sys_int = next_out
+            sys_float = t_begin + sys_int * interval
+            outputs.append(sys_float)
+            if on_output is not None:
+                on_output(sys_float, np.stack([x, y, z], axis=1), v.copy())
+            next_out += bsi
+        drift((min_end - sys_int) * interval)
+        sys_int = min_end
+        sys_float = t_begin + sys_int * interval
+        active = te == sys_int
+        nactive.append(int(active.sum()))
+        new = force(active)
+        acc[active] = new[active]
+        for i in np.nonzero(active)[0]:
+            ti_step = int(get_timestep(acc[i:i + 1], h[i:i + 1], **kw)[0])
+            if sys_int == time_base:
+                ti_step = 0
+            if time_base - sys_int < ti_step:
+                ti_step = time_base - sys_int
+            tstart = (tb[i] + te[i]) / 2
+            tend = te[i] + 0.5 * ti_step
+            dt = (tend - tstart) * interval
+            v[i] += acc[i] * dt
+            tb[i] = te[i]
+            te[i] = tb[i] + ti_step
+        steps += 1
+        if sys_float >= t_end:
+            break
+    v += acc * -1.0 * 0.5 * ((te - tb) * interval)[:, None]  # run.jl:171-173
+    return np.stack([x, y, z], axis=1), v, acc, {
+        "steps": steps, "nactive": nactive, "outputs": outputs, "ti_end": te, "ti_beg": tb}
+
+
 def rel_err(a, b):
     """Per-particle |a-b|/|b| for (n,3) vectors or (n,) scalars."""
     a = np.asarray(a)

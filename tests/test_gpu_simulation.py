@@ -1,0 +1,140 @@
+"""End-to-end runs through the Simulation / run mirror on the GPU, compared
+with the oracle's restatement of run(sim) (src/run.jl:46-184) on identical
+seeded inputs (needs an MI355X)."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c1_inputs(pkg):
+    """Config 1: Plummer N=4096, G=M=a=1, eps=0.01, dt=1e-3, 100 KDK steps."""
+    return pkg.ics.plummer(4096, seed=1)
+
+
+@pytest.fixture(scope="module")
+def c1_oracle(orc, c1_inputs):
+    pos, vel, mass = c1_inputs
+    soft = [0.01] * 6
+    x, v, a, info = orc.run_const_kdk(pos, vel, mass, soft, 1.0, t_begin=0.0, t_end=0.1,
+                                      dt=1e-3, between_snapshots=0.1)
+    pot0 = orc.compute_potential(pos, mass, soft)
+    pot1 = orc.compute_potential(x, mass, soft)
+    E0 = sum(orc.energy(pos, vel, mass, pot0)[:2])
+    E1 = sum(orc.energy(x, v, mass, pot1)[:2])
+    return x, v, info, E0, E1
+
+
+# FP32 state: positions are rounded to FP32 at every one of the 100 drifts
+@pytest.mark.parametrize("precision,tol_x", [("fp64", 1e-11), ("split", 2e-8), ("fp32", 1e-5)])
+def test_config1_run_matches_oracle(pkg, orc, c1_inputs, c1_oracle, precision, tol_x):
+    pos, vel, mass = c1_inputs
+    xo, vo, info, E0o, E1o = c1_oracle
+    data = pkg.StarData(pos, Vel=vel, Mass=mass)
+    sim = pkg.Simulation(data, precision=precision, TimeStep=1e-3, TimeEnd=0.1,
+                         TimeBetweenSnapshots=0.1, ForceSofteningTable=[0.01] * 6)
+    E0 = pkg.total_energy(sim)
+    pkg.run(sim)
+    d = pkg.get_all_data(sim)
+    E1 = pkg.total_energy(sim)
+    assert sim.timeinfo.stepcount == info["steps"] == 100
+    # FP32 state rounds positions to FP32 at every drift: compare relative to |x|
+    scale = np.maximum(1.0, np.abs(xo)) if precision == "fp32" else 1.0
+    err = np.max(np.abs(d.Pos - xo) / scale)
+    print(f"{precision}: max |dx| {err:.3e}, dE/E gpu {abs(E1 - E0) / abs(E0):.3e} "
+          f"oracle {abs(E1o - E0o) / abs(E0o):.3e}")
+    assert err < tol_x
+    assert abs(E0 - E0o) / abs(E0o) < (1e-12 if precision == "fp64" else 1e-6)
+    assert abs((E1 - E0) - (E1o - E0o)) / abs(E0o) < (1e-12 if precision == "fp64" else 1e-6)
+    sim.close()
+
+
+def test_snapshots_analysers_and_stop_file(pkg, orc, tmp_path):
+    pos, vel, mass = pkg.ics.plummer(512, seed=3)
+    soft = [0.02] * 6
+    seen = []
+
+    def out(sim):
+        seen.append(sim.timeinfo.system_time_float)
+
+    oracle_times = []
+    orc.run_const_kdk(pos, vel, mass, soft, 1.0, t_begin=0.0, t_end=0.02, dt=3e-3,
+                      between_snapshots=0.0045, on_output=lambda t, p, v: oracle_times.append(t))
+    data = pkg.StarData(pos, Vel=vel, Mass=mass)
+    sim = pkg.Simulation(data, TimeStep=3e-3, TimeEnd=0.02, TimeBetweenSnapshots=0.0045,
+                         ForceSofteningTable=soft, output_func=out, OutputDir=str(tmp_path),
+                         analysers={"K": pkg.total_kinetic})
+    res = pkg.run(sim)
+    assert np.allclose(seen, oracle_times, rtol=0, atol=1e-15)
+    assert res.shape == (orc.count_steps(0.0, 0.02, 3e-3), 1)
+    # stop file: run ends after the current step (src/run.jl:153-157)
+    sim2 = pkg.Simulation(pkg.StarData(pos, Vel=vel, Mass=mass), TimeStep=1e-3, TimeEnd=1.0,
+                          ForceSofteningTable=soft, OutputDir=str(tmp_path),
+                          poststep=[lambda s: open(os.path.join(str(tmp_path), "stop"), "w").close()])
+    pkg.run(sim2)
+    assert sim2.timeinfo.stepcount == 1
+    sim.close()
+    sim2.close()
+
+
+def test_public_api_pieces(pkg, orc):
+    pos, vel, mass = pkg.ics.plummer(800, seed=4)
+    soft = [0.01] * 6
+    sim = pkg.Simulation(pkg.StarData(pos, Vel=vel, Mass=mass), precision="fp64", TimeStep=1e-3,
+                         ForceSofteningTable=soft)
+    pkg.compute_force(sim)
+    d = pkg.get_all_data(sim)
+    assert np.max(orc.rel_err(d.Acc, orc.compute_force(pos, mass, soft))) < 1e-12
+    pkg.compute_potential(sim)      # with the reference's self term
+    d = pkg.get_all_data(sim)
+    ref = orc.compute_potential(pos, mass, soft, self_term=True)
+    assert np.max(np.abs(d.Potential - ref) / np.abs(ref)) < 1e-12
+    pts = np.array([[0.1, 0.2, 0.3], [5.0, 0.0, 0.0]])
+    a = pkg.compute_force(sim, pts, 0.05)
+    assert np.max(orc.rel_err(a, orc.force_at_points(pts, 0.05, pos, mass))) < 1e-12
+    p = pkg.compute_potential(sim, pts[0], 0.05)
+    assert p == pytest.approx(orc.potential_at_points(pts[:1], 0.05, pos, mass)[0], rel=1e-12)
+    pkg.set_softlen(sim, 0.03)
+    assert pkg.softlen(pkg.STAR, sim) == 0.03
+    pkg.compute_force(sim)
+    d = pkg.get_all_data(sim)
+    assert np.max(orc.rel_err(d.Acc, orc.compute_force(pos, mass, [0.03] * 6))) < 1e-12
+    K, W, P, L = orc.energy(pos, vel, mass, orc.compute_potential(pos, mass, [0.03] * 6))
+    assert pkg.total_kinetic(sim) == pytest.approx(K, rel=1e-13)
+    assert pkg.total_potential(sim) == pytest.approx(W, rel=1e-12)
+    assert np.allclose(pkg.total_momentum(sim), P, atol=1e-15)
+    assert np.allclose(pkg.total_angular_momentum(sim), L, rtol=1e-12, atol=1e-15)
+    sim.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_adaptive_run_matches_oracle(pkg, orc, precision):
+    pos, vel, mass = pkg.ics.plummer(256, seed=9)
+    soft = [0.05] * 6
+    kw = dict(t_begin=0.0, t_end=0.05, time_base=1 << 16, err_tol=1e-4)
+    xo, vo, ao, info = orc.run_adaptive(pos, vel, mass, soft, 1.0, **kw)
+    data = pkg.StarData(pos, Vel=vel, Mass=mass)
+    sim = pkg.Simulation(data, precision=precision, TimeStep=0.0, TimeEnd=0.05, TimeBase=1 << 16,
+                         ErrTolTimestep=1e-4, TimeBetweenSnapshots=0.05,
+                         ForceSofteningTable=soft)
+    pkg.run(sim)
+    d = pkg.get_all_data(sim)
+    assert sim._adaptive.nactive_history == info["nactive"][: len(sim._adaptive.nactive_history)]
+    assert np.array_equal(d.Ti_endstep, info["ti_end"]) and np.array_equal(d.Ti_begstep, info["ti_beg"])
+    tol = 1e-12 if precision == "fp64" else 1e-7
+    assert np.max(np.abs(d.Pos - xo)) < tol
+    assert np.max(np.abs(d.Vel - vo)) < 100 * tol
+    sim.close()
+
+
+def test_adaptive_timestep_error_surfaces(pkg):
+    pos, vel, mass = pkg.ics.binary(1.0, 1e-6, 1.0)   # huge acceleration at tiny h
+    sim = pkg.Simulation(pkg.StarData(pos, Vel=vel, Mass=mass), TimeStep=0.0, TimeEnd=1.0,
+                         MinStep=1e-3, ForceSofteningTable=[1e-9] * 6)
+    with pytest.raises(pkg.NBError) as ei:
+        pkg.run(sim)
+    assert "MinStep" in str(ei.value)
+    sim.close()

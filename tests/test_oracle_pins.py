@@ -365,3 +365,40 @@ def test_ensemble_runner_matches_run_loop(pkg, orc):
         pot = orc.compute_potential(xo, ens["mass"][sl], [0.0] * 6, skip_self=True)
         K, W, _, _ = orc.energy(xo, vo - 0.5 * dt * ao, ens["mass"][sl], pot)
         assert E[s, -1] == pytest.approx(K + W, rel=1e-12)
+
+
+def test_adaptive_timestep_rule(orc):
+    """get_timestep: largest TimeBase/2^k not above sqrt(2 eta h/|a|)/TimeInterval
+    (src/base/Timestep.jl:1-23), and the StaticTest timestep check
+    dt_sim < sqrt(2 eta h/|a|) (test/StaticTest.jl:190-198)."""
+    interval = 0.1 / (1 << 30)
+    kw = dict(err_tol=0.025, min_step=1e-8, max_step=math.inf, time_interval=interval,
+              time_base=1 << 30)
+    acc = np.array([[3.0, 4.0, 0.0], [0.0, 0.0, 1e-3]])
+    h = np.array([1e-4, 1e-2])
+    steps = orc.get_timestep(acc, h, **kw)
+    for k in range(2):
+        dt_theory = math.sqrt(2 * 0.025 * h[k] / np.linalg.norm(acc[k]))
+        dt_sim = steps[k] * interval
+        assert dt_sim <= dt_theory
+        assert dt_theory < 2 * dt_sim or steps[k] == 1 << 30   # capped at TimeBase
+        assert steps[k] & (steps[k] - 1) == 0   # a power of two
+    with pytest.raises(RuntimeError):
+        orc.get_timestep(np.array([[1e12, 0, 0]]), np.array([1e-4]), **kw)
+
+
+def test_adaptive_run_binary_energy(pkg, orc):
+    """Adaptive run of an eccentric binary: block steps shrink near pericentre
+    and the energy error stays small (the integrator is symplectic per step)."""
+    pos, vel, mass = pkg.ics.binary(1.0, 1.0, 1.0)
+    vel = vel * 0.7
+    soft = [1e-3] * 6
+    x, v, a, info = orc.run_adaptive(pos, vel, mass, soft, 1.0, t_begin=0.0, t_end=2.0,
+                                     time_base=1 << 20, err_tol=0.002)
+    pot0 = orc.compute_potential(pos, mass, soft)
+    pot1 = orc.compute_potential(x, mass, soft)
+    E0 = sum(orc.energy(pos, vel, mass, pot0)[:2])
+    E1 = sum(orc.energy(x, v, mass, pot1)[:2])
+    assert abs(E1 - E0) / abs(E0) < 1e-3
+    assert info["steps"] > 50 and info["outputs"][-1] == pytest.approx(2.0)
+    assert np.all(info["ti_end"] == 1 << 20)
