@@ -5,7 +5,8 @@ reference's Simulation / run API."""
 from . import _native
 from ._native import NBError, device_info, version
 from .simulation import (
-    AllPairs, BLACKHOLE, BULGE, CPU, DISK, DirectSum, Euler, GAS, GPU, HALO, Leapfrog, STAR,
+    AllPairs, BLACKHOLE, BULGE, CPU, DISK, Constants, DirectSum, Euler, GAS, GPU, HALO, Leapfrog,
+    MOND1983Milgrom, Newton, QUMOND, STAR,
     Simulation, StarData, Tiled, advance_and_find_timestep, compute_force, compute_potential,
     find_next_sync_point_and_drift, get_all_data, get_local_data, init_timesteps,
     leapfrog_half_kick, run, set_softlen, set_softlen_, softlen, step, total_angular_momentum,
@@ -16,7 +17,8 @@ from .ensemble import Ensemble
 from . import adaptive  # noqa: F401  (registers the nb_adaptive_* entry points)
 
 __all__ = [
-    "NBError", "device_info", "version", "AllPairs", "Tiled", "CPU", "GPU", "DirectSum",
+    "NBError", "device_info", "version", "Constants", "Newton", "MOND1983Milgrom", "QUMOND",
+    "AllPairs", "Tiled", "CPU", "GPU", "DirectSum",
     "Leapfrog", "Euler", "GAS", "HALO", "DISK", "BULGE", "STAR", "BLACKHOLE", "Simulation",
     "StarData", "run", "step", "compute_force", "compute_potential", "total_kinetic",
     "total_potential", "total_energy", "total_momentum", "total_angular_momentum",

@@ -96,6 +96,7 @@ _SIGS = {
     "nb_energy": (C.c_int, [_P, _D, _D, _D, _D]),
     "nb_set_central": (C.c_int, [_P, C.c_int64]),
     "nb_add_background": (C.c_int, [_P, C.c_int32, _D]),
+    "nb_set_mond": (C.c_int, [_P, C.c_double, C.c_double]),
     "nb_sync": (C.c_int, [_P]),
     "nb_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "nb_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
@@ -111,6 +112,10 @@ _OPTIONAL = {}
 
 
 def _load():
+    global LIB_PATH
+    override = os.environ.get("NBODY_MI355X_LIB")
+    if override:
+        LIB_PATH = Path(override)
     if not LIB_PATH.exists():
         raise ImportError(
             f"{LIB_PATH} is missing: the HIP extension has not been built "
@@ -317,6 +322,9 @@ class Context:
         if p.size < 5:
             p = np.concatenate([p, np.zeros(5 - p.size)])
         call("nb_add_background", self.handle, int(kind), dptr(p))
+
+    def set_mond(self, acc0: float, nu_index: float = 2.0):
+        call("nb_set_mond", self.handle, float(acc0), float(nu_index))
 
     def sync(self):
         call("nb_sync", self.handle)

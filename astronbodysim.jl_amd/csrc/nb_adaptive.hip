@@ -380,6 +380,11 @@ int nb_adaptive_compute_force(nb_ctx *c, int64_t t_int, int64_t *nactive) {
         a.ax = gax; a.ay = gay; a.az = gaz; a.pot = nullptr;
         NB_TRY(launch_force_f32(a, 1, false, s));
     }
+    // post-processing of the active targets, in the order of
+    // src/base/PostProcessing/Force.jl:33-46: background fields, then MOND
+    for (const BgField &f : c->bg)
+        NB_TRY(background(gx, gy, gz, na, c->is_double, f, gax, gay, gaz, nullptr, s));
+    if (c->mond_acc0 > 0.0) NB_TRY(mond(gax, gay, gaz, na, c->mond_acc0, c->mond_index, s));
     k_scatter_acc<<<gridn(na), 256, 0, s>>>(gax, gay, gaz, c->ad.idx, c->ad.cnt, c->ax, c->ay, c->az);
     NB_TRY(launch_ok("k_scatter_acc"));
     return NB_OK;

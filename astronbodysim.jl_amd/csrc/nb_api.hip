@@ -240,6 +240,7 @@ int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = tru
     for (const BgField &f : c->bg)
         NB_TRY(background(c->x, c->y, c->z, n, c->is_double, f, c->ax, c->ay, c->az,
                           with_pot ? c->pot : nullptr, s));
+    if (c->mond_acc0 > 0.0) NB_TRY(mond(c->ax, c->ay, c->az, n, c->mond_acc0, c->mond_index, s));
     return NB_OK;
 }
 
@@ -812,6 +813,18 @@ int nb_add_background(nb_ctx *c, int32_t kind, const double *params) {
     }
     drop_graph(c);
     c->bg.push_back(f);
+    return NB_OK;
+}
+
+int nb_set_mond(nb_ctx *c, double acc0, double nu_index) {
+    NB_TRY(check_ctx(c, false));
+    if (!isfinite(acc0) || !(nu_index > 0.0) || !isfinite(nu_index)) {
+        set_error("invalid MOND parameters (ACC0=%g, nu index=%g)", acc0, nu_index);
+        return NB_ERR_INVALID;
+    }
+    drop_graph(c);
+    c->mond_acc0 = acc0 > 0.0 ? acc0 : 0.0;
+    c->mond_index = nu_index;
     return NB_OK;
 }
 

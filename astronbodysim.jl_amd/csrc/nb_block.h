@@ -145,12 +145,13 @@ __device__ __forceinline__ void tile_f32_hilo(const float4 *__restrict__ s,
     }
 }
 
-// Tile classification (block-uniform), from the Euclidean distance d between
-// a source tile's box and the block's target box:
+// Tile classification (wave-uniform), from the Euclidean distance d between
+// a source tile's box and the box of the wave's targets:
 //   d <= h_max (+ a resolution margin)      -> "soft": softening-checked pairs
 //                                              (hi/lo coordinates in SPLIT);
-//   d <= R/2 (SPLIT only; R = block radius   -> "close": hi/lo coordinates,
-//     about its FP64 origin)                    Newtonian (every pair has r > h);
+//   d <= R/2 (SPLIT only; R = largest         -> "close": hi/lo coordinates,
+//     distance of the wave's targets from       Newtonian (every pair has r > h);
+//     the block's FP64 origin)
 //   otherwise                               -> "far": 12-VALU Newtonian loop on
 //                                              hi coordinates.
 // For a far pair the separation x_j - x_i, formed from two FP32 roundings of
@@ -257,15 +258,37 @@ __device__ __forceinline__ void force_block(
         ax[k] = ay[k] = az[k] = aq[k] = 0.0f;
         Ax[k] = Ay[k] = Az[k] = Aq[k] = 0.0;
     }
-    // Target box grown by the classification distance plus a margin that
-    // keeps every far pair at FP32-resolvable, nonzero separation.
+    // Tiles are classified per wave (the branch is wave-uniform): each wave
+    // uses the box of its own targets, their largest h, and their largest
+    // distance R from the block origin.  A margin keeps every far pair at
+    // FP32-resolvable, nonzero separation.
+    double wlo[3], whi[3], whmax = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        wlo[d] = DBL_MAX;
+        whi[d] = -DBL_MAX;
+    }
+#pragma unroll
+    for (int k = 0; k < TPT; ++k) {
+        if (!valid[k]) continue;
+        wlo[0] = fmin(wlo[0], px[k]); whi[0] = fmax(whi[0], px[k]);
+        wlo[1] = fmin(wlo[1], py[k]); whi[1] = fmax(whi[1], py[k]);
+        wlo[2] = fmin(wlo[2], pz[k]); whi[2] = fmax(whi[2], pz[k]);
+        whmax = fmax(whmax, ph[k]);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        wlo[d] = wave_min(wlo[d]);
+        whi[d] = wave_max(whi[d]);
+    }
+    whmax = wave_max(whmax);
     double rb = 0.0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         const double o = d == 0 ? ox : (d == 1 ? oy : oz);
-        rb = fmax(rb, fmax(fabs(st.lo[d] - o), fabs(st.hi[d] - o)));
+        if (whi[d] >= wlo[d]) rb = fmax(rb, fmax(fabs(wlo[d] - o), fabs(whi[d] - o)));
     }
-    const double soft_d = st.hmax * (1.0 + 1e-6) + 1e-6 * rb + 1e-300;
+    const double soft_d = whmax * (1.0 + 1e-6) + 1e-6 * rb + 1e-300;
     const double close_d = HILO ? fmax(soft_d, 0.5 * rb) : soft_d;
     const double soft_d2 = soft_d * soft_d, close_d2 = close_d * close_d;
 
@@ -312,9 +335,9 @@ __device__ __forceinline__ void force_block(
             if (rv) { rx = sx[j]; ry = sy[j]; rz = sz[j]; rm = sm[j]; }
         }
         const double *tb = tile_box + ((base - j0) / BS) * 8;
-        const double gx = fmax(0.0, fmax(tb[0] - st.hi[0], st.lo[0] - tb[3]));
-        const double gy = fmax(0.0, fmax(tb[1] - st.hi[1], st.lo[1] - tb[4]));
-        const double gz = fmax(0.0, fmax(tb[2] - st.hi[2], st.lo[2] - tb[5]));
+        const double gx = fmax(0.0, fmax(tb[0] - whi[0], wlo[0] - tb[3]));
+        const double gy = fmax(0.0, fmax(tb[1] - whi[1], wlo[1] - tb[4]));
+        const double gz = fmax(0.0, fmax(tb[2] - whi[2], wlo[2] - tb[5]));
         const double d2 = gx * gx + gy * gy + gz * gz;
         if (d2 <= soft_d2) {
             if (HILO)

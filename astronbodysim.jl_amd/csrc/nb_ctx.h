@@ -72,6 +72,8 @@ struct nb_ctx {
     nb_tuning tun{2, 1, 0, 0};
     bool uploaded = false;
     std::vector<nb::BgField> bg;
+    double mond_acc0 = 0.0;       // > 0: Milgrom-1983 rescaling after the background fields
+    double mond_index = 2.0;
     // timing
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;

@@ -31,6 +31,7 @@ int drift(void *x, void *y, void *z, const void *vx, const void *vy, const void 
 int set_time(double *time, int slot, double t, hipStream_t s);
 int background(const void *x, const void *y, const void *z, int64_t n, bool is_double,
                const BgField &f, double *ax, double *ay, double *az, double *pot, hipStream_t s);
+int mond(double *ax, double *ay, double *az, int64_t n, double acc0, double nidx, hipStream_t s);
 int self_term(const void *m, const double *h, int64_t n, double G, bool is_double, double *pot,
               hipStream_t s);
 int energy_reduce(const void *x, const void *y, const void *z, const void *vx, const void *vy,

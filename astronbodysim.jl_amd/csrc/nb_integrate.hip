@@ -222,6 +222,29 @@ int background(const void *x, const void *y, const void *z, int64_t n, bool is_d
     return check_launch("k_background");
 }
 
+// ---- MOND (Milgrom 1983) rescaling --------------------------------------------
+// Acc <- Acc * nu(|Acc| / ACC0, n), nu(y, n) = (0.5 (1 + sqrt(1 + 4 y^-n)))^(1/n)
+// (src/mond/milgrom1983.jl:6-8,33-47).  A zero acceleration stays zero (the
+// reference's formula gives 0 * Inf there).
+__global__ void k_mond(double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
+                       int64_t n, double acc0, double nidx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double a = sqrt(ax[i] * ax[i] + ay[i] * ay[i] + az[i] * az[i]);
+    if (a == 0.0) return;
+    const double y = a / acc0;
+    const double nu = pow(0.5 * (1.0 + sqrt(1.0 + 4.0 * pow(y, -nidx))), 1.0 / nidx);
+    ax[i] *= nu;
+    ay[i] *= nu;
+    az[i] *= nu;
+}
+
+int mond(double *ax, double *ay, double *az, int64_t n, double acc0, double nidx, hipStream_t s) {
+    if (n <= 0) return NB_OK;
+    k_mond<<<grid_for(n), 256, 0, s>>>(ax, ay, az, n, acc0, nidx);
+    return check_launch("k_mond");
+}
+
 // ---- self term of the public compute_potential(sim) -----------------------
 // Potential[i] += G * m_i / (h_i / 2.8)   (src/base/Potential.jl:5-14)
 template <typename T>

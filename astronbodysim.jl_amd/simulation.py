@@ -110,9 +110,29 @@ class TimeConfig:
     algorithm: TimeIntegration
 
 
+class GravityModel:
+    """Gravity-model traits (src/base/Config.jl:110)."""
+
+
+class Newton(GravityModel):
+    pass
+
+
+class MOND1983Milgrom(GravityModel):
+    """Newtonian forces rescaled by nu(|a|/ACC0) after every force evaluation
+    (src/mond/milgrom1983.jl:33-47, src/base/PostProcessing/Force.jl:41-44)."""
+
+
+class QUMOND(GravityModel):
+    """Mesh (FDM/FFT) MOND: needs the reference's mesh solvers, which this
+    direct-summation library does not provide."""
+
+
 @dataclass
 class GravityConfig:
     ForceSofteningTable: np.ndarray
+    model: GravityModel = None
+    MOND_nuIndex: float = 2.0
 
 
 @dataclass
@@ -126,6 +146,7 @@ class DeviceConfig:
 @dataclass
 class Constants:
     G: float
+    ACC0: float = 1.2e-10    # MOND acceleration scale, in the caller's units
 
 
 @dataclass
@@ -200,6 +221,7 @@ class Simulation:
                  MinStep: float = 1.0e-8, MaxStep: float = math.inf,
                  TimeIntegrationAlgorithm: TimeIntegration = None,
                  ForceSofteningTable=None, G: float = 1.0, constants: Constants = None,
+                 model: GravityModel = None, MOND_nuIndex: float = 2.0,
                  OutputDir: str = "output", output_func: Optional[Callable] = None,
                  pot: bool = False, SaveRestart: bool = False, SaveRestartFreq: float = 0.1,
                  analysers: Optional[Dict[str, Callable]] = None,
@@ -220,6 +242,10 @@ class Simulation:
         if precision not in nat.MODES:
             raise ValueError(f"unknown precision {precision!r}; one of {sorted(nat.MODES)}")
         G = constants.G if constants is not None else G
+        acc0 = constants.ACC0 if constants is not None else Constants(G).ACC0
+        model = Newton() if model is None else model
+        if isinstance(model, QUMOND):
+            raise NotImplementedError("QUMOND is solved on a mesh; this library serves direct summation")
         table = (np.full(6, 1.0e-4) if ForceSofteningTable is None
                  else np.array(np.broadcast_to(np.asarray(ForceSofteningTable, dtype=np.float64), (6,))))
         # TimeConfig  src/base/Config.jl:408-449
@@ -237,9 +263,9 @@ class Simulation:
         self.config = SimConfig(
             time=TimeConfig(float(TimeBegin), float(TimeEnd), float(TimeBetweenSnapshots), bsi,
                             step_cfg, algo),
-            grav=GravityConfig(table),
+            grav=GravityConfig(table, model, float(MOND_nuIndex)),
             device=DeviceConfig(device, int(GPU_NumThreads), GPU_Algorithm, precision),
-            constants=Constants(float(G)),
+            constants=Constants(float(G), float(acc0)),
             output=OutputConfig(OutputDir, output_func, bool(pot), bool(SaveRestart),
                                 float(SaveRestartFreq)),
             floattype=floattype,
@@ -260,6 +286,8 @@ class Simulation:
         self.ctx.upload(data.Pos, data.Vel, data.Mass, data.Collection)
         for f in self.bgforce:
             self.ctx.add_background(*f)
+        if isinstance(model, MOND1983Milgrom):
+            self.ctx.set_mond(acc0, MOND_nuIndex)
         self._adaptive = None
         if isinstance(step_cfg, AdaptiveTimestep):
             from .adaptive import AdaptiveState

@@ -165,6 +165,12 @@ NB_API int nb_set_central(nb_ctx *ctx, int64_t count);
  * params = {GM, cx, cy, cz, r_g}  (src/tools/BlackHole.jl:48-60). */
 NB_API int nb_add_background(nb_ctx *ctx, int32_t kind, const double *params);
 
+/* MOND: after the background fields, Acc <- Acc * nu(|Acc|/acc0, nu_index) with
+ * nu(y, n) = (0.5 (1 + sqrt(1 + 4 y^-n)))^(1/n); replaces mond_Milgrom1983
+ * src/mond/milgrom1983.jl:6-8,33-47 and its GPU kernel
+ * src/directsumgpu/gravity.jl:55-62,74-78.  acc0 <= 0 switches it off. */
+NB_API int nb_set_mond(nb_ctx *ctx, double acc0, double nu_index);
+
 /* ---- synchronisation, tuning, timing ------------------------------------ */
 
 NB_API int nb_sync(nb_ctx *ctx);

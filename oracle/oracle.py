@@ -186,6 +186,24 @@ def background_point_mass(pos, GM, center, acc=None, pot=None, G=1.0):
                                  _p(acc), _p(pot))
 
 
+def mond_nu(y, n=2.0):
+    """Generalised MOND interpolation nu(y, n) (src/mond/milgrom1983.jl:6-8)."""
+    y = np.asarray(y, dtype=np.float64)
+    return (0.5 * (1.0 + np.sqrt(1.0 + 4.0 * y ** (-n)))) ** (1.0 / n)
+
+
+def mond(acc, acc0, n=2.0):
+    """Milgrom-1983 rescaling Acc * nu(|Acc|/ACC0, n) of every row of `acc`
+    (src/mond/milgrom1983.jl:33-47, all particles active).  A zero row stays
+    zero."""
+    acc = np.asarray(acc, dtype=np.float64)
+    g = np.sqrt(np.sum(acc * acc, axis=1))
+    out = acc.copy()
+    nz = g > 0
+    out[nz] *= mond_nu(g[nz] / acc0, n)[:, None]
+    return out
+
+
 def energy(pos, vel, mass, pot=None):
     """a14: (K, W, P, L) with W = 0.5 * sum(m * phi) as total_potential(sim)."""
     x, y, z = _soa(pos)
@@ -363,8 +381,10 @@ def get_timestep(acc, h, *, err_tol, min_step, max_step, time_interval, time_bas
 
 def run_adaptive(pos, vel, mass, soft, G, *, t_begin, t_end, time_base=1 << 30, err_tol=0.025,
                  min_step=1e-8, max_step=math.inf, between_snapshots=None, collection=None,
-                 on_output=None, nthreads=0, max_steps=10 ** 7):
+                 on_output=None, nthreads=0, max_steps=10 ** 7, post=None):
     """Restatement of run(sim) for DirectSum + AdaptiveTimestep + Leapfrog.
+    `post(pos, acc)` (optional) returns the post-processed accelerations
+    (src/base/PostProcessing/Force.jl:33-46) of the rows it is given.
     Returns (pos, vel, acc, info) with info = {"steps", "nactive", "outputs",
     "ti_end", "ti_beg"}."""
     pos = np.array(pos, dtype=np.float64).reshape(-1, 3)
@@ -388,8 +408,11 @@ def run_adaptive(pos, vel, mass, soft, G, *, t_begin, t_end, time_base=1 << 30, 
     outputs = []
 
     def force(active):
-        return compute_force(np.stack([x, y, z], axis=1), m, soft, G, coll,
-                             active=active.astype(np.uint8), nthreads=nthreads)
+        p = np.stack([x, y, z], axis=1)
+        a = compute_force(p, m, soft, G, coll, active=active.astype(np.uint8), nthreads=nthreads)
+        if post is not None:
+            a[active] = post(p[active], a[active])
+        return a
 
     def drift(dt):
         lib.ds_drift(n, _p(x), _p(y), _p(z), _p(v), float(dt))

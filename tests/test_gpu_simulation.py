@@ -138,3 +138,46 @@ def test_adaptive_timestep_error_surfaces(pkg):
         pkg.run(sim)
     assert "MinStep" in str(ei.value)
     sim.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_mond_postprocessing(pkg, orc, precision):
+    """MOND1983Milgrom: Newtonian force, background field, then nu(|a|/ACC0)
+    (src/base/PostProcessing/Force.jl:33-46)."""
+    pos, vel, mass = pkg.ics.plummer(1024, seed=21)
+    soft = [0.01] * 6
+    acc0, nidx = 0.3, 1.5
+    bg = (1, [0.5, 2.0, 0.0, 0.0])
+    sim = pkg.Simulation(pkg.StarData(pos, Vel=vel, Mass=mass), precision=precision,
+                         TimeStep=1e-3, ForceSofteningTable=soft, model=pkg.MOND1983Milgrom(),
+                         constants=pkg.Constants(1.0, acc0), MOND_nuIndex=nidx, bgforce=[bg])
+    pkg.compute_force(sim)
+    a = pkg.get_all_data(sim).Acc
+    ref = orc.compute_force(pos, mass, soft)
+    orc.background_point_mass(pos, 0.5, np.array([2.0, 0.0, 0.0]), acc=ref)
+    ref = orc.mond(ref, acc0, nidx)
+    tol = 1e-12 if precision == "fp64" else 1e-6
+    assert np.max(orc.rel_err(a, ref)) < tol
+    sim.close()
+
+
+def test_mond_adaptive_active_subset(pkg, orc):
+    """MOND on the active targets of adaptive steps (src/mond/milgrom1983.jl:40-44)."""
+    pos, vel, mass = pkg.ics.plummer(256, seed=23)
+    soft = [0.05] * 6
+    acc0, nidx = 0.5, 2.0
+    kw = dict(t_begin=0.0, t_end=0.03, time_base=1 << 16, err_tol=1e-4)
+    xo, vo, ao, info = orc.run_adaptive(pos, vel, mass, soft, 1.0,
+                                        post=lambda p, a: orc.mond(a, acc0, nidx), **kw)
+    assert len(set(info["nactive"])) > 1          # some steps move a strict subset
+    sim = pkg.Simulation(pkg.StarData(pos, Vel=vel, Mass=mass), precision="fp64", TimeStep=0.0,
+                         TimeEnd=0.03, TimeBase=1 << 16, ErrTolTimestep=1e-4,
+                         TimeBetweenSnapshots=0.03, ForceSofteningTable=soft,
+                         model=pkg.MOND1983Milgrom(), constants=pkg.Constants(1.0, acc0),
+                         MOND_nuIndex=nidx)
+    pkg.run(sim)
+    d = pkg.get_all_data(sim)
+    assert sim._adaptive.nactive_history == info["nactive"][: len(sim._adaptive.nactive_history)]
+    assert np.array_equal(d.Ti_endstep, info["ti_end"])
+    assert np.max(np.abs(d.Pos - xo)) < 1e-12
+    sim.close()

@@ -402,3 +402,30 @@ def test_adaptive_run_binary_energy(pkg, orc):
     assert abs(E1 - E0) / abs(E0) < 1e-3
     assert info["steps"] > 50 and info["outputs"][-1] == pytest.approx(2.0)
     assert np.all(info["ti_end"] == 1 << 20)
+
+
+def test_mond_nu_inverts_the_mu_functions(orc):
+    """g = nu(gN/a0) gN must satisfy Milgrom's law gN = mu(g/a0) g with the
+    'simple' mu(x) = x/(1+x) for index 1 and the 'standard' mu(x) = x/sqrt(1+x^2)
+    for index 2 (the closed forms src/mond/milgrom1983.jl:13-26 name)."""
+    gN = np.logspace(-6, 6, 200)
+    for n, mu in ((1.0, lambda x: x / (1.0 + x)), (2.0, lambda x: x / np.sqrt(1.0 + x * x))):
+        g = orc.mond_nu(gN, n) * gN
+        assert np.allclose(mu(g) * g, gN, rtol=1e-12, atol=0)
+    # Newtonian limit nu -> 1 and deep-MOND limit nu -> y^-1/2
+    assert orc.mond_nu(1e12, 2.0) == pytest.approx(1.0, abs=1e-12)
+    assert orc.mond_nu(1e-12, 3.0) * 1e-6 == pytest.approx(1.0, rel=1e-5)
+
+
+def test_mond_point_mass_flat_rotation(orc):
+    """Deep-MOND point mass: v_c^4 = G M a0 independently of r."""
+    G, M, a0 = 1.0, 1.0, 1e-2
+    r = np.array([1e3, 1e4, 1e5])
+    pos = np.zeros((4, 3))
+    pos[1:, 0] = r
+    mass = np.array([M, 0.0, 0.0, 0.0])
+    acc = orc.compute_force(pos, mass, [1e-6] * 6, G)
+    am = orc.mond(acc, a0, 2.0)
+    assert np.all(am[0] == 0.0)                  # massless sources: the centre feels nothing
+    v4 = (np.linalg.norm(am[1:], axis=1) * r) ** 2
+    assert np.allclose(v4, G * M * a0, rtol=1e-3)
