@@ -13,11 +13,15 @@ from .simulation import (
     total_energy, total_kinetic, total_momentum, total_potential,
 )
 from . import ics
+from .restart import loadrestart, saverestart
+from .simulation import output, restart
+from .snapshot import HeaderGadget2, read_gadget2, write_gadget2
 from .ensemble import Ensemble
 from . import adaptive  # noqa: F401  (registers the nb_adaptive_* entry points)
 
 __all__ = [
-    "NBError", "device_info", "version", "Constants", "Newton", "MOND1983Milgrom", "QUMOND",
+    "NBError", "device_info", "version", "saverestart", "loadrestart", "restart", "output",
+    "HeaderGadget2", "read_gadget2", "write_gadget2", "Constants", "Newton", "MOND1983Milgrom", "QUMOND",
     "AllPairs", "Tiled", "CPU", "GPU", "DirectSum",
     "Leapfrog", "Euler", "GAS", "HALO", "DISK", "BULGE", "STAR", "BLACKHOLE", "Simulation",
     "StarData", "run", "step", "compute_force", "compute_potential", "total_kinetic",

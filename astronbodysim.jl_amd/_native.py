@@ -85,6 +85,9 @@ _SIGS = {
     "nb_set_params": (C.c_int, [_P, C.c_double, _D]),
     "nb_upload": (C.c_int, [_P, _D, _D, _D, _U8]),
     "nb_download": (C.c_int, [_P, _D, _D, _D, _D]),
+    "nb_upload_acc": (C.c_int, [_P, _D, _D]),
+    "nb_snapshot_begin": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "nb_snapshot_fetch": (C.c_int, [_P, C.c_int32, _D, _D, _D, _D]),
     "nb_compute_force": (C.c_int, [_P]),
     "nb_compute_potential": (C.c_int, [_P, C.c_int32]),
     "nb_force_at_points": (C.c_int, [_P, C.c_int64, _D, C.c_double, _D]),
@@ -271,6 +274,32 @@ class Context:
             if v is not None:
                 out[k] = v
         return out
+
+    def upload_acc(self, acc=None, pot=None):
+        """Set Acc (n,3) and/or Potential (n,) in the caller's order."""
+        n = self.n
+        a = None if acc is None else np.ascontiguousarray(acc, dtype=np.float64).reshape(n, 3)
+        p = None if pot is None else np.ascontiguousarray(pot, dtype=np.float64).reshape(n)
+        call("nb_upload_acc", self.handle, dptr(a), dptr(p))
+
+    SNAP_FIELDS = {"pos": 1, "vel": 2, "acc": 4, "pot": 8}
+
+    def snapshot_begin(self, slot: int, fields=("pos", "vel")):
+        """Enqueue an asynchronous copy of `fields` into pinned slot `slot`."""
+        mask = 0
+        for f in fields:
+            mask |= self.SNAP_FIELDS[f]
+        call("nb_snapshot_begin", self.handle, int(slot), mask)
+
+    def snapshot_fetch(self, slot: int, fields=("pos", "vel")):
+        """Wait for slot `slot` and return its fields (caller order).  Safe to
+        call from another thread while this context keeps working."""
+        n = self.n
+        bufs = {k: (np.empty((n, 3) if k != "pot" else (n,), dtype=np.float64) if k in fields else None)
+                for k in self.SNAP_FIELDS}
+        call("nb_snapshot_fetch", self.handle, int(slot), dptr(bufs["pos"]), dptr(bufs["vel"]),
+             dptr(bufs["acc"]), dptr(bufs["pot"]))
+        return {k: v for k, v in bufs.items() if v is not None}
 
     # -- forces --------------------------------------------------------------
     def compute_force(self):

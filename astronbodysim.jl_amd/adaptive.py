@@ -26,6 +26,7 @@ nat.register({
     "nb_adaptive_compute_force": (C.c_int, [_P, C.c_int64, _I64]),
     "nb_adaptive_kick": (C.c_int, [_P, C.c_int64]),
     "nb_adaptive_get_steps": (C.c_int, [_P, _I64, _I64]),
+    "nb_adaptive_set_steps": (C.c_int, [_P, _I64, _I64]),
 })
 
 
@@ -91,6 +92,15 @@ class AdaptiveState:
     # advance_and_find_timestep (adaptive)  src/base/Timestep.jl:47-82
     def advance_and_find_timestep(self):
         call("nb_adaptive_kick", self.sim.ctx.handle, int(self.sim.timeinfo.system_time_int))
+
+    def set_steps(self, endstep, begstep):
+        """Restore the integer timeline (caller order) after init_timesteps."""
+        e = np.ascontiguousarray(endstep, dtype=np.int64)
+        b = np.ascontiguousarray(begstep, dtype=np.int64)
+        if e.shape != (len(self.sim),) or b.shape != e.shape:
+            raise ValueError("endstep / begstep must have one entry per particle")
+        call("nb_adaptive_set_steps", self.sim.ctx.handle, e.ctypes.data_as(_I64),
+             b.ctypes.data_as(_I64))
 
     def download_steps(self):
         n = len(self.sim)

@@ -167,6 +167,16 @@ __global__ void k_scatter_acc(const double *__restrict__ gax, const double *__re
     az[i] = gaz[k];
 }
 
+__global__ void k_steps_in(const int64_t *__restrict__ e, const int64_t *__restrict__ b,
+                           const int64_t *__restrict__ perm, int64_t n, int64_t *__restrict__ te,
+                           int64_t *__restrict__ tb) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t u = perm ? perm[i] : i;
+    te[i] = e[u];
+    tb[i] = b[u];
+}
+
 }  // namespace nb
 
 using namespace nb;
@@ -403,6 +413,31 @@ int nb_adaptive_kick(nb_ctx *c, int64_t t_int) {
             c->ti_beg, c->n, t_int, params_of(c), c->flag);
     NB_TRY(launch_ok("k_adaptive_advance"));
     return read_flag(c);
+}
+
+int nb_adaptive_set_steps(nb_ctx *c, const int64_t *endstep, const int64_t *begstep) {
+    NB_TRY(check_adaptive(c));
+    const int64_t n = c->n;
+    if (n == 0) return NB_OK;
+    if (!endstep || !begstep) {
+        set_error("endstep and begstep are required");
+        return NB_ERR_INVALID;
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        if (begstep[i] < 0 || endstep[i] < begstep[i] || endstep[i] > c->ad.time_base) {
+            set_error("step %lld: begstep %lld / endstep %lld outside 0 <= beg <= end <= TimeBase",
+                      (long long)i, (long long)begstep[i], (long long)endstep[i]);
+            return NB_ERR_INVALID;
+        }
+    }
+    // staging: the gathered-target buffer holds 7n doubles, 2n int64 fit
+    int64_t *se = (int64_t *)c->ad.buf, *sb = se + n;
+    NB_HIP(hipMemcpyAsync(se, endstep, sizeof(int64_t) * n, hipMemcpyHostToDevice, c->stream));
+    NB_HIP(hipMemcpyAsync(sb, begstep, sizeof(int64_t) * n, hipMemcpyHostToDevice, c->stream));
+    k_steps_in<<<gridn(n), 256, 0, c->stream>>>(se, sb, c->perm, n, c->ti_end, c->ti_beg);
+    NB_TRY(launch_ok("k_steps_in"));
+    NB_HIP(hipStreamSynchronize(c->stream));
+    return NB_OK;
 }
 
 int nb_adaptive_get_steps(nb_ctx *c, int64_t *endstep, int64_t *begstep) {

@@ -417,6 +417,7 @@ int nb_destroy(nb_ctx *c) {
     dfree(c->ti_end); dfree(c->ti_beg);
     dfree(c->ad.flags); dfree(c->ad.idx); dfree(c->ad.cnt); dfree(c->ad.buf); dfree(c->ad.tmp);
     sort_free(c);
+    snapshot_free(c);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return NB_OK;
@@ -489,6 +490,26 @@ int nb_upload(nb_ctx *c, const double *pos, const double *vel, const double *mas
     c->calls_since_sort = 0;
     NB_TRY(rebuild_central(c));
     NB_HIP(hipStreamSynchronize(s));
+    return NB_OK;
+}
+
+int nb_upload_acc(nb_ctx *c, const double *acc, const double *pot) {
+    NB_TRY(check_ctx(c));
+    const int64_t n = c->n;
+    if (n == 0) return NB_OK;
+    hipStream_t s = c->stream;
+    const size_t nn = (size_t)n;
+    // caller order -> internal order: internal i takes caller perm[i]
+    if (acc) {
+        NB_HIP(hipMemcpyAsync(c->stage, acc, sizeof(double) * 3 * nn, hipMemcpyHostToDevice, s));
+        NB_TRY(to_soa(c->stage, c->perm, n, c->ax, c->ay, c->az, true, s));
+        NB_HIP(hipStreamSynchronize(s));
+    }
+    if (pot) {
+        NB_HIP(hipMemcpyAsync(c->stage, pot, sizeof(double) * nn, hipMemcpyHostToDevice, s));
+        NB_TRY(scalar_in(c->stage, c->perm, n, c->pot, true, s));
+        NB_HIP(hipStreamSynchronize(s));
+    }
     return NB_OK;
 }
 

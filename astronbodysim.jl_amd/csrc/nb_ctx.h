@@ -36,6 +36,17 @@ struct AdaptState {
     size_t tmp_bytes = 0;
 };
 
+// One pinned host slot of the asynchronous snapshot download (nb_snapshot.hip).
+constexpr int SNAP_SLOTS = 2;
+struct SnapSlot {
+    double *host = nullptr;    // pinned: pos 3n | vel 3n | acc 3n | pot n (caller order)
+    size_t cap = 0;            // doubles
+    hipEvent_t ev = nullptr;   // recorded after the slot's copies
+    int what = 0;
+    int64_t n = 0;
+    bool pending = false;
+};
+
 }  // namespace nb
 
 struct nb_ctx {
@@ -74,6 +85,7 @@ struct nb_ctx {
     std::vector<nb::BgField> bg;
     double mond_acc0 = 0.0;       // > 0: Milgrom-1983 rescaling after the background fields
     double mond_index = 2.0;
+    nb::SnapSlot snap[nb::SNAP_SLOTS];
     // timing
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
@@ -118,4 +130,6 @@ int ctx_run_force(nb_ctx *c, bool with_pot, bool allow_timing);
 int ctx_rebuild_central(nb_ctx *c);
 int ctx_gather_central(nb_ctx *c);
 void ctx_drop_graph(nb_ctx *c);
+// nb_snapshot.hip
+void snapshot_free(nb_ctx *c);
 }  // namespace nb

@@ -13,6 +13,9 @@ reference (paths relative to the AstroNbodySim.jl repository root):
   total_angular_momentum             src/base/Energy.jl:13-100
   get_all_data / get_local_data      src/base/Config.jl:1182-1235
   softlen / set_softlen!             src/tools/SofteningLength.jl:10-93
+  output(sim)  (Gadget-2 snapshots)  src/base/Output.jl:9-63
+  analysis.csv / timing.csv          src/base/Logging.jl:1-21, src/base/Analyse.jl:7-22,
+                                     src/base/Timing.jl:32-37
 
 Quantities are unitless (the reference's ``units = nothing`` path); a caller
 with Unitful data strips units first, as preprocessdata does
@@ -156,6 +159,8 @@ class OutputConfig:
     pot: bool
     SaveRestart: bool
     SaveRestartFreq: float
+    format2: bool = True
+    acc: bool = False
 
 
 @dataclass
@@ -222,8 +227,9 @@ class Simulation:
                  TimeIntegrationAlgorithm: TimeIntegration = None,
                  ForceSofteningTable=None, G: float = 1.0, constants: Constants = None,
                  model: GravityModel = None, MOND_nuIndex: float = 2.0,
-                 OutputDir: str = "output", output_func: Optional[Callable] = None,
-                 pot: bool = False, SaveRestart: bool = False, SaveRestartFreq: float = 0.1,
+                 OutputDir: Optional[str] = None, output_func: Optional[Callable] = None,
+                 pot: bool = False, acc: bool = False, format2: bool = True,
+                 SaveRestart: bool = False, SaveRestartFreq: float = 1.0,
                  analysers: Optional[Dict[str, Callable]] = None,
                  bgforce: Optional[list] = None, poststep: Optional[List[Callable]] = None,
                  write_timing_csv: bool = False):
@@ -266,7 +272,8 @@ class Simulation:
             grav=GravityConfig(table, model, float(MOND_nuIndex)),
             device=DeviceConfig(device, int(GPU_NumThreads), GPU_Algorithm, precision),
             constants=Constants(float(G), float(acc0)),
-            output=OutputConfig(OutputDir, output_func, bool(pot), bool(SaveRestart),
+            output=OutputConfig(os.path.join(os.getcwd(), "output") if OutputDir is None else OutputDir,
+                                output_func, bool(pot), bool(SaveRestart),
                                 float(SaveRestartFreq)),
             floattype=floattype,
         )
@@ -277,6 +284,10 @@ class Simulation:
             next_output_time_float=float(TimeBegin) + float(TimeBetweenSnapshots),
             system_time_float=float(TimeBegin))
         self.loginfo = {"analysers": dict(analysers or {}), "timers": {k: 0.0 for k in _TIMERS}}
+        self.outputinfo = {"snapshotcount": 0}
+        self._writer = None
+        self._next_restart = math.inf
+        self._resume_tail = False
         self.bgforce = list(bgforce or [])
         self.poststep = list(poststep or [])
         self.write_timing_csv = write_timing_csv
@@ -298,7 +309,12 @@ class Simulation:
         return len(self._host)
 
     def close(self):
-        self.ctx.close()
+        try:
+            if self._writer is not None:
+                self._writer.close()
+                self._writer = None
+        finally:
+            self.ctx.close()
 
 
 # ---------------------------------------------------------------------------
@@ -473,10 +489,32 @@ def step(sim: Simulation):
     advance_and_find_timestep(sim)
 
 
+def output(sim: Simulation):
+    """Default output function: one Gadget-2 snapshot
+    OutputDir/snapshot_NNNN.gadget2 (src/base/Output.jl:36-63), written by a
+    worker thread from an asynchronous device copy so that the run goes on
+    meanwhile.  The header follows src/base/Output.jl:9-25."""
+    from .snapshot import SnapshotWriter, header_for
+    cfg = sim.config.output
+    os.makedirs(cfg.dir, exist_ok=True)
+    if cfg.pot:
+        compute_potential(sim)
+    if sim._writer is None:
+        sim._writer = SnapshotWriter(sim.ctx)
+    fname = os.path.join(cfg.dir, "snapshot_%04d.gadget2" % sim.outputinfo["snapshotcount"])
+    h = sim._host
+    header = header_for(h.Collection, sim.timeinfo.system_time_float, sim.timeinfo.redshift)
+    sim._writer.submit(fname, header, h.ID, h.Mass, h.Collection, acc=cfg.acc, pot=cfg.pot,
+                       format2=cfg.format2)
+    sim.outputinfo["snapshotcount"] += 1
+
+
 def _output(sim: Simulation):
     f = sim.config.output.func
     if f is not None:
         f(sim)
+    else:
+        output(sim)
 
 
 def _fast_forward_steps(sim: Simulation) -> int:
@@ -507,11 +545,9 @@ def run(sim: Simulation, use_graph: bool = True):
     Stretches of steps with no host-side event are issued as one
     nb_kdk_steps call (identical step sequence, no host round trip)."""
     cfg = sim.config
-    ti = sim.timeinfo
     outdir = cfg.output.dir
     stopfile = os.path.join(outdir, "stop")
-    if cfg.output.func is not None or cfg.output.SaveRestart or sim.write_timing_csv:
-        os.makedirs(outdir, exist_ok=True)
+    os.makedirs(outdir, exist_ok=True)                     # run.jl:63
     if os.path.isfile(stopfile):
         os.remove(stopfile)
     compute_force(sim)                                       # run.jl:76
@@ -521,60 +557,113 @@ def run(sim: Simulation, use_graph: bool = True):
     if isinstance(cfg.time.algorithm, Leapfrog):
         leapfrog_half_kick(sim, +1)                          # run.jl:85-87
     _output(sim)                                             # run.jl:90
-    analysis = []
-    restart_every = cfg.output.SaveRestartFreq * (cfg.time.End - cfg.time.Begin)
-    next_restart = restart_every
-    timing_rows = []
-    state = True
-    while state:
-        t_total = time.time_ns()
-        nfast = _fast_forward_steps(sim)
-        if nfast >= 2:
-            dt = cfg.time.step.dt
-            t0 = time.time_ns()
-            sim.ctx.kdk_steps(nfast, dt, ti.system_time_float, use_graph=use_graph)
-            sim.ctx.sync()
-            add_timer(sim, "FORCE", t0, time.time_ns())
-            for _ in range(nfast):
-                ti.last_system_time_float = ti.system_time_float
-                ti.system_time_float = ti.system_time_float + dt
-            ti.dt = dt
-            ti.stepcount += nfast
-            add_timer(sim, "TOTAL", t_total, time.time_ns())
-            continue
-        step(sim)                                            # run.jl:112
-        t0 = time.time_ns()
-        if sim.loginfo["analysers"]:                         # run.jl:115-119
-            analysis.append([f(sim) for f in sim.loginfo["analysers"].values()])
-        add_timer(sim, "ANALYSIS", t0, time.time_ns())
-        t0 = time.time_ns()
-        for ps in sim.poststep:                              # run.jl:122-128
-            ps(sim)
-        add_timer(sim, "POSTSTEP", t0, time.time_ns())
-        if cfg.output.SaveRestart and ti.system_time_float >= next_restart:   # run.jl:138-141
-            from .restart import saverestart
-            saverestart(sim)
-            next_restart += restart_every
-        if os.path.isfile(stopfile):                         # run.jl:153-157
-            import warnings
-            warnings.warn("Interrupted by user")
-            state = False
-        if ti.system_time_float >= cfg.time.End:            # run.jl:158-160
-            state = False
-        ti.stepcount += 1                                    # run.jl:163-164
+    sim._next_restart = cfg.output.SaveRestartFreq * (cfg.time.End - cfg.time.Begin)
+    return _main_loop(sim, use_graph, append=False, run_loop=True)
+
+
+def restart(sim: Simulation, use_graph: bool = True):
+    """restart(sim)  src/run.jl:186-189: reopen the loggers for appending and
+    continue the main loop of a Simulation returned by loadrestart.  The
+    restart file is written inside a loop iteration (run.jl:138-141), so the
+    rest of that iteration (end check, step counter) is finished first."""
+    ti = sim.timeinfo
+    run_loop = True
+    if getattr(sim, "_resume_tail", False):
+        sim._resume_tail = False
+        run_loop = ti.system_time_float < sim.config.time.End
+        ti.stepcount += 1
         ti.last_system_time_float = ti.system_time_float
-        add_timer(sim, "TOTAL", t_total, time.time_ns())
-        if sim.write_timing_csv:
-            timing_rows.append([ti.stepcount, ti.system_time_float] +
-                               [sim.loginfo["timers"][k] for k in _TIMERS])
-    if isinstance(cfg.time.algorithm, Leapfrog):
-        leapfrog_half_kick(sim, -1)                          # run.jl:171-173
-    sim.ctx.sync()
-    if sim.write_timing_csv and timing_rows:
-        with open(os.path.join(outdir, "timing.csv"), "w") as f:
-            f.write("step,time," + ",".join(_TIMERS) + "\n")
-            for r in timing_rows:
-                f.write(",".join(str(v) for v in r) + "\n")
-    if sim.loginfo["analysers"]:
+    os.makedirs(sim.config.output.dir, exist_ok=True)
+    return _main_loop(sim, use_graph, append=True, run_loop=run_loop)
+
+
+def _main_loop(sim: Simulation, use_graph: bool, append: bool, run_loop: bool):
+    """The while-loop and the closing half kick of src/run.jl:100-184."""
+    cfg = sim.config
+    ti = sim.timeinfo
+    outdir = cfg.output.dir
+    stopfile = os.path.join(outdir, "stop")
+    analysis = []
+    anames = list(sim.loginfo["analysers"])
+    aio = tio = None
+    mode = "a" if append else "w"
+    if anames:                                               # Logging.jl:13-16
+        path = os.path.join(outdir, "analysis.csv")
+        fresh = not (append and os.path.isfile(path))
+        aio = open(path, mode if not fresh else "w")
+        if fresh:
+            aio.write(",".join(["time"] + anames) + "\n")
+    if sim.write_timing_csv:                                 # Logging.jl:8-11
+        path = os.path.join(outdir, "timing.csv")
+        fresh = not (append and os.path.isfile(path))
+        tio = open(path, mode if not fresh else "w")
+        if fresh:
+            tio.write(",".join(["time"] + list(_TIMERS)) + "\n")
+    restart_every = cfg.output.SaveRestartFreq * (cfg.time.End - cfg.time.Begin)
+    state = run_loop
+    try:
+        while state:
+            t_total = time.time_ns()
+            nfast = _fast_forward_steps(sim)
+            if nfast >= 2:
+                dt = cfg.time.step.dt
+                t0 = time.time_ns()
+                sim.ctx.kdk_steps(nfast, dt, ti.system_time_float, use_graph=use_graph)
+                sim.ctx.sync()
+                add_timer(sim, "FORCE", t0, time.time_ns())
+                for _ in range(nfast):
+                    ti.last_system_time_float = ti.system_time_float
+                    ti.system_time_float = ti.system_time_float + dt
+                ti.dt = dt
+                ti.stepcount += nfast
+                add_timer(sim, "TOTAL", t_total, time.time_ns())
+                continue
+            step(sim)                                        # run.jl:112
+            t0 = time.time_ns()
+            if anames:                                       # run.jl:115-119, Analyse.jl:7-22
+                row = [ti.system_time_float] + [sim.loginfo["analysers"][k](sim) for k in anames]
+                analysis.append(row)
+                aio.write(",".join(_csv(v) for v in row) + "\n")
+                aio.flush()
+            add_timer(sim, "ANALYSIS", t0, time.time_ns())
+            t0 = time.time_ns()
+            for ps in sim.poststep:                          # run.jl:122-128
+                ps(sim)
+            add_timer(sim, "POSTSTEP", t0, time.time_ns())
+            if cfg.output.SaveRestart and ti.system_time_float >= sim._next_restart:  # run.jl:138-141
+                from .restart import saverestart
+                sim._next_restart += restart_every
+                saverestart(sim)
+            if os.path.isfile(stopfile):                     # run.jl:153-157
+                import warnings
+                warnings.warn("Interrupted by user")
+                state = False
+            if ti.system_time_float >= cfg.time.End:        # run.jl:158-160
+                state = False
+            ti.stepcount += 1                                # run.jl:163-164
+            ti.last_system_time_float = ti.system_time_float
+            add_timer(sim, "TOTAL", t_total, time.time_ns())
+            if tio is not None:                              # Timing.jl:32-37
+                tio.write(",".join([repr(ti.system_time_float)] +
+                                   [str(int(sim.loginfo["timers"][k])) for k in _TIMERS]) + "\n")
+                tio.flush()
+        if isinstance(cfg.time.algorithm, Leapfrog):
+            leapfrog_half_kick(sim, -1)                      # run.jl:171-173
+        sim.ctx.sync()
+    finally:
+        for f in (aio, tio):
+            if f is not None:
+                f.close()
+    if sim._writer is not None:
+        sim._writer.flush()                                  # every snapshot is on disk
+    if anames:
         return np.array(analysis)
     return None
+
+
+def _csv(v) -> str:
+    """csvstring  src/base/Analyse.jl:1-5: vectors are quoted."""
+    a = np.asarray(v)
+    if a.ndim == 0:
+        return repr(float(a)) if np.issubdtype(a.dtype, np.floating) else str(a.item())
+    return '"' + "[" + ", ".join(repr(float(x)) for x in a.ravel()) + "]" + '"'

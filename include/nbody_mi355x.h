@@ -106,6 +106,25 @@ NB_API int nb_upload(nb_ctx *ctx, const double *pos, const double *vel, const do
 /* Copy state device->host in the caller's particle order; any pointer may be
  * NULL.  Replaces get_all_data / get_local_data  src/base/Config.jl:1199-1201,1228-1230. */
 NB_API int nb_download(nb_ctx *ctx, double *pos, double *vel, double *acc, double *pot);
+/* Set Acc and/or Potential (caller order; NULL skips).  A resumed run needs the
+ * accelerations of the last force evaluation: with adaptive steps those of
+ * inactive particles were taken at earlier positions and cannot be recomputed
+ * (restart state, src/restart.jl:14-19,127-136). */
+NB_API int nb_upload_acc(nb_ctx *ctx, const double *acc, const double *pot);
+
+/* Asynchronous snapshot download, the device half of output(sim, gadget2)
+ * src/base/Output.jl:36-50 (called from src/run.jl:90,116-131), which copies
+ * synchronously.  nb_snapshot_begin enqueues, on the context's stream, the
+ * copies of the fields in `what` (1 Pos, 2 Vel, 4 Acc, 8 Potential; caller
+ * order) into pinned host slot `slot` (0 or 1) and returns at once.
+ * nb_snapshot_fetch waits for that slot's copies and moves them into the
+ * caller's arrays (NULL skips a field; asking for a field the slot did not
+ * take is an error).  A slot must be fetched before it is begun again.
+ * fetch may run on another host thread while the context keeps working, but
+ * not at the same time as begin of the same slot or nb_destroy. */
+NB_API int nb_snapshot_begin(nb_ctx *ctx, int32_t slot, int32_t what);
+NB_API int nb_snapshot_fetch(nb_ctx *ctx, int32_t slot, double *pos, double *vel, double *acc,
+                             double *pot);
 
 /* ---- forces and potentials -------------------------------------------- */
 
@@ -221,6 +240,10 @@ NB_API int nb_adaptive_compute_force(nb_ctx *ctx, int64_t t_int, int64_t *nactiv
 NB_API int nb_adaptive_kick(nb_ctx *ctx, int64_t t_int);
 /* Ti_endstep / Ti_begstep in the caller's order (either may be NULL). */
 NB_API int nb_adaptive_get_steps(nb_ctx *ctx, int64_t *endstep, int64_t *begstep);
+/* Restore Ti_endstep / Ti_begstep (caller order) after nb_adaptive_init, for a
+ * run resumed from a restart file (the reference keeps them inside the
+ * serialised Simulation, src/restart.jl:14-19,127-136). */
+NB_API int nb_adaptive_set_steps(nb_ctx *ctx, const int64_t *endstep, const int64_t *begstep);
 
 /* ---- batched ensembles: many independent systems in one launch ----------- */
 /* No reference counterpart (the reference runs one system per Simulation);

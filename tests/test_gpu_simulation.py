@@ -9,6 +9,12 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(autouse=True)
+def _in_tmp(tmp_path, monkeypatch):
+    """Default snapshots go to ./output, as in the reference: run in a scratch dir."""
+    monkeypatch.chdir(tmp_path)
+
+
 @pytest.fixture(scope="module")
 def c1_inputs(pkg):
     """Config 1: Plummer N=4096, G=M=a=1, eps=0.01, dt=1e-3, 100 KDK steps."""
@@ -69,7 +75,11 @@ def test_snapshots_analysers_and_stop_file(pkg, orc, tmp_path):
                          analysers={"K": pkg.total_kinetic})
     res = pkg.run(sim)
     assert np.allclose(seen, oracle_times, rtol=0, atol=1e-15)
-    assert res.shape == (orc.count_steps(0.0, 0.02, 3e-3), 1)
+    nsteps = orc.count_steps(0.0, 0.02, 3e-3)
+    assert res.shape == (nsteps, 2)                  # time, then each analyser (Analyse.jl:7-22)
+    lines = open(os.path.join(str(tmp_path), "analysis.csv")).read().splitlines()
+    assert lines[0] == "time,K" and len(lines) == nsteps + 1
+    assert np.allclose([float(l.split(",")[0]) for l in lines[1:]], res[:, 0], rtol=1e-15)
     # stop file: run ends after the current step (src/run.jl:153-157)
     sim2 = pkg.Simulation(pkg.StarData(pos, Vel=vel, Mass=mass), TimeStep=1e-3, TimeEnd=1.0,
                           ForceSofteningTable=soft, OutputDir=str(tmp_path),
@@ -156,8 +166,12 @@ def test_mond_postprocessing(pkg, orc, precision):
     ref = orc.compute_force(pos, mass, soft)
     orc.background_point_mass(pos, 0.5, np.array([2.0, 0.0, 0.0]), acc=ref)
     ref = orc.mond(ref, acc0, nidx)
-    tol = 1e-12 if precision == "fp64" else 1e-6
-    assert np.max(orc.rel_err(a, ref)) < tol
+    e = orc.rel_err(a, ref)
+    # split: FP32 pair terms, gated like the force itself (median 1e-6, max 1e-5)
+    if precision == "fp64":
+        assert e.max() < 1e-12
+    else:
+        assert np.median(e) <= 1e-6 and e.max() < 1e-5
     sim.close()
 
 
@@ -181,3 +195,77 @@ def test_mond_adaptive_active_subset(pkg, orc):
     assert np.array_equal(d.Ti_endstep, info["ti_end"])
     assert np.max(np.abs(d.Pos - xo)) < 1e-12
     sim.close()
+
+
+@pytest.mark.parametrize("format2", [True, False])
+def test_gadget2_snapshots_match_oracle(pkg, orc, format2):
+    """Default output (src/base/Output.jl:36-63): asynchronous Gadget-2 snapshots
+    hold the state at each output time while the run goes on."""
+    from astronbodysim_jl_amd.snapshot import read_gadget2
+    pos, vel, mass = pkg.ics.plummer(2000, seed=31)
+    soft = [0.02] * 6
+    coll = np.where(np.arange(2000) % 3 == 0, pkg.HALO, pkg.DISK).astype(np.uint8)
+    snaps = []
+    orc.run_const_kdk(pos, vel, mass, soft, 1.0, t_begin=0.0, t_end=0.05, dt=2e-3,
+                      between_snapshots=0.01, collection=coll,
+                      on_output=lambda t, p, v: snaps.append((t, p.copy(), v.copy())))
+    data = pkg.StarData(pos, Vel=vel, Mass=mass, Collection=coll)
+    sim = pkg.Simulation(data, precision="fp64", TimeStep=2e-3, TimeEnd=0.05,
+                         TimeBetweenSnapshots=0.01, ForceSofteningTable=soft, format2=format2,
+                         acc=format2, pot=format2)
+    pkg.run(sim)
+    assert sim.outputinfo["snapshotcount"] == len(snaps)
+    order = np.argsort(coll, kind="stable")                 # files are grouped by type
+    for k, (t, p, v) in enumerate(snaps):
+        h, b = read_gadget2(os.path.join("output", "snapshot_%04d.gadget2" % k))
+        assert h.time == t and h.npart == [0, 667, 1333, 0, 0, 0] and h.Omega0 == 0.3
+        assert np.array_equal(b["ID"], np.arange(1, 2001)[order])
+        assert np.array_equal(b["POS"], p[order].astype(np.float32).astype(np.float64))
+        assert np.array_equal(b["VEL"], v[order].astype(np.float32).astype(np.float64))
+        assert np.array_equal(b["Mass"], mass[order].astype(np.float32).astype(np.float64))
+        if format2:
+            assert "POT" in b and "ACCE" in b
+            ref = orc.compute_potential(p, mass, soft, collection=coll, self_term=True)
+            assert np.allclose(b["POT"], ref[order], rtol=1e-6, atol=0)
+    sim.close()
+
+
+@pytest.mark.parametrize("adaptive", [False, True])
+def test_restart_resumes_bit_identical(pkg, tmp_path, adaptive):
+    """saverestart inside run (src/run.jl:138-141), loadrestart
+    (src/restart.jl:127-136) and restart(sim) (src/run.jl:186-189): in FP64 the
+    resumed run ends in exactly the state of the uninterrupted one."""
+    import shutil
+    pos, vel, mass = pkg.ics.plummer(700, seed=41)
+    soft = [0.03] * 6
+    kw = dict(TimeStep=0.0, TimeBase=1 << 14, ErrTolTimestep=1e-3) if adaptive else dict(TimeStep=2e-3)
+    rfile = os.path.join(str(tmp_path), "out", "restart.1.npz")
+    mid = os.path.join(str(tmp_path), "mid.npz")
+
+    def grab(sim):
+        if os.path.isfile(rfile) and not os.path.isfile(mid):
+            shutil.copy(rfile, mid)
+
+    sim = pkg.Simulation(pkg.StarData(pos, Vel=vel, Mass=mass), precision="fp64", TimeEnd=0.04,
+                         TimeBetweenSnapshots=0.01, ForceSofteningTable=soft,
+                         OutputDir=os.path.join(str(tmp_path), "out"), SaveRestart=True,
+                         SaveRestartFreq=0.5, poststep=[grab], **kw)
+    pkg.run(sim)
+    ref = pkg.get_all_data(sim)
+    ref_ti = sim.timeinfo
+    nsnap = sim.outputinfo["snapshotcount"]
+    sim.close()
+    assert os.path.isfile(mid)
+    sim2 = pkg.loadrestart(mid)
+    assert 0.02 <= sim2.timeinfo.system_time_float < 0.04
+    pkg.restart(sim2)
+    d = pkg.get_all_data(sim2)
+    assert np.array_equal(d.Pos, ref.Pos) and np.array_equal(d.Vel, ref.Vel)
+    assert np.array_equal(d.Acc, ref.Acc)
+    assert sim2.timeinfo.stepcount == ref_ti.stepcount
+    assert sim2.timeinfo.system_time_float == ref_ti.system_time_float
+    assert sim2.outputinfo["snapshotcount"] == nsnap
+    if adaptive:
+        assert np.array_equal(d.Ti_endstep, ref.Ti_endstep)
+        assert np.array_equal(d.Ti_begstep, ref.Ti_begstep)
+    sim2.close()
