@@ -201,7 +201,7 @@ int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = tru
         a.skip_src = (c->mode == NB_MODE_SPLIT_CENTRAL && c->ncentral > 0) ? c->central : nullptr;
         a.accumulate = 0;
         if (f64) NB_TRY(launch_force_f64(a, with_pot, s));
-        else NB_TRY(launch_force_split(a, tpt, with_pot, s, c->tun.mfma != 0));
+        else NB_TRY(launch_force_split(a, tpt, with_pot, s, c->tun.mfma));
     } else {
         ForceArgs<float> a{};
         a.tx = (const float *)c->x; a.ty = (const float *)c->y; a.tz = (const float *)c->z;
@@ -860,7 +860,7 @@ int nb_set_tuning(nb_ctx *c, const nb_tuning *t) {
     if (!t || (t->targets_per_thread != 1 && t->targets_per_thread != 2 &&
                t->targets_per_thread != 4) ||
         t->j_segments < 1 || t->j_segments > 256 || t->sort_interval < 0 || t->mfma < 0 ||
-        t->mfma > 1) {
+        t->mfma > 2) {
         set_error("invalid tuning");
         return NB_ERR_INVALID;
     }

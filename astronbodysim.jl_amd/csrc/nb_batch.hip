@@ -100,7 +100,7 @@ struct BatchArgs {
 // Persistent ragged all-pairs kernel: blocks pull (system, target tile) items
 // from an atomic queue until it is empty; the last block to finish resets the
 // queue for the next launch (so the launch can be replayed from a graph).
-template <int TPT, bool POT, bool MF = false>
+template <int TPT, bool POT, int MF = 0>
 __global__ __launch_bounds__(BS) void k_batch_force(BatchArgs a) {
     __shared__ BlockShared<double, MF> sh;
     __shared__ int64_t item_sh;
@@ -637,7 +637,8 @@ static int bforce(nb_batch *b, bool with_pot, bool allow_timing) {
         int nblk = b->tun.blocks > 0 ? b->tun.blocks : 8 * b->cu_count;
         nblk = (int)std::max<int64_t>(1, std::min<int64_t>(nblk, b->nitems));
 #define NB_BL(T_, P_) k_batch_force<T_, P_><<<nblk, BS, 0, s>>>(a)
-        if (tpt == 1 && !with_pot && b->tun.mfma) k_batch_force<1, false, true><<<nblk, BS, 0, s>>>(a);
+        if (tpt == 1 && !with_pot && b->tun.mfma == 2) k_batch_force<1, false, 2><<<nblk, BS, 0, s>>>(a);
+        else if (tpt == 1 && !with_pot && b->tun.mfma) k_batch_force<1, false, 1><<<nblk, BS, 0, s>>>(a);
         else if (tpt == 1) { if (with_pot) NB_BL(1, true); else NB_BL(1, false); }
         else if (tpt == 2) { if (with_pot) NB_BL(2, true); else NB_BL(2, false); }
         else { if (with_pot) NB_BL(4, true); else NB_BL(4, false); }
@@ -1153,7 +1154,7 @@ int nb_batch_set_tuning(nb_batch *b, const nb_batch_tuning *t) {
     if (!t || (t->targets_per_thread != 1 && t->targets_per_thread != 2 &&
                t->targets_per_thread != 4) ||
         t->blocks < 0 || t->blocks > (1 << 20) || t->steps_per_graph < 2 ||
-        t->steps_per_graph > 4096 || t->sort_interval < 0 || t->mfma < 0 || t->mfma > 1) {
+        t->steps_per_graph > 4096 || t->sort_interval < 0 || t->mfma < 0 || t->mfma > 2) {
         set_error("invalid batch tuning");
         return NB_ERR_INVALID;
     }
@@ -1185,10 +1186,10 @@ int nb_batch_autotune(nb_batch *b) {
     NB_HIP(hipEventCreate(&e0));
     NB_HIP(hipEventCreate(&e1));
     int rc = NB_OK;
-    for (int ti = 0; ti < 4 && rc == NB_OK; ++ti) {
+    for (int ti = 0; ti < 5 && rc == NB_OK; ++ti) {
         for (int mi = 0; mi < 4 && rc == NB_OK; ++mi) {
             b->tun.targets_per_thread = ti < 3 ? tpts[ti] : 1;
-            b->tun.mfma = ti == 3 ? 1 : 0;
+            b->tun.mfma = ti < 3 ? 0 : ti - 2;
             b->tun.blocks = mult[mi] * b->cu_count;
             rc = bforce(b, false, false);
             if (rc) break;

@@ -157,21 +157,34 @@ __device__ __forceinline__ void tile_f32_hilo(const float4 *__restrict__ s,
 // For a far pair the separation x_j - x_i, formed from two FP32 roundings of
 // coordinates of size <= R + r, carries a relative error <= 2 eps (1 + R/r)
 // <= 6 eps since r > R/2.
-// With MF (SPLIT only) the far tiles use the matrix cores for the sum:
+// With MF >= 1 (SPLIT only) the far tiles use the matrix cores for the sum:
 //   a_i = sum_j m_j r_ij^-3 (x'_j - x'_i) = S_xyz - x'_i S_m,
 //   S = sum_j r_ij^-3 (m_j x'_j, m_j y'_j, m_j z'_j, m_j),
 // one v_mfma_f32_4x4x1f32 per source and wave: 16 blocks of 4 targets x 1
-// source x 4 columns.  The VALU keeps the 8-op r^-3 part of every pair.  In
-// a far tile |x'_i| <= R < 2r and |x'_j| <= R + r < 3r, so each term of S is
-// bounded by a few times the pair's force and the subtraction costs a few eps.
+// source x 4 columns.  With MF = 1 the VALU keeps the 8-op r^-3 part of every
+// pair.  In a far tile |x'_i| <= R < 2r and |x'_j| <= R + r < 3r, so each term
+// of S is bounded by a few times the pair's force and the subtraction costs a
+// few eps.
+// With MF = 2 the squared distance of the farthest tiles comes from the matrix
+// cores too:
+//   r_ij^2 = |x'_i|^2 + |x'_j|^2 - 2 x'_i . x'_j,
+// four v_mfma_f32_4x4x1f32 per 4 sources and wave (K = x, y, z, |x'_j|^2 with
+// C = |x'_i|^2), so the VALU does only rsq and two multiplies per pair.  The
+// expansion loses about 3 eps (|x'_i| + |x'_j|)^2 / r^2 relative; the tiles it
+// serves have r >= MF2_REACH * Re, where Re is the wave's largest Euclidean
+// distance from the block origin, so (|x'_i| + |x'_j|) <= r + 2 Re
+// <= (1 + 2 / MF2_REACH) r and the loss stays below about 8 eps.  Tiles in
+// between take the MF = 1 loop.
+constexpr double MF2_REACH = 3.0;
 
 // Shared memory of one all-pairs block.
-template <typename T, bool MF = false>
+template <typename T, int MF = 0>
 struct BlockShared {
     static constexpr bool HILO = sizeof(T) == 8;
     float4 sj[2][BS];
     float4 sjl[HILO ? 2 : 1][HILO ? BS : 1];
     float sB[MF ? 2 : 1][MF ? 4 : 1][MF ? BS : 1];   // MFMA B columns, component-major
+    float sQ[MF == 2 ? 2 : 1][MF == 2 ? BS : 1];     // |x'_j|^2 of each source
     double red[11][NW];
 };
 
@@ -183,9 +196,9 @@ typedef float nb_f4v __attribute__((ext_vector_type(4)));
 // part of the four pairs first (independent, so it interleaves), then four
 // MFMAs alternating between two accumulators (no back-to-back dependence).
 __device__ __forceinline__ void tile_mfma(const float4 *__restrict__ s, const float *__restrict__ sB,
-                                          int q, float xi, float yi, float zi, nb_f4v &D0,
-                                          nb_f4v &D1) {
+                                          int q, float xi, float yi, float zi, double (&S)[4]) {
     const float *bp = sB + q * BS;
+    nb_f4v D0 = {0.0f, 0.0f, 0.0f, 0.0f}, D1 = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll 2
     for (int jj = 0; jj < BS; jj += 4) {
         const float4 bq = *reinterpret_cast<const float4 *>(bp + jj);
@@ -203,13 +216,61 @@ __device__ __forceinline__ void tile_mfma(const float4 *__restrict__ s, const fl
         D0 = __builtin_amdgcn_mfma_f32_4x4x1f32(r3[2], bq.z, D0, 0, 0, 0);
         D1 = __builtin_amdgcn_mfma_f32_4x4x1f32(r3[3], bq.w, D1, 0, 0, 0);
     }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) S[v] += (double)D0[v] + (double)D1[v];
+}
+
+// Farthest tiles with MF = 2.  Lane l = 4b + q supplies, for the four sources
+// jj..jj+3, A = component of source jj + q and B = the matching component of
+// its own target (-2x'_i, -2y'_i, -2z'_i, 1); R[v] of lane l then holds r^2 of
+// (target l, source jj + v).  The r^-3 values feed the same column MFMAs as
+// tile_mfma.  Two groups of four sources are interleaved so that no MFMA
+// waits on the one just issued.
+__device__ __forceinline__ void tile_mfma2(const float4 *__restrict__ s, const float *__restrict__ sq,
+                                           const float *__restrict__ sB, int q, float bx, float by,
+                                           float bz, nb_f4v ci, double (&S)[4]) {
+    const float *bp = sB + q * BS;
+    nb_f4v D0 = {0.0f, 0.0f, 0.0f, 0.0f}, D1 = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 1
+    for (int jj = 0; jj < BS; jj += 8) {
+        const float4 a0 = s[jj + q], a1 = s[jj + 4 + q];
+        const float q0 = sq[jj + q], q1 = sq[jj + 4 + q];
+        nb_f4v R0 = __builtin_amdgcn_mfma_f32_4x4x1f32(a0.x, bx, ci, 0, 0, 0);
+        nb_f4v R1 = __builtin_amdgcn_mfma_f32_4x4x1f32(a1.x, bx, ci, 0, 0, 0);
+        R0 = __builtin_amdgcn_mfma_f32_4x4x1f32(a0.y, by, R0, 0, 0, 0);
+        R1 = __builtin_amdgcn_mfma_f32_4x4x1f32(a1.y, by, R1, 0, 0, 0);
+        R0 = __builtin_amdgcn_mfma_f32_4x4x1f32(a0.z, bz, R0, 0, 0, 0);
+        R1 = __builtin_amdgcn_mfma_f32_4x4x1f32(a1.z, bz, R1, 0, 0, 0);
+        R0 = __builtin_amdgcn_mfma_f32_4x4x1f32(q0, 1.0f, R0, 0, 0, 0);
+        R1 = __builtin_amdgcn_mfma_f32_4x4x1f32(q1, 1.0f, R1, 0, 0, 0);
+        const float4 b0 = *reinterpret_cast<const float4 *>(bp + jj);
+        const float4 b1 = *reinterpret_cast<const float4 *>(bp + jj + 4);
+        float r3a[4], r3b[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const float ra = __builtin_amdgcn_rsqf(R0[u]);
+            const float rb = __builtin_amdgcn_rsqf(R1[u]);
+            r3a[u] = (ra * ra) * ra;
+            r3b[u] = (rb * rb) * rb;
+        }
+        D0 = __builtin_amdgcn_mfma_f32_4x4x1f32(r3a[0], b0.x, D0, 0, 0, 0);
+        D1 = __builtin_amdgcn_mfma_f32_4x4x1f32(r3a[1], b0.y, D1, 0, 0, 0);
+        D0 = __builtin_amdgcn_mfma_f32_4x4x1f32(r3a[2], b0.z, D0, 0, 0, 0);
+        D1 = __builtin_amdgcn_mfma_f32_4x4x1f32(r3a[3], b0.w, D1, 0, 0, 0);
+        D0 = __builtin_amdgcn_mfma_f32_4x4x1f32(r3b[0], b1.x, D0, 0, 0, 0);
+        D1 = __builtin_amdgcn_mfma_f32_4x4x1f32(r3b[1], b1.y, D1, 0, 0, 0);
+        D0 = __builtin_amdgcn_mfma_f32_4x4x1f32(r3b[2], b1.z, D0, 0, 0, 0);
+        D1 = __builtin_amdgcn_mfma_f32_4x4x1f32(r3b[3], b1.w, D1, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) S[v] += (double)D0[v] + (double)D1[v];
 }
 
 // One block's share of an all-pairs sum: targets i0 + k*BS + tid (k < TPT,
 // i < i_end) against the sources [j0, j1).  tile_box points at the box of
 // the BS-source tile that starts at j0 (tiles are BS-aligned from j0).
 // Returns FP64 sums (without G) in Ax..Aq and the target validity.
-template <typename T, int TPT, bool POT, bool MF = false>
+template <typename T, int TPT, bool POT, int MF = 0>
 __device__ __forceinline__ void force_block(
     BlockShared<T, MF> &sh, const T *__restrict__ tx, const T *__restrict__ ty,
     const T *__restrict__ tz, const double *__restrict__ th, double th_const, int64_t i0,
@@ -221,7 +282,6 @@ __device__ __forceinline__ void force_block(
     static_assert(!MF || (HILO && TPT == 1 && !POT), "MFMA far tiles: SPLIT, 1 target/lane, acc only");
     const int tid = threadIdx.x;
     __syncthreads();  // the previous call's readers of sh are done
-    nb_f4v D0 = {0.0f, 0.0f, 0.0f, 0.0f}, D1 = {0.0f, 0.0f, 0.0f, 0.0f};
     double S[4] = {0.0, 0.0, 0.0, 0.0};
 
     double px[TPT], py[TPT], pz[TPT], ph[TPT];
@@ -291,6 +351,22 @@ __device__ __forceinline__ void force_block(
     const double soft_d = whmax * (1.0 + 1e-6) + 1e-6 * rb + 1e-300;
     const double close_d = HILO ? fmax(soft_d, 0.5 * rb) : soft_d;
     const double soft_d2 = soft_d * soft_d, close_d2 = close_d * close_d;
+    // MF = 2: the Euclidean reach of the wave's targets and their MFMA operands
+    double mf2_d2 = DBL_MAX;
+    float b2x = 0.0f, b2y = 0.0f, b2z = 0.0f;
+    nb_f4v ci = {0.0f, 0.0f, 0.0f, 0.0f};
+    if constexpr (MF == 2) {
+        const double dx = px[0] - ox, dy = py[0] - oy, dz = pz[0] - oz;
+        const double re2 = valid[0] ? dx * dx + dy * dy + dz * dz : 0.0;
+        const double re = sqrt(wave_max(re2));
+        const double d = fmax(close_d, MF2_REACH * re);
+        mf2_d2 = d * d;
+        b2x = -2.0f * xi[0];
+        b2y = -2.0f * yi[0];
+        b2z = -2.0f * zi[0];
+        const float c = (float)re2;
+        ci = nb_f4v{c, c, c, c};
+    }
 
     // Register prefetch of the first tile.
     T rx = 0, ry = 0, rz = 0, rm = 0;
@@ -318,6 +394,7 @@ __device__ __forceinline__ void force_block(
                 sh.sB[buf][2][tid] = (float)(md * dz);
                 sh.sB[buf][3][tid] = (float)md;
             }
+            if constexpr (MF == 2) sh.sQ[MF == 2 ? buf : 0][MF == 2 ? tid : 0] = (float)(dx * dx + dy * dy + dz * dz);
         } else {
             sh.sj[buf][tid] = make_float4(PAD_COORD, PAD_COORD, PAD_COORD, 0.0f);
             if (HILO) sh.sjl[HILO ? buf : 0][HILO ? tid : 0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -327,6 +404,8 @@ __device__ __forceinline__ void force_block(
                 sh.sB[buf][2][tid] = 0.0f;
                 sh.sB[buf][3][tid] = 0.0f;
             }
+            // |PAD|^2: r^2 ~ 3e36 keeps r^-3 at zero (and finite) for padding
+            if constexpr (MF == 2) sh.sQ[MF == 2 ? buf : 0][MF == 2 ? tid : 0] = 3.0f * PAD_COORD * PAD_COORD;
         }
         __syncthreads();
         {
@@ -349,14 +428,12 @@ __device__ __forceinline__ void force_block(
         } else if (HILO && d2 <= close_d2) {
             tile_f32_hilo<TPT, false, POT>(sh.sj[buf], sh.sjl[HILO ? buf : 0], xi, yi, zi, xil,
                                            yil, zil, h2c, h2, hinv, hinv3, ax, ay, az, aq);
-        } else if constexpr (MF) {
-            tile_mfma(sh.sj[buf], &sh.sB[buf][0][0], tid & 3, xi[0], yi[0], zi[0], D0, D1);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                S[v] += (double)D0[v] + (double)D1[v];
-                D0[v] = 0.0f;
-                D1[v] = 0.0f;
-            }
+        } else if constexpr (MF != 0) {
+            if (MF == 2 && d2 > mf2_d2)
+                tile_mfma2(sh.sj[buf], &sh.sQ[MF == 2 ? buf : 0][0], &sh.sB[buf][0][0], tid & 3, b2x,
+                           b2y, b2z, ci, S);
+            else
+                tile_mfma(sh.sj[buf], &sh.sB[buf][0][0], tid & 3, xi[0], yi[0], zi[0], S);
         } else {
             tile_f32<TPT, false, POT>(sh.sj[buf], xi, yi, zi, h2c, h2, hinv, hinv3, ax, ay, az, aq);
         }
@@ -367,7 +444,7 @@ __device__ __forceinline__ void force_block(
             if (POT) { Aq[k] += (double)aq[k]; aq[k] = 0.0f; }
         }
     }
-    if constexpr (MF) {
+    if constexpr (MF != 0) {
         // Lane t = 4b + i collects target t's four columns: column c sits in
         // S[i] of lane 4b + c.
         const int lane = tid & 63, qb = lane & ~3, i = lane & 3;

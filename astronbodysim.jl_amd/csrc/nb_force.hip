@@ -4,18 +4,11 @@
 // thread per target, sources re-read from global memory) with an LDS-tiled
 // kernel: each block owns BS*TPT targets held in registers and streams the
 // sources through a double-buffered LDS stage of BS particles; every lane
-// reads the same source (LDS broadcast).  Pair arithmetic is FP32 (v_rsq_f32,
-// no MFMA); each tile's FP32 partial sums are folded into FP64 accumulators.
-//
-// Precision of close pairs (SPLIT): positions are FP64 in HBM; the block
-// picks an FP64 origin (the centroid of its targets) and both targets and the
-// staged sources are converted to FP32 *relative to that origin*, so the
-// rounding error of a difference x_j - x_i is set by the block's extent, not
-// by the absolute coordinate.
-//
-// Softening test: tiles whose bounding box is separated from the block's
-// target box by more than the largest target h are evaluated without the
-// per-pair r < h test (pure Newtonian); all other tiles take the checked path.
+// reads the same source (LDS broadcast).  Pair arithmetic is FP32 (v_rsq_f32);
+// each tile's FP32 partial sums are folded into FP64 accumulators.  The block
+// routine itself, with its per-wave tile classes (softening-checked, hi/lo
+// Newtonian, far) and the optional matrix-core far tiles, is force_block in
+// nb_block.h.
 #include <hip/hip_runtime.h>
 #include <float.h>
 #include <stdint.h>
@@ -26,17 +19,10 @@
 
 namespace nb {
 
-// FP32-pair kernel (see force_block in nb_block.h).  T = double (SPLIT: FP64 state, per-block origin, hi/lo
+// FP32-pair kernel.  T = double (SPLIT: FP64 state, per-block origin, hi/lo
 // coordinates on near tiles) or float (FP32 state, absolute coordinates).
-//
-// Tile classification (block-uniform): a source tile is "far" when its box is
-// separated from the block's target box by more than G = max(h_max, R) along
-// some axis, where R is the block's radius about its origin (SPLIT) or 0
-// (FP32).  Far tiles run the 12-VALU Newtonian loop on hi coordinates: every
-// pair there has r > h (no softening) and r > R, so the separation carries a
-// relative rounding error below 4 FP32 epsilons.  Near tiles run the
-// softening-checked loop (on hi/lo coordinates in SPLIT).
-template <typename T, int TPT, bool POT, bool MF = false>
+// MF selects the matrix-core far tiles (SPLIT, TPT 1, forces only).
+template <typename T, int TPT, bool POT, int MF = 0>
 __global__ __launch_bounds__(BS) void k_force_f32(ForceArgs<T> a) {
     __shared__ BlockShared<T, MF> sh;
     const int tid = threadIdx.x;
@@ -226,11 +212,12 @@ static int launch_f32_impl(const ForceArgs<T> &a, int tpt, bool pot, hipStream_t
 int launch_force_f32(const ForceArgs<float> &a, int tpt, bool pot, hipStream_t s) {
     return launch_f32_impl<float>(a, tpt, pot, s);
 }
-int launch_force_split(const ForceArgs<double> &a, int tpt, bool pot, hipStream_t s, bool mfma) {
+int launch_force_split(const ForceArgs<double> &a, int tpt, bool pot, hipStream_t s, int mfma) {
     if (mfma && tpt == 1 && !pot) {
         const int64_t nb = (int64_t)a.nblk_i * a.nseg;
         if (nb <= 0) return NB_OK;
-        k_force_f32<double, 1, false, true><<<(unsigned)nb, BS, 0, s>>>(a);
+        if (mfma == 2) k_force_f32<double, 1, false, 2><<<(unsigned)nb, BS, 0, s>>>(a);
+        else k_force_f32<double, 1, false, 1><<<(unsigned)nb, BS, 0, s>>>(a);
         return check_launch("k_force_f32<mfma>");
     }
     return launch_f32_impl<double>(a, tpt, pot, s);

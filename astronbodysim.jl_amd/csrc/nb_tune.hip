@@ -93,11 +93,11 @@ int autotune_ctx(nb_ctx *c) {
     nb_tuning best = keep;
     float best_ms = 1e30f;
     // A single evaluation is timed per candidate when it is long.
-    // candidates: (tpt, mfma) in {(1,0), (2,0), (4,0)} plus (1,1) in the SPLIT modes
+    // candidates: (tpt, mfma) in {(1,0), (2,0), (4,0)} plus (1,1), (1,2) in the SPLIT modes
     const bool split = c->mode == NB_MODE_SPLIT || c->mode == NB_MODE_SPLIT_CENTRAL;
-    for (int ti = 0; ti < (split ? 4 : 3); ++ti) {
+    for (int ti = 0; ti < (split ? 5 : 3); ++ti) {
         const int tpt = ti < 3 ? tpts[ti] : 1;
-        const int mf = ti == 3 ? 1 : 0;
+        const int mf = ti < 3 ? 0 : ti - 2;
         if (c->mode == NB_MODE_FP64 && tpt != 1) continue;
         const int64_t nblk = (n + (int64_t)BS * tpt - 1) / ((int64_t)BS * tpt);
         for (int si = 0; si < 8; ++si) {

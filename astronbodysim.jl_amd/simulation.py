@@ -274,7 +274,7 @@ class Simulation:
             constants=Constants(float(G), float(acc0)),
             output=OutputConfig(os.path.join(os.getcwd(), "output") if OutputDir is None else OutputDir,
                                 output_func, bool(pot), bool(SaveRestart),
-                                float(SaveRestartFreq)),
+                                float(SaveRestartFreq), bool(format2), bool(acc)),
             floattype=floattype,
         )
         self.pids = list(pids)

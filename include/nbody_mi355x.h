@@ -76,8 +76,8 @@ typedef struct nb_tuning {
     int32_t targets_per_thread; /* 1, 2 or 4                                   */
     int32_t j_segments;         /* source-range split (>=1; partials reduced in fixed order) */
     int32_t sort_interval;      /* re-sort into Morton order every k force calls (0 = never) */
-    int32_t mfma;               /* 1: far tiles accumulate on the matrix cores (SPLIT modes,
-                                   targets_per_thread 1, acceleration-only evaluations) */
+    int32_t mfma;               /* far tiles on the matrix cores (SPLIT modes, targets_per_thread
+                                   1, forces only): 1 the sum, 2 also r^2 on the farthest tiles */
 } nb_tuning;
 
 /* ---- library / device ------------------------------------------------- */
@@ -255,7 +255,8 @@ typedef struct nb_batch_tuning {
     int32_t steps_per_graph;    /* KDK steps per captured HIP graph (even, >= 2)     */
     int32_t sort_interval;      /* re-sort every system into Morton order every k force calls (0 = never) */
     int32_t resident;           /* 1: multi-step resident kernel when every system has <= 256 particles */
-    int32_t mfma;               /* 1: far tiles accumulate on the matrix cores (targets_per_thread 1) */
+    int32_t mfma;               /* 1: far tiles accumulate on the matrix cores, 2: r^2 too
+                                   (targets_per_thread 1) */
     int32_t reserved[2];
 } nb_batch_tuning;
 

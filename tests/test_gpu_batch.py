@@ -39,7 +39,7 @@ def make(pkg, ens, mode="split", dt=1e-3, **tun):
 
 @pytest.mark.parametrize("mode,tol,tpt,mfma", [("split", GATE, 1, 0), ("split", GATE, 2, 0),
                                                ("split", GATE, 4, 0), ("split", GATE, 1, 1),
-                                               ("fp64", 1e-12, 1, 0)])
+                                               ("split", GATE, 1, 2), ("fp64", 1e-12, 1, 0)])
 def test_ragged_force(pkg, orc, clusters, mode, tol, tpt, mfma):
     ref, _ = oracle_forces(orc, clusters)
     with make(pkg, clusters, mode, targets_per_thread=tpt, blocks=37, resident=0, mfma=mfma) as e:
