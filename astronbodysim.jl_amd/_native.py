@@ -66,7 +66,7 @@ class Tuning(C.Structure):
         ("targets_per_thread", C.c_int32),
         ("j_segments", C.c_int32),
         ("sort_interval", C.c_int32),
-        ("mfma", C.c_int32),
+        ("far_path", C.c_int32),
     ]
 
 
@@ -360,15 +360,15 @@ class Context:
 
     # -- tuning / timing -----------------------------------------------------
     def set_tuning(self, targets_per_thread: int, j_segments: int, sort_interval: int = 0,
-                   mfma: int = 0):
-        t = Tuning(int(targets_per_thread), int(j_segments), int(sort_interval), int(mfma))
+                   far_path: int = 0):
+        t = Tuning(int(targets_per_thread), int(j_segments), int(sort_interval), int(far_path))
         call("nb_set_tuning", self.handle, C.byref(t))
 
     def get_tuning(self) -> dict:
         t = Tuning()
         call("nb_get_tuning", self.handle, C.byref(t))
         return {"targets_per_thread": t.targets_per_thread, "j_segments": t.j_segments,
-                "sort_interval": t.sort_interval, "mfma": t.mfma}
+                "sort_interval": t.sort_interval, "far_path": t.far_path}
 
     def autotune(self):
         call("nb_autotune", self.handle)

@@ -201,7 +201,7 @@ int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = tru
         a.skip_src = (c->mode == NB_MODE_SPLIT_CENTRAL && c->ncentral > 0) ? c->central : nullptr;
         a.accumulate = 0;
         if (f64) NB_TRY(launch_force_f64(a, with_pot, s));
-        else NB_TRY(launch_force_split(a, tpt, with_pot, s, c->tun.mfma));
+        else NB_TRY(launch_force_split(a, tpt, with_pot, s, c->tun.far_path));
     } else {
         ForceArgs<float> a{};
         a.tx = (const float *)c->x; a.ty = (const float *)c->y; a.tz = (const float *)c->z;
@@ -716,7 +716,8 @@ int nb_kdk_steps(nb_ctx *c, int64_t nsteps, double dt, double t_begin, int32_t u
         const int64_t chunk = std::min<int64_t>(nsteps & ~(int64_t)1, 16);
         const bool reuse = c->gexec && c->g_steps == chunk && c->g_dt == dt &&
                            c->g_tun.targets_per_thread == c->tun.targets_per_thread &&
-                           c->g_tun.j_segments == c->tun.j_segments && c->g_nbg == c->bg.size();
+                           c->g_tun.j_segments == c->tun.j_segments &&
+                           c->g_tun.far_path == c->tun.far_path && c->g_nbg == c->bg.size();
         if (!reuse) {
             drop_graph(c);
             // Make sure partial buffers exist before capture (no allocation inside).
@@ -859,8 +860,8 @@ int nb_set_tuning(nb_ctx *c, const nb_tuning *t) {
     NB_TRY(check_ctx(c, false));
     if (!t || (t->targets_per_thread != 1 && t->targets_per_thread != 2 &&
                t->targets_per_thread != 4) ||
-        t->j_segments < 1 || t->j_segments > 256 || t->sort_interval < 0 || t->mfma < 0 ||
-        t->mfma > 2) {
+        t->j_segments < 1 || t->j_segments > 256 || t->sort_interval < 0 || t->far_path < 0 ||
+        t->far_path > 3) {
         set_error("invalid tuning");
         return NB_ERR_INVALID;
     }

@@ -128,7 +128,7 @@ __global__ __launch_bounds__(BS) void k_batch_force(BatchArgs a) {
 #pragma unroll
         for (int q = 0; q < TPT; ++q) {
             if (!valid[q]) continue;
-            const int64_t i = i0 + q * BS + tid;
+            const int64_t i = target_of<TPT>(i0, q, tid);
             a.ax[i] = G * Ax[q];
             a.ay[i] = G * Ay[q];
             a.az[i] = G * Az[q];
@@ -637,8 +637,12 @@ static int bforce(nb_batch *b, bool with_pot, bool allow_timing) {
         int nblk = b->tun.blocks > 0 ? b->tun.blocks : 8 * b->cu_count;
         nblk = (int)std::max<int64_t>(1, std::min<int64_t>(nblk, b->nitems));
 #define NB_BL(T_, P_) k_batch_force<T_, P_><<<nblk, BS, 0, s>>>(a)
-        if (tpt == 1 && !with_pot && b->tun.mfma == 2) k_batch_force<1, false, 2><<<nblk, BS, 0, s>>>(a);
-        else if (tpt == 1 && !with_pot && b->tun.mfma) k_batch_force<1, false, 1><<<nblk, BS, 0, s>>>(a);
+        const int fp = with_pot ? FAR_VALU : b->tun.far_path;
+        if (tpt == 1 && fp == FAR_MFMA_R2) k_batch_force<1, false, FAR_MFMA_R2><<<nblk, BS, 0, s>>>(a);
+        else if (tpt == 1 && fp == FAR_MFMA_SUM) k_batch_force<1, false, FAR_MFMA_SUM><<<nblk, BS, 0, s>>>(a);
+        else if (fp == FAR_WEIGHTED && tpt == 1) k_batch_force<1, false, FAR_WEIGHTED><<<nblk, BS, 0, s>>>(a);
+        else if (fp == FAR_WEIGHTED && tpt == 2) k_batch_force<2, false, FAR_WEIGHTED><<<nblk, BS, 0, s>>>(a);
+        else if (fp == FAR_WEIGHTED && tpt == 4) k_batch_force<4, false, FAR_WEIGHTED><<<nblk, BS, 0, s>>>(a);
         else if (tpt == 1) { if (with_pot) NB_BL(1, true); else NB_BL(1, false); }
         else if (tpt == 2) { if (with_pot) NB_BL(2, true); else NB_BL(2, false); }
         else { if (with_pot) NB_BL(4, true); else NB_BL(4, false); }
@@ -1154,7 +1158,7 @@ int nb_batch_set_tuning(nb_batch *b, const nb_batch_tuning *t) {
     if (!t || (t->targets_per_thread != 1 && t->targets_per_thread != 2 &&
                t->targets_per_thread != 4) ||
         t->blocks < 0 || t->blocks > (1 << 20) || t->steps_per_graph < 2 ||
-        t->steps_per_graph > 4096 || t->sort_interval < 0 || t->mfma < 0 || t->mfma > 2) {
+        t->steps_per_graph > 4096 || t->sort_interval < 0 || t->far_path < 0 || t->far_path > 3) {
         set_error("invalid batch tuning");
         return NB_ERR_INVALID;
     }
@@ -1180,16 +1184,17 @@ int nb_batch_autotune(nb_batch *b) {
     const nb_batch_tuning keep = b->tun;
     nb_batch_tuning best = keep;
     float best_ms = 1e30f;
-    const int tpts[3] = {1, 2, 4};
     const int mult[4] = {2, 4, 8, 16};
     hipEvent_t e0, e1;
     NB_HIP(hipEventCreate(&e0));
     NB_HIP(hipEventCreate(&e1));
     int rc = NB_OK;
-    for (int ti = 0; ti < 5 && rc == NB_OK; ++ti) {
+    // (targets per thread, far path) as in the one-system autotuner
+    const int cand[8][2] = {{1, 0}, {2, 0}, {4, 0}, {1, 3}, {2, 3}, {4, 3}, {1, 1}, {1, 2}};
+    for (int ti = 0; ti < 8 && rc == NB_OK; ++ti) {
         for (int mi = 0; mi < 4 && rc == NB_OK; ++mi) {
-            b->tun.targets_per_thread = ti < 3 ? tpts[ti] : 1;
-            b->tun.mfma = ti < 3 ? 0 : ti - 2;
+            b->tun.targets_per_thread = cand[ti][0];
+            b->tun.far_path = cand[ti][1];
             b->tun.blocks = mult[mi] * b->cu_count;
             rc = bforce(b, false, false);
             if (rc) break;

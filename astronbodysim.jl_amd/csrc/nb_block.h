@@ -176,6 +176,24 @@ __device__ __forceinline__ void tile_f32_hilo(const float4 *__restrict__ s,
 // <= (1 + 2 / MF2_REACH) r and the loss stays below about 8 eps.  Tiles in
 // between take the MF = 1 loop.
 constexpr double MF2_REACH = 3.0;
+// Far tiles with r >= FARW_REACH * Re in SPLIT mode ("weighted" far tiles):
+//   f_ij = m_j r_ij^-3 = q^3,  q = rsq(w_j r_ij^2),  w_j = m_j^(-2/3),
+//   w_j r_ij^2 = w_j |x'_j|^2 + w_j |x'_i|^2 - 2 (w_j x'_j) . x'_i,
+// four FMAs from the source's (w x', w y', w z', w |x'|^2, w) and the target's
+// (-2x'_i, -2y'_i, -2z'_i, |x'_i|^2), with the sum kept as
+//   S = sum_j f_ij (x'_j, y'_j, z'_j, 1),   a_i = S_xyz - x'_i S_m:
+// ten VALU and one v_rsq_f32 per pair against twelve and one.  The expansion
+// loses about 3 eps (|x'_i| + |x'_j|)^2 / r^2 <= 12 eps relative in w r^2 for
+// r >= 2 Re, since |x'_j| <= r + Re.  Massless sources get w = 0 and
+// w |x'|^2 = 2^120, so their f is exactly zero.
+constexpr double FARW_REACH = 2.0;
+
+// Far-tile paths (the far_path tuning value).
+constexpr int FAR_VALU = 0;       // 12 VALU + rsq per pair
+constexpr int FAR_MFMA_SUM = 1;   // sum on the matrix cores
+constexpr int FAR_MFMA_R2 = 2;    // r^2 and sum on the matrix cores (farthest tiles)
+constexpr int FAR_WEIGHTED = 3;   // weighted far tiles: 10 VALU + rsq per pair
+constexpr float FARW_MASSLESS = 1.329228e36f;   // 2^120
 
 // Shared memory of one all-pairs block.
 template <typename T, int MF = 0>
@@ -183,8 +201,12 @@ struct BlockShared {
     static constexpr bool HILO = sizeof(T) == 8;
     float4 sj[2][BS];
     float4 sjl[HILO ? 2 : 1][HILO ? BS : 1];
-    float sB[MF ? 2 : 1][MF ? 4 : 1][MF ? BS : 1];   // MFMA B columns, component-major
+    static constexpr bool COLS = MF == 1 || MF == 2;   // the MFMA far paths
+    float sB[COLS ? 2 : 1][COLS ? 4 : 1][COLS ? BS : 1];   // MFMA B columns, component-major
     float sQ[MF == 2 ? 2 : 1][MF == 2 ? BS : 1];     // |x'_j|^2 of each source
+    static constexpr bool WFAR = HILO && MF == 3;
+    float4 sW[WFAR ? 2 : 1][WFAR ? BS : 1];          // (w x', w y', w z', w |x'|^2)
+    float sw[WFAR ? 2 : 1][WFAR ? BS : 1];           // w = m^(-2/3)
     double red[11][NW];
 };
 
@@ -266,7 +288,120 @@ __device__ __forceinline__ void tile_mfma2(const float4 *__restrict__ s, const f
     for (int v = 0; v < 4; ++v) S[v] += (double)D0[v] + (double)D1[v];
 }
 
-// One block's share of an all-pairs sum: targets i0 + k*BS + tid (k < TPT,
+// Weighted far tile (see FARW_REACH): TPT targets per lane, S-form sums.
+template <int TPT>
+__device__ __forceinline__ void tile_w(const float4 *__restrict__ s, const float4 *__restrict__ sW,
+                                       const float *__restrict__ sw, const float (&bx)[TPT],
+                                       const float (&by)[TPT], const float (&bz)[TPT],
+                                       const float (&ci)[TPT], float (&Sx)[TPT], float (&Sy)[TPT],
+                                       float (&Sz)[TPT], float (&Sm)[TPT]) {
+#pragma unroll 1
+    for (int j = 0; j < BS; ++j) {
+        const float4 W = sW[j];
+        const float w = sw[j];
+        const float4 p = s[j];
+#pragma unroll
+        for (int k = 0; k < TPT; ++k) {
+            const float r2w = __builtin_fmaf(
+                W.x, bx[k], __builtin_fmaf(W.y, by[k], __builtin_fmaf(W.z, bz[k], __builtin_fmaf(w, ci[k], W.w))));
+            const float q = __builtin_amdgcn_rsqf(r2w);
+            const float f = (q * q) * q;
+            Sx[k] = __builtin_fmaf(f, p.x, Sx[k]);
+            Sy[k] = __builtin_fmaf(f, p.y, Sy[k]);
+            Sz[k] = __builtin_fmaf(f, p.z, Sz[k]);
+            Sm[k] += f;
+        }
+    }
+}
+
+// A wave-uniform double moved to scalar registers.
+__device__ __forceinline__ double uniform_f64(double v) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Per-wave tile classes.  Each wave uses the box of its own targets, their
+// largest h, their largest distance R from the block origin per axis and
+// their largest Euclidean distance Re from it.  A margin keeps every far pair
+// at FP32-resolvable, nonzero separation.  The main kernel and the matrix-core
+// far-tile kernel both decide with these two functions, evaluated without FMA
+// contraction, so that each (wave, tile) pair is summed by exactly one of them.
+struct WaveGeom {
+    double lo[3], hi[3];
+    double soft_d2, close_d2, farw_d2;
+};
+
+template <int TPT>
+__device__ __forceinline__ WaveGeom wave_geometry(const double (&px)[TPT], const double (&py)[TPT],
+                                                  const double (&pz)[TPT], const double (&ph)[TPT],
+                                                  const bool (&valid)[TPT], double ox, double oy,
+                                                  double oz, bool hilo) {
+#pragma clang fp contract(off)
+    WaveGeom w;
+    double whmax = 0.0, re2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        w.lo[d] = DBL_MAX;
+        w.hi[d] = -DBL_MAX;
+    }
+#pragma unroll
+    for (int k = 0; k < TPT; ++k) {
+        if (!valid[k]) continue;
+        w.lo[0] = fmin(w.lo[0], px[k]); w.hi[0] = fmax(w.hi[0], px[k]);
+        w.lo[1] = fmin(w.lo[1], py[k]); w.hi[1] = fmax(w.hi[1], py[k]);
+        w.lo[2] = fmin(w.lo[2], pz[k]); w.hi[2] = fmax(w.hi[2], pz[k]);
+        whmax = fmax(whmax, ph[k]);
+        const double dx = px[k] - ox, dy = py[k] - oy, dz = pz[k] - oz;
+        re2 = fmax(re2, dx * dx + dy * dy + dz * dz);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        w.lo[d] = wave_min(w.lo[d]);
+        w.hi[d] = wave_max(w.hi[d]);
+    }
+    whmax = wave_max(whmax);
+    const double re = sqrt(wave_max(re2));
+    double rb = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double o = d == 0 ? ox : (d == 1 ? oy : oz);
+        if (w.hi[d] >= w.lo[d]) rb = fmax(rb, fmax(fabs(w.lo[d] - o), fabs(w.hi[d] - o)));
+    }
+    const double soft_d = whmax * (1.0 + 1e-6) + 1e-6 * rb + 1e-300;
+    const double close_d = hilo ? fmax(soft_d, 0.5 * rb) : soft_d;
+    const double farw_d = fmax(close_d, FARW_REACH * re);
+    w.soft_d2 = uniform_f64(soft_d * soft_d);
+    w.close_d2 = uniform_f64(close_d * close_d);
+    w.farw_d2 = uniform_f64(farw_d * farw_d);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        w.lo[d] = uniform_f64(w.lo[d]);
+        w.hi[d] = uniform_f64(w.hi[d]);
+    }
+    return w;
+}
+
+// Squared Euclidean gap between a source tile's box (tb: lo xyz, hi xyz) and
+// the wave's target box.
+__device__ __forceinline__ double tile_gap2(const double *__restrict__ tb, const WaveGeom &w) {
+#pragma clang fp contract(off)
+    const double gx = fmax(0.0, fmax(tb[0] - w.hi[0], w.lo[0] - tb[3]));
+    const double gy = fmax(0.0, fmax(tb[1] - w.hi[1], w.lo[1] - tb[4]));
+    const double gz = fmax(0.0, fmax(tb[2] - w.hi[2], w.lo[2] - tb[5]));
+    return gx * gx + gy * gy + gz * gz;
+}
+
+// Target k (< TPT) of thread tid in a block whose targets start at i0: each
+// wave owns 64*TPT consecutive targets, so that a wave's targets (sorted in
+// Morton order) stay compact and its tile classes stay tight.
+template <int TPT>
+__device__ __forceinline__ int64_t target_of(int64_t i0, int k, int tid) {
+    return i0 + (int64_t)(tid >> 6) * (64 * TPT) + k * 64 + (tid & 63);
+}
+
+// One block's share of an all-pairs sum: targets target_of(i0, k, tid) (k < TPT,
 // i < i_end) against the sources [j0, j1).  tile_box points at the box of
 // the BS-source tile that starts at j0 (tiles are BS-aligned from j0).
 // Returns FP64 sums (without G) in Ax..Aq and the target validity.
@@ -279,7 +414,9 @@ __device__ __forceinline__ void force_block(
     const uint8_t *__restrict__ skip_src, bool use_origin, double (&Ax)[TPT], double (&Ay)[TPT],
     double (&Az)[TPT], double (&Aq)[TPT], bool (&valid)[TPT]) {
     constexpr bool HILO = BlockShared<T, MF>::HILO;
-    static_assert(!MF || (HILO && TPT == 1 && !POT), "MFMA far tiles: SPLIT, 1 target/lane, acc only");
+    static_assert(!(MF == FAR_MFMA_SUM || MF == FAR_MFMA_R2) || (HILO && TPT == 1 && !POT),
+                  "MFMA far tiles: SPLIT, 1 target/lane, acc only");
+    static_assert(MF != FAR_WEIGHTED || !POT, "weighted far tiles: acc only");
     const int tid = threadIdx.x;
     __syncthreads();  // the previous call's readers of sh are done
     double S[4] = {0.0, 0.0, 0.0, 0.0};
@@ -287,7 +424,7 @@ __device__ __forceinline__ void force_block(
     double px[TPT], py[TPT], pz[TPT], ph[TPT];
 #pragma unroll
     for (int k = 0; k < TPT; ++k) {
-        const int64_t i = i0 + k * BS + tid;
+        const int64_t i = target_of<TPT>(i0, k, tid);
         valid[k] = i < i_end;
         px[k] = valid[k] ? (double)tx[i] : 0.0;
         py[k] = valid[k] ? (double)ty[i] : 0.0;
@@ -318,39 +455,22 @@ __device__ __forceinline__ void force_block(
         ax[k] = ay[k] = az[k] = aq[k] = 0.0f;
         Ax[k] = Ay[k] = Az[k] = Aq[k] = 0.0;
     }
-    // Tiles are classified per wave (the branch is wave-uniform): each wave
-    // uses the box of its own targets, their largest h, and their largest
-    // distance R from the block origin.  A margin keeps every far pair at
-    // FP32-resolvable, nonzero separation.
-    double wlo[3], whi[3], whmax = 0.0;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        wlo[d] = DBL_MAX;
-        whi[d] = -DBL_MAX;
-    }
+    // Tiles are classified per wave (the branch is wave-uniform).
+    const WaveGeom wg = wave_geometry<TPT>(px, py, pz, ph, valid, ox, oy, oz, HILO);
+    const double soft_d2 = wg.soft_d2, close_d2 = wg.close_d2;
+    const double close_d = sqrt(close_d2);
+    // weighted far tiles: per-target operands and S-form sums
+    constexpr bool WFAR = BlockShared<T, MF>::WFAR && !POT;
+    float wbx[TPT], wby[TPT], wbz[TPT], wci[TPT], Sx[TPT], Sy[TPT], Sz[TPT], Sm[TPT];
 #pragma unroll
     for (int k = 0; k < TPT; ++k) {
-        if (!valid[k]) continue;
-        wlo[0] = fmin(wlo[0], px[k]); whi[0] = fmax(whi[0], px[k]);
-        wlo[1] = fmin(wlo[1], py[k]); whi[1] = fmax(whi[1], py[k]);
-        wlo[2] = fmin(wlo[2], pz[k]); whi[2] = fmax(whi[2], pz[k]);
-        whmax = fmax(whmax, ph[k]);
+        const double dx = px[k] - ox, dy = py[k] - oy, dz = pz[k] - oz;
+        wbx[k] = -2.0f * xi[k];
+        wby[k] = -2.0f * yi[k];
+        wbz[k] = -2.0f * zi[k];
+        wci[k] = (float)(dx * dx + dy * dy + dz * dz);
+        Sx[k] = Sy[k] = Sz[k] = Sm[k] = 0.0f;
     }
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        wlo[d] = wave_min(wlo[d]);
-        whi[d] = wave_max(whi[d]);
-    }
-    whmax = wave_max(whmax);
-    double rb = 0.0;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const double o = d == 0 ? ox : (d == 1 ? oy : oz);
-        if (whi[d] >= wlo[d]) rb = fmax(rb, fmax(fabs(wlo[d] - o), fabs(whi[d] - o)));
-    }
-    const double soft_d = whmax * (1.0 + 1e-6) + 1e-6 * rb + 1e-300;
-    const double close_d = HILO ? fmax(soft_d, 0.5 * rb) : soft_d;
-    const double soft_d2 = soft_d * soft_d, close_d2 = close_d * close_d;
     // MF = 2: the Euclidean reach of the wave's targets and their MFMA operands
     double mf2_d2 = DBL_MAX;
     float b2x = 0.0f, b2y = 0.0f, b2z = 0.0f;
@@ -367,7 +487,6 @@ __device__ __forceinline__ void force_block(
         const float c = (float)re2;
         ci = nb_f4v{c, c, c, c};
     }
-
     // Register prefetch of the first tile.
     T rx = 0, ry = 0, rz = 0, rm = 0;
     bool rv = false;
@@ -387,7 +506,7 @@ __device__ __forceinline__ void force_block(
                 sh.sjl[HILO ? buf : 0][HILO ? tid : 0] =
                     make_float4((float)(dx - (double)hx), (float)(dy - (double)hy),
                                 (float)(dz - (double)hz), 0.0f);
-            if constexpr (MF) {
+            if constexpr (MF == 1 || MF == 2) {
                 const double md = (double)rm;
                 sh.sB[buf][0][tid] = (float)(md * dx);
                 sh.sB[buf][1][tid] = (float)(md * dy);
@@ -395,10 +514,25 @@ __device__ __forceinline__ void force_block(
                 sh.sB[buf][3][tid] = (float)md;
             }
             if constexpr (MF == 2) sh.sQ[MF == 2 ? buf : 0][MF == 2 ? tid : 0] = (float)(dx * dx + dy * dy + dz * dz);
+            if constexpr (WFAR) {
+                const double q2 = (double)hx * hx + (double)hy * hy + (double)hz * hz;
+                float4 W;
+                float w;
+                if (rm > 0) {
+                    const double wd = 1.0 / cbrt((double)rm * (double)rm);
+                    W = make_float4((float)(wd * hx), (float)(wd * hy), (float)(wd * hz), (float)(wd * q2));
+                    w = (float)wd;
+                } else {
+                    W = make_float4(0.0f, 0.0f, 0.0f, FARW_MASSLESS);
+                    w = 0.0f;
+                }
+                sh.sW[WFAR ? buf : 0][WFAR ? tid : 0] = W;
+                sh.sw[WFAR ? buf : 0][WFAR ? tid : 0] = w;
+            }
         } else {
             sh.sj[buf][tid] = make_float4(PAD_COORD, PAD_COORD, PAD_COORD, 0.0f);
             if (HILO) sh.sjl[HILO ? buf : 0][HILO ? tid : 0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if constexpr (MF) {
+            if constexpr (MF == 1 || MF == 2) {
                 sh.sB[buf][0][tid] = 0.0f;
                 sh.sB[buf][1][tid] = 0.0f;
                 sh.sB[buf][2][tid] = 0.0f;
@@ -406,6 +540,10 @@ __device__ __forceinline__ void force_block(
             }
             // |PAD|^2: r^2 ~ 3e36 keeps r^-3 at zero (and finite) for padding
             if constexpr (MF == 2) sh.sQ[MF == 2 ? buf : 0][MF == 2 ? tid : 0] = 3.0f * PAD_COORD * PAD_COORD;
+            if constexpr (WFAR) {
+                sh.sW[WFAR ? buf : 0][WFAR ? tid : 0] = make_float4(0.0f, 0.0f, 0.0f, FARW_MASSLESS);
+                sh.sw[WFAR ? buf : 0][WFAR ? tid : 0] = 0.0f;
+            }
         }
         __syncthreads();
         {
@@ -413,11 +551,7 @@ __device__ __forceinline__ void force_block(
             rv = j < j1 && !(skip_src && skip_src[j]);
             if (rv) { rx = sx[j]; ry = sy[j]; rz = sz[j]; rm = sm[j]; }
         }
-        const double *tb = tile_box + ((base - j0) / BS) * 8;
-        const double gx = fmax(0.0, fmax(tb[0] - whi[0], wlo[0] - tb[3]));
-        const double gy = fmax(0.0, fmax(tb[1] - whi[1], wlo[1] - tb[4]));
-        const double gz = fmax(0.0, fmax(tb[2] - whi[2], wlo[2] - tb[5]));
-        const double d2 = gx * gx + gy * gy + gz * gz;
+        const double d2 = tile_gap2(tile_box + ((base - j0) / BS) * 8, wg);
         if (d2 <= soft_d2) {
             if (HILO)
                 tile_f32_hilo<TPT, true, POT>(sh.sj[buf], sh.sjl[HILO ? buf : 0], xi, yi, zi, xil,
@@ -428,7 +562,18 @@ __device__ __forceinline__ void force_block(
         } else if (HILO && d2 <= close_d2) {
             tile_f32_hilo<TPT, false, POT>(sh.sj[buf], sh.sjl[HILO ? buf : 0], xi, yi, zi, xil,
                                            yil, zil, h2c, h2, hinv, hinv3, ax, ay, az, aq);
-        } else if constexpr (MF != 0) {
+        } else if (WFAR && d2 > wg.farw_d2) {
+            tile_w<TPT>(sh.sj[buf], sh.sW[WFAR ? buf : 0], sh.sw[WFAR ? buf : 0], wbx, wby, wbz, wci,
+                        Sx, Sy, Sz, Sm);
+#pragma unroll
+            for (int k = 0; k < TPT; ++k) {
+                const double sm = (double)Sm[k];
+                Ax[k] += (double)Sx[k] - (px[k] - ox) * sm;
+                Ay[k] += (double)Sy[k] - (py[k] - oy) * sm;
+                Az[k] += (double)Sz[k] - (pz[k] - oz) * sm;
+                Sx[k] = Sy[k] = Sz[k] = Sm[k] = 0.0f;
+            }
+        } else if constexpr (MF == 1 || MF == 2) {
             if (MF == 2 && d2 > mf2_d2)
                 tile_mfma2(sh.sj[buf], &sh.sQ[MF == 2 ? buf : 0][0], &sh.sB[buf][0][0], tid & 3, b2x,
                            b2y, b2z, ci, S);
@@ -444,7 +589,7 @@ __device__ __forceinline__ void force_block(
             if (POT) { Aq[k] += (double)aq[k]; aq[k] = 0.0f; }
         }
     }
-    if constexpr (MF != 0) {
+    if constexpr (MF == 1 || MF == 2) {
         // Lane t = 4b + i collects target t's four columns: column c sits in
         // S[i] of lane 4b + c.
         const int lane = tid & 63, qb = lane & ~3, i = lane & 3;

@@ -23,7 +23,7 @@ namespace nb {
 // coordinates on near tiles) or float (FP32 state, absolute coordinates).
 // MF selects the matrix-core far tiles (SPLIT, TPT 1, forces only).
 template <typename T, int TPT, bool POT, int MF = 0>
-__global__ __launch_bounds__(BS) void k_force_f32(ForceArgs<T> a) {
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(3))) void k_force_f32(ForceArgs<T> a) {
     __shared__ BlockShared<T, MF> sh;
     const int tid = threadIdx.x;
     const int logical = xcd_remap(blockIdx.x, gridDim.x);
@@ -40,7 +40,7 @@ __global__ __launch_bounds__(BS) void k_force_f32(ForceArgs<T> a) {
 #pragma unroll
     for (int k = 0; k < TPT; ++k) {
         if (!valid[k]) continue;
-        const int64_t i = i0 + k * BS + tid;
+        const int64_t i = target_of<TPT>(i0, k, tid);
         if (a.nseg == 1) {
             a.ax[i] = a.G * Ax[k];
             a.ay[i] = a.G * Ay[k];
@@ -188,7 +188,7 @@ int launch_tile_boxes(const void *x, const void *y, const void *z, int is_double
     return check_launch("k_tile_boxes");
 }
 
-template <typename T>
+template <typename T, int MF>
 static int launch_f32_impl(const ForceArgs<T> &a, int tpt, bool pot, hipStream_t s) {
     const int64_t nb = (int64_t)a.nblk_i * a.nseg;
     if (nb <= 0) return NB_OK;
@@ -197,7 +197,7 @@ static int launch_f32_impl(const ForceArgs<T> &a, int tpt, bool pot, hipStream_t
         return NB_ERR_INVALID;
     }
     dim3 g((unsigned)nb), b(BS);
-#define NB_L(TPT_, POT_) k_force_f32<T, TPT_, POT_><<<g, b, 0, s>>>(a)
+#define NB_L(TPT_, POT_) k_force_f32<T, TPT_, POT_, (POT_ ? 0 : MF)><<<g, b, 0, s>>>(a)
     if (tpt == 1) { if (pot) NB_L(1, true); else NB_L(1, false); }
     else if (tpt == 2) { if (pot) NB_L(2, true); else NB_L(2, false); }
     else if (tpt == 4) { if (pot) NB_L(4, true); else NB_L(4, false); }
@@ -210,17 +210,18 @@ static int launch_f32_impl(const ForceArgs<T> &a, int tpt, bool pot, hipStream_t
 }
 
 int launch_force_f32(const ForceArgs<float> &a, int tpt, bool pot, hipStream_t s) {
-    return launch_f32_impl<float>(a, tpt, pot, s);
+    return launch_f32_impl<float, FAR_VALU>(a, tpt, pot, s);
 }
-int launch_force_split(const ForceArgs<double> &a, int tpt, bool pot, hipStream_t s, int mfma) {
-    if (mfma && tpt == 1 && !pot) {
+int launch_force_split(const ForceArgs<double> &a, int tpt, bool pot, hipStream_t s, int far_path) {
+    if ((far_path == FAR_MFMA_SUM || far_path == FAR_MFMA_R2) && tpt == 1 && !pot) {
         const int64_t nb = (int64_t)a.nblk_i * a.nseg;
         if (nb <= 0) return NB_OK;
-        if (mfma == 2) k_force_f32<double, 1, false, 2><<<(unsigned)nb, BS, 0, s>>>(a);
-        else k_force_f32<double, 1, false, 1><<<(unsigned)nb, BS, 0, s>>>(a);
+        if (far_path == FAR_MFMA_R2) k_force_f32<double, 1, false, FAR_MFMA_R2><<<(unsigned)nb, BS, 0, s>>>(a);
+        else k_force_f32<double, 1, false, FAR_MFMA_SUM><<<(unsigned)nb, BS, 0, s>>>(a);
         return check_launch("k_force_f32<mfma>");
     }
-    return launch_f32_impl<double>(a, tpt, pot, s);
+    if (far_path == FAR_WEIGHTED) return launch_f32_impl<double, FAR_WEIGHTED>(a, tpt, pot, s);
+    return launch_f32_impl<double, FAR_VALU>(a, tpt, pot, s);
 }
 
 int launch_force_f64(const ForceArgs<double> &a, bool pot, hipStream_t s) {

@@ -58,7 +58,7 @@ int launch_tile_boxes(const void *x, const void *y, const void *z, int is_double
                       double *box, hipStream_t s);
 int launch_force_f32(const ForceArgs<float> &a, int tpt, bool pot, hipStream_t s);
 int launch_force_split(const ForceArgs<double> &a, int tpt, bool pot, hipStream_t s,
-                       int mfma = 0);
+                       int far_path = 0);
 int launch_force_f64(const ForceArgs<double> &a, bool pot, hipStream_t s);
 int launch_finalize(const double *part, int nseg, int64_t n, double G, double *ax, double *ay,
                     double *az, double *pot, bool with_pot, hipStream_t s);

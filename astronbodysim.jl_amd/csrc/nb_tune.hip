@@ -88,16 +88,17 @@ int autotune_ctx(nb_ctx *c) {
     if (c->mode != NB_MODE_FP64) NB_TRY(morton_reorder(c));
     c->sort.interval = 0;
     const int64_t ntiles = (n + BS - 1) / BS;
-    const int tpts[3] = {1, 2, 4};
     const int segs[8] = {1, 2, 4, 8, 16, 32, 64, 128};
     nb_tuning best = keep;
     float best_ms = 1e30f;
     // A single evaluation is timed per candidate when it is long.
-    // candidates: (tpt, mfma) in {(1,0), (2,0), (4,0)} plus (1,1), (1,2) in the SPLIT modes
+    // candidates (targets per thread, far path); the matrix-core and weighted
+    // far paths exist in the SPLIT modes only
+    const int cand[8][2] = {{1, 0}, {2, 0}, {4, 0}, {1, 3}, {2, 3}, {4, 3}, {1, 1}, {1, 2}};
     const bool split = c->mode == NB_MODE_SPLIT || c->mode == NB_MODE_SPLIT_CENTRAL;
-    for (int ti = 0; ti < (split ? 5 : 3); ++ti) {
-        const int tpt = ti < 3 ? tpts[ti] : 1;
-        const int mf = ti < 3 ? 0 : ti - 2;
+    for (int ti = 0; ti < (split ? 8 : 3); ++ti) {
+        const int tpt = cand[ti][0];
+        const int mf = cand[ti][1];
         if (c->mode == NB_MODE_FP64 && tpt != 1) continue;
         const int64_t nblk = (n + (int64_t)BS * tpt - 1) / ((int64_t)BS * tpt);
         for (int si = 0; si < 8; ++si) {
@@ -109,7 +110,7 @@ int autotune_ctx(nb_ctx *c) {
             if (nblk * nseg < 64 && nseg < ntiles && si + 1 < 8) continue;  // far too few blocks
             c->tun.targets_per_thread = tpt;
             c->tun.j_segments = nseg;
-            c->tun.mfma = mf;
+            c->tun.far_path = mf;
             const int reps = (double)n * (double)n > 1e11 ? 1 : 3;
             float ms = 0.f;
             int rc = time_force(c, reps, &ms);
@@ -157,9 +158,9 @@ extern "C" int nb_tuner_save(const char *path) {
         const nb::TuneEntry &e = nb::g_table[k];
         fprintf(f,
                 "%s\n  {\"mode\": %d, \"log2n\": %d, \"targets_per_thread\": %d, "
-                "\"j_segments\": %d, \"sort_interval\": %d, \"mfma\": %d, \"ms\": %.6g}",
+                "\"j_segments\": %d, \"sort_interval\": %d, \"far_path\": %d, \"ms\": %.6g}",
                 k ? "," : "", e.mode, e.log2n, e.t.targets_per_thread, e.t.j_segments,
-                e.t.sort_interval, e.t.mfma, e.ms);
+                e.t.sort_interval, e.t.far_path, e.ms);
     }
     fprintf(f, "\n]}\n");
     fclose(f);
@@ -214,9 +215,9 @@ extern "C" int nb_tuner_load(const char *path) {
             return NB_ERR_INVALID;
         }
         json_int(obj, "sort_interval", &si);
-        json_int(obj, "mfma", &mf);
+        json_int(obj, "far_path", &mf);
         if ((tpt != 1 && tpt != 2 && tpt != 4) || nseg < 1 || nseg > 256 || mode < 0 || mode > 3 ||
-            si < 0 || mf < 0 || mf > 1 || (mf == 1 && tpt != 1)) {
+            si < 0 || mf < 0 || mf > 3 || ((mf == 1 || mf == 2) && tpt != 1)) {
             nb::set_error("%s: entry out of range", path);
             return NB_ERR_INVALID;
         }

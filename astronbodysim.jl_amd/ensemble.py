@@ -23,7 +23,7 @@ class BatchTuning(C.Structure):
         ("steps_per_graph", C.c_int32),
         ("sort_interval", C.c_int32),
         ("resident", C.c_int32),
-        ("mfma", C.c_int32),
+        ("far_path", C.c_int32),
         ("reserved", C.c_int32 * 2),
     ]
 
@@ -145,9 +145,9 @@ class Ensemble:
         call("nb_batch_sync", self.handle)
 
     def set_tuning(self, targets_per_thread=1, blocks=0, steps_per_graph=16, sort_interval=16,
-                   resident=1, mfma=0):
+                   resident=1, far_path=0):
         t = BatchTuning(int(targets_per_thread), int(blocks), int(steps_per_graph),
-                        int(sort_interval), int(resident), int(mfma))
+                        int(sort_interval), int(resident), int(far_path))
         call("nb_batch_set_tuning", self.handle, C.byref(t))
 
     def get_tuning(self) -> dict:
@@ -155,7 +155,7 @@ class Ensemble:
         call("nb_batch_get_tuning", self.handle, C.byref(t))
         return {"targets_per_thread": t.targets_per_thread, "blocks": t.blocks,
                 "steps_per_graph": t.steps_per_graph, "sort_interval": t.sort_interval,
-                "resident": t.resident, "mfma": t.mfma}
+                "resident": t.resident, "far_path": t.far_path}
 
     def autotune(self):
         call("nb_batch_autotune", self.handle)

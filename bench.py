@@ -328,7 +328,7 @@ def main():
                      "frac_unpacked": achieved / PEAK_FP32_UNPACKED_TFLOPS,
                      "traffic": traffic_from_profile(),
                      "kernel": "k_force_f32<double,%d,false,%s>" % (
-                         tuning["targets_per_thread"], "true" if tuning["mfma"] else "false"),
+                         tuning["targets_per_thread"], tuning["far_path"]),
                      "flops_per_pair": FLOPS_PER_PAIR,
                      "avg_kernel_ms": avg_force_s * 1e3, "launches": nforce},
         "cpu_baseline": cpu,

@@ -76,8 +76,10 @@ typedef struct nb_tuning {
     int32_t targets_per_thread; /* 1, 2 or 4                                   */
     int32_t j_segments;         /* source-range split (>=1; partials reduced in fixed order) */
     int32_t sort_interval;      /* re-sort into Morton order every k force calls (0 = never) */
-    int32_t mfma;               /* far tiles on the matrix cores (SPLIT modes, targets_per_thread
-                                   1, forces only): 1 the sum, 2 also r^2 on the farthest tiles */
+    int32_t far_path;           /* far-tile arithmetic (SPLIT modes; forces only, potentials take
+                                   0): 0 FP32 VALU, 12 ops + rsq per pair; 3 weighted VALU,
+                                   10 ops + rsq on the farthest tiles; 1 sum on the matrix
+                                   cores, 2 also r^2 there (1 and 2: targets_per_thread 1) */
 } nb_tuning;
 
 /* ---- library / device ------------------------------------------------- */
@@ -255,8 +257,7 @@ typedef struct nb_batch_tuning {
     int32_t steps_per_graph;    /* KDK steps per captured HIP graph (even, >= 2)     */
     int32_t sort_interval;      /* re-sort every system into Morton order every k force calls (0 = never) */
     int32_t resident;           /* 1: multi-step resident kernel when every system has <= 256 particles */
-    int32_t mfma;               /* 1: far tiles accumulate on the matrix cores, 2: r^2 too
-                                   (targets_per_thread 1) */
+    int32_t far_path;           /* as nb_tuning.far_path */
     int32_t reserved[2];
 } nb_batch_tuning;
 

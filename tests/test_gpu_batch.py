@@ -37,12 +37,14 @@ def make(pkg, ens, mode="split", dt=1e-3, **tun):
     return e
 
 
-@pytest.mark.parametrize("mode,tol,tpt,mfma", [("split", GATE, 1, 0), ("split", GATE, 2, 0),
+@pytest.mark.parametrize("mode,tol,tpt,far", [("split", GATE, 1, 0), ("split", GATE, 2, 0),
                                                ("split", GATE, 4, 0), ("split", GATE, 1, 1),
-                                               ("split", GATE, 1, 2), ("fp64", 1e-12, 1, 0)])
-def test_ragged_force(pkg, orc, clusters, mode, tol, tpt, mfma):
+                                               ("split", GATE, 1, 2), ("split", GATE, 1, 3),
+                                               ("split", GATE, 2, 3), ("split", GATE, 4, 3),
+                                               ("fp64", 1e-12, 1, 0)])
+def test_ragged_force(pkg, orc, clusters, mode, tol, tpt, far):
     ref, _ = oracle_forces(orc, clusters)
-    with make(pkg, clusters, mode, targets_per_thread=tpt, blocks=37, resident=0, mfma=mfma) as e:
+    with make(pkg, clusters, mode, targets_per_thread=tpt, blocks=37, resident=0, far_path=far) as e:
         e.compute_force()
         acc = e.download(pos=False, vel=False)["acc"]
     err = orc.rel_err(acc, ref)

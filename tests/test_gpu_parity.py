@@ -61,12 +61,13 @@ def test_potential_plummer4096(pkg, orc, plummer4096, mode, tol):
     assert np.allclose(pot_b - pot, mass / (0.01 / 2.8), rtol=1e-12)
 
 
-@pytest.mark.parametrize("tpt,nseg,sort,mfma", [
+@pytest.mark.parametrize("tpt,nseg,sort,far", [
     (1, 1, 0, 0), (1, 7, 0, 0), (2, 1, 0, 0), (2, 3, 0, 0), (4, 1, 0, 0), (4, 16, 0, 0),
-    (1, 1, 1, 1), (1, 5, 1, 1), (1, 1, 0, 1), (2, 2, 1, 0), (1, 1, 1, 2), (1, 3, 0, 2)])
-def test_launch_configurations_agree(pkg, orc, plummer4096, tpt, nseg, sort, mfma):
+    (1, 1, 1, 1), (1, 5, 1, 1), (1, 1, 0, 1), (2, 2, 1, 0), (1, 1, 1, 2), (1, 3, 0, 2), (1, 1, 1, 3), (2, 3, 1, 3),
+    (4, 2, 1, 3)])
+def test_launch_configurations_agree(pkg, orc, plummer4096, tpt, nseg, sort, far):
     pos, vel, mass, ref, _ = plummer4096
-    with make_ctx(pkg, pos, vel, mass, [0.01] * 6, "split", tuning=(tpt, nseg, sort, mfma)) as ctx:
+    with make_ctx(pkg, pos, vel, mass, [0.01] * 6, "split", tuning=(tpt, nseg, sort, far)) as ctx:
         ctx.compute_force()
         acc = ctx.download(pos=False, vel=False)["acc"]
     assert np.median(orc.rel_err(acc, ref)) <= GATE
@@ -188,18 +189,18 @@ def test_graph_replay_matches_eager(pkg):
         assert np.array_equal(out[0]["vel"], out[k]["vel"])
 
 
-@pytest.mark.parametrize("mfma", [0, 1, 2])
-def test_collision_far_tiles(pkg, orc, mfma):
+@pytest.mark.parametrize("far", [0, 1, 2, 3])
+def test_collision_far_tiles(pkg, orc, far):
     """Two separated Plummer spheres sorted into Morton order: most tiles are
     far (pure Newtonian on hi coordinates, or on the matrix cores)."""
     pos, vel, mass = pkg.ics.collision(8192, seeds=(3, 4))
     soft = [0.002] * 6
     ref = orc.compute_force(pos, mass, soft)
-    with make_ctx(pkg, pos, vel, mass, soft, "split", tuning=(1, 4, 16, mfma)) as ctx:
+    with make_ctx(pkg, pos, vel, mass, soft, "split", tuning=(1, 4, 16, far)) as ctx:
         ctx.compute_force()
         acc = ctx.download(pos=False, vel=False)["acc"]
     e = orc.rel_err(acc, ref)
-    print(f"mfma={mfma}: median {np.median(e):.3e} p99 {np.percentile(e, 99):.3e} max {e.max():.3e}")
+    print(f"far_path={far}: median {np.median(e):.3e} p99 {np.percentile(e, 99):.3e} max {e.max():.3e}")
     assert np.median(e) <= 2e-7 and np.percentile(e, 99) <= 2e-6
 
 
