@@ -44,12 +44,21 @@ using namespace nb;
 
 namespace {
 
-int default_tpt(int64_t n) { return n >= (1 << 17) ? 2 : 1; }
+// Defaults before any autotuned entry: one target per lane; in the SPLIT
+// modes the bf16 matrix-core far path from 2^16 particles on (where most
+// tiles are far), with source segments for about 16k blocks, else 8k.
+int default_tpt(int64_t) { return 1; }
 
-int default_segments(int64_t n, int tpt) {
+int default_far_path(int mode, int64_t n) {
+    const bool split = mode == NB_MODE_SPLIT || mode == NB_MODE_SPLIT_CENTRAL;
+    return split && n >= (1 << 16) ? FAR_MFMA_BF16 : FAR_VALU;
+}
+
+int default_segments(int64_t n, int tpt, int far_path) {
     const int64_t nblk = (n + (int64_t)BS * tpt - 1) / ((int64_t)BS * tpt);
     const int64_t ntiles = (n + BS - 1) / BS;
-    int64_t nseg = (2048 + nblk - 1) / std::max<int64_t>(nblk, 1);
+    const int64_t want = far_path == FAR_MFMA_BF16 ? 16384 : 8192;
+    int64_t nseg = (want + nblk - 1) / std::max<int64_t>(nblk, 1);
     nseg = std::min<int64_t>(std::max<int64_t>(nseg, 1), std::max<int64_t>(ntiles, 1));
     nseg = std::min<int64_t>(nseg, 64);
     return (int)nseg;
@@ -381,7 +390,8 @@ int nb_create(nb_ctx **out, int64_t n, int32_t precision_mode) {
             break;
         }
         c->tun.targets_per_thread = default_tpt(n);
-        c->tun.j_segments = default_segments(n, c->tun.targets_per_thread);
+        c->tun.far_path = default_far_path(precision_mode, n);
+        c->tun.j_segments = default_segments(n, c->tun.targets_per_thread, c->tun.far_path);
         c->tun.sort_interval = precision_mode == NB_MODE_FP64 ? 0 : 16;
         tuner_lookup(c->mode, n, &c->tun);
         c->sort.interval = c->tun.sort_interval;
@@ -861,7 +871,7 @@ int nb_set_tuning(nb_ctx *c, const nb_tuning *t) {
     if (!t || (t->targets_per_thread != 1 && t->targets_per_thread != 2 &&
                t->targets_per_thread != 4) ||
         t->j_segments < 1 || t->j_segments > 256 || t->sort_interval < 0 || t->far_path < 0 ||
-        t->far_path > 3) {
+        t->far_path > 4) {
         set_error("invalid tuning");
         return NB_ERR_INVALID;
     }

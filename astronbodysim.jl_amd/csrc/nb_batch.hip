@@ -640,6 +640,7 @@ static int bforce(nb_batch *b, bool with_pot, bool allow_timing) {
         const int fp = with_pot ? FAR_VALU : b->tun.far_path;
         if (tpt == 1 && fp == FAR_MFMA_R2) k_batch_force<1, false, FAR_MFMA_R2><<<nblk, BS, 0, s>>>(a);
         else if (tpt == 1 && fp == FAR_MFMA_SUM) k_batch_force<1, false, FAR_MFMA_SUM><<<nblk, BS, 0, s>>>(a);
+        else if (tpt == 1 && fp == FAR_MFMA_BF16) k_batch_force<1, false, FAR_MFMA_BF16><<<nblk, BS, 0, s>>>(a);
         else if (fp == FAR_WEIGHTED && tpt == 1) k_batch_force<1, false, FAR_WEIGHTED><<<nblk, BS, 0, s>>>(a);
         else if (fp == FAR_WEIGHTED && tpt == 2) k_batch_force<2, false, FAR_WEIGHTED><<<nblk, BS, 0, s>>>(a);
         else if (fp == FAR_WEIGHTED && tpt == 4) k_batch_force<4, false, FAR_WEIGHTED><<<nblk, BS, 0, s>>>(a);
@@ -1158,7 +1159,7 @@ int nb_batch_set_tuning(nb_batch *b, const nb_batch_tuning *t) {
     if (!t || (t->targets_per_thread != 1 && t->targets_per_thread != 2 &&
                t->targets_per_thread != 4) ||
         t->blocks < 0 || t->blocks > (1 << 20) || t->steps_per_graph < 2 ||
-        t->steps_per_graph > 4096 || t->sort_interval < 0 || t->far_path < 0 || t->far_path > 3) {
+        t->steps_per_graph > 4096 || t->sort_interval < 0 || t->far_path < 0 || t->far_path > 4) {
         set_error("invalid batch tuning");
         return NB_ERR_INVALID;
     }
@@ -1190,8 +1191,8 @@ int nb_batch_autotune(nb_batch *b) {
     NB_HIP(hipEventCreate(&e1));
     int rc = NB_OK;
     // (targets per thread, far path) as in the one-system autotuner
-    const int cand[8][2] = {{1, 0}, {2, 0}, {4, 0}, {1, 3}, {2, 3}, {4, 3}, {1, 1}, {1, 2}};
-    for (int ti = 0; ti < 8 && rc == NB_OK; ++ti) {
+    const int cand[9][2] = {{1, 0}, {2, 0}, {4, 0}, {1, 3}, {2, 3}, {4, 3}, {1, 4}, {1, 1}, {1, 2}};
+    for (int ti = 0; ti < 9 && rc == NB_OK; ++ti) {
         for (int mi = 0; mi < 4 && rc == NB_OK; ++mi) {
             b->tun.targets_per_thread = cand[ti][0];
             b->tun.far_path = cand[ti][1];

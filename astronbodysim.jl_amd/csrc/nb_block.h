@@ -188,11 +188,18 @@ constexpr double MF2_REACH = 3.0;
 // w |x'|^2 = 2^120, so their f is exactly zero.
 constexpr double FARW_REACH = 2.0;
 
-// Far-tile paths (the far_path tuning value).
-constexpr int FAR_VALU = 0;       // 12 VALU + rsq per pair
-constexpr int FAR_MFMA_SUM = 1;   // sum on the matrix cores
-constexpr int FAR_MFMA_R2 = 2;    // r^2 and sum on the matrix cores (farthest tiles)
-constexpr int FAR_WEIGHTED = 3;   // weighted far tiles: 10 VALU + rsq per pair
+// With FAR_MFMA_BF16 the squared distances of the tiles past FARW_REACH * Re
+// come from bf16 matrix cores (v_mfma_f32_16x16x32_bf16), which run alongside
+// the VALU, and the sum stays on the VALU in the S form above with
+// f = m_j rsq(r^2)^3: rsq, three multiplies, three FMAs and an add per pair.
+// Every FP32 coordinate is split into three bf16 (x = x0 + x1 + x2 exactly);
+// the K = 32 product holds the cross terms x_i^a x_j^b with a + b <= 3 of
+// each axis (the dropped x_i^2 x_j^2 terms are below 2^-32 |x_i||x_j|),
+// |x'_j|^2 and |x'_i|^2, accumulated in FP32, so r^2 carries the few-eps
+// (|x'_i| + |x'_j|)^2 error of the weighted expansion.
+
+typedef __bf16 nb_bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned nb_u4v __attribute__((ext_vector_type(4)));
 constexpr float FARW_MASSLESS = 1.329228e36f;   // 2^120
 
 // Shared memory of one all-pairs block.
@@ -207,6 +214,8 @@ struct BlockShared {
     static constexpr bool WFAR = HILO && MF == 3;
     float4 sW[WFAR ? 2 : 1][WFAR ? BS : 1];          // (w x', w y', w z', w |x'|^2)
     float sw[WFAR ? 2 : 1][WFAR ? BS : 1];           // w = m^(-2/3)
+    static constexpr bool ROWS = HILO && MF == 4;
+    nb_bf8 sA[1][ROWS ? BS : 1][4];                   // bf16 A rows (K = 32), single-buffered
     double red[11][NW];
 };
 
@@ -322,6 +331,85 @@ __device__ __forceinline__ double uniform_f64(double v) {
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
+// ---- bf16-split squared distances on the matrix cores (FAR_MFMA_BF16) ----
+// Three bf16 whose sum is x: each remainder is exact in FP32.
+__device__ __forceinline__ void split_bf16(float x, __bf16 (&p)[3]) {
+    p[0] = (__bf16)x;
+    const float r = x - (float)p[0];
+    p[1] = (__bf16)r;
+    p[2] = (__bf16)(r - (float)p[1]);
+}
+
+// K layout of the squared-distance product (A = source row, B = target column):
+//   k = 8d + c, d = x, y, z, c = 0..7:  A = s_d^MF3_IA[c],  B = -2 t_d^MF3_IB[c]
+//   k = 24..26:                         A = split of |x'_j|^2, B = 1
+//   k = 27..29:                         A = 1, B = split of |x'_i|^2
+//   k = 30, 31:                         0
+__device__ constexpr int MF3_IA[8] = {0, 0, 1, 0, 2, 1, 1, 2};
+__device__ constexpr int MF3_IB[8] = {0, 1, 0, 2, 0, 1, 2, 1};
+
+__device__ __forceinline__ void mf3_rows(float x, float y, float z, float q, bool source,
+                                         nb_bf8 (&out)[4]) {
+    __bf16 sp[3][3], qs[3];
+    split_bf16(x, sp[0]);
+    split_bf16(y, sp[1]);
+    split_bf16(z, sp[2]);
+    split_bf16(q, qs);
+    const __bf16 one = (__bf16)1.0f, zero = (__bf16)0.0f;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int k = 8 * d + c;
+            out[k >> 3][k & 7] = source ? sp[d][MF3_IA[c]] : (__bf16)(-2.0f * (float)sp[d][MF3_IB[c]]);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        out[3][a] = source ? qs[a] : one;
+        out[3][3 + a] = source ? one : qs[a];
+    }
+    out[3][6] = zero;
+    out[3][7] = zero;
+}
+
+__device__ __forceinline__ nb_bf8 shfl_bf8(nb_bf8 v, int src) {
+    nb_u4v u = __builtin_bit_cast(nb_u4v, v);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) u[e] = (unsigned)__shfl((int)u[e], src, 64);
+    return __builtin_bit_cast(nb_bf8, u);
+}
+
+// Farthest tiles with FAR_MFMA_BF16.  Chunk of 16 sources jc..jc+15: MFMA g gives
+// lane l the r^2 of targets 16g + (l & 15) (the B columns) against sources
+// jc + 4(l >> 4) + v in element v (the A rows); the lane then accumulates
+// S[g] for its four targets over its four sources.
+__device__ __forceinline__ void tile_mf3(const float4 *__restrict__ s, const nb_bf8 (*__restrict__ sa)[4],
+                                         int lane, const nb_bf8 (&Bg)[4], float (&S)[4][4]) {
+    const int c16 = lane & 15, grp = lane >> 4;
+    const nb_f4v zero = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 1
+    for (int jc = 0; jc < BS; jc += 16) {
+        const nb_bf8 a = sa[jc + c16][grp];
+        nb_f4v D[4];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) D[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, Bg[g], zero, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const float4 p = s[jc + 4 * grp + v];
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const float ri = __builtin_amdgcn_rsqf(D[g][v]);
+                const float f = (p.w * ri) * (ri * ri);
+                S[g][0] = __builtin_fmaf(f, p.x, S[g][0]);
+                S[g][1] = __builtin_fmaf(f, p.y, S[g][1]);
+                S[g][2] = __builtin_fmaf(f, p.z, S[g][2]);
+                S[g][3] += f;
+            }
+        }
+    }
+}
+
 // Per-wave tile classes.  Each wave uses the box of its own targets, their
 // largest h, their largest distance R from the block origin per axis and
 // their largest Euclidean distance Re from it.  A margin keeps every far pair
@@ -417,6 +505,8 @@ __device__ __forceinline__ void force_block(
     static_assert(!(MF == FAR_MFMA_SUM || MF == FAR_MFMA_R2) || (HILO && TPT == 1 && !POT),
                   "MFMA far tiles: SPLIT, 1 target/lane, acc only");
     static_assert(MF != FAR_WEIGHTED || !POT, "weighted far tiles: acc only");
+    static_assert(MF != FAR_MFMA_BF16 || (HILO && TPT == 1 && !POT),
+                  "bf16 matrix-core far tiles: SPLIT, 1 target/lane, acc only");
     const int tid = threadIdx.x;
     __syncthreads();  // the previous call's readers of sh are done
     double S[4] = {0.0, 0.0, 0.0, 0.0};
@@ -459,6 +549,29 @@ __device__ __forceinline__ void force_block(
     const WaveGeom wg = wave_geometry<TPT>(px, py, pz, ph, valid, ox, oy, oz, HILO);
     const double soft_d2 = wg.soft_d2, close_d2 = wg.close_d2;
     const double close_d = sqrt(close_d2);
+    // FAR_MFMA_BF16: lane l holds, for target group g, the K slice (l >> 4) of
+    // the B column of target 16g + (l & 15); S3 accumulates those targets.
+    constexpr bool ROWS = BlockShared<T, MF>::ROWS;
+    nb_bf8 Bg[4];
+    float S3[4][4];
+    if constexpr (ROWS) {
+        const int lane = tid & 63;
+        const double dx = px[0] - ox, dy = py[0] - oy, dz = pz[0] - oz;
+        nb_bf8 mine[4];
+        mf3_rows(xi[0], yi[0], zi[0], (float)(dx * dx + dy * dy + dz * dz), false, mine);
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+#pragma unroll
+            for (int sl = 0; sl < 4; ++sl) {
+                const nb_bf8 t = shfl_bf8(mine[sl], 16 * g + (lane & 15));
+                if (sl == (lane >> 4)) Bg[g] = t;
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) S3[g][c] = 0.0f;
+    }
     // weighted far tiles: per-target operands and S-form sums
     constexpr bool WFAR = BlockShared<T, MF>::WFAR && !POT;
     float wbx[TPT], wby[TPT], wbz[TPT], wci[TPT], Sx[TPT], Sy[TPT], Sz[TPT], Sm[TPT];
@@ -498,6 +611,8 @@ __device__ __forceinline__ void force_block(
     int it = 0;
     for (int64_t base = j0; base < j1; base += BS, ++it) {
         const int buf = it & 1;
+        // the single-buffered bf16 rows: the last tile's readers finish first
+        if constexpr (ROWS) if (it > 0) __syncthreads();
         if (rv) {
             const double dx = (double)rx - ox, dy = (double)ry - oy, dz = (double)rz - oz;
             const float hx = (float)dx, hy = (float)dy, hz = (float)dz;
@@ -514,6 +629,9 @@ __device__ __forceinline__ void force_block(
                 sh.sB[buf][3][tid] = (float)md;
             }
             if constexpr (MF == 2) sh.sQ[MF == 2 ? buf : 0][MF == 2 ? tid : 0] = (float)(dx * dx + dy * dy + dz * dz);
+            if constexpr (ROWS)
+                mf3_rows(hx, hy, hz, (float)(dx * dx + dy * dy + dz * dz), true,
+                         sh.sA[0][ROWS ? tid : 0]);
             if constexpr (WFAR) {
                 const double q2 = (double)hx * hx + (double)hy * hy + (double)hz * hz;
                 float4 W;
@@ -540,6 +658,9 @@ __device__ __forceinline__ void force_block(
             }
             // |PAD|^2: r^2 ~ 3e36 keeps r^-3 at zero (and finite) for padding
             if constexpr (MF == 2) sh.sQ[MF == 2 ? buf : 0][MF == 2 ? tid : 0] = 3.0f * PAD_COORD * PAD_COORD;
+            if constexpr (ROWS)
+                mf3_rows(PAD_COORD, PAD_COORD, PAD_COORD, 3.0f * PAD_COORD * PAD_COORD, true,
+                         sh.sA[0][ROWS ? tid : 0]);
             if constexpr (WFAR) {
                 sh.sW[WFAR ? buf : 0][WFAR ? tid : 0] = make_float4(0.0f, 0.0f, 0.0f, FARW_MASSLESS);
                 sh.sw[WFAR ? buf : 0][WFAR ? tid : 0] = 0.0f;
@@ -562,6 +683,28 @@ __device__ __forceinline__ void force_block(
         } else if (HILO && d2 <= close_d2) {
             tile_f32_hilo<TPT, false, POT>(sh.sj[buf], sh.sjl[HILO ? buf : 0], xi, yi, zi, xil,
                                            yil, zil, h2c, h2, hinv, hinv3, ax, ay, az, aq);
+        } else if (ROWS && d2 > wg.farw_d2) {
+            tile_mf3(sh.sj[buf], sh.sA[0], tid & 63, Bg, S3);
+            // Target 16g + c sits in lanes c, 16 + c, 32 + c, 48 + c (slot g);
+            // after the butterfly each of them holds the tile's sums and lane
+            // l keeps slot l >> 4, its own target.
+            const int lane = tid & 63;
+            float col[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    float t = S3[g][c];
+                    t += __shfl_xor(t, 16, 64);
+                    t += __shfl_xor(t, 32, 64);
+                    if (g == (lane >> 4)) col[c] = t;
+                    S3[g][c] = 0.0f;
+                }
+            }
+            const double sm = (double)col[3];
+            Ax[0] += (double)col[0] - (px[0] - ox) * sm;
+            Ay[0] += (double)col[1] - (py[0] - oy) * sm;
+            Az[0] += (double)col[2] - (pz[0] - oz) * sm;
         } else if (WFAR && d2 > wg.farw_d2) {
             tile_w<TPT>(sh.sj[buf], sh.sW[WFAR ? buf : 0], sh.sw[WFAR ? buf : 0], wbx, wby, wbz, wci,
                         Sx, Sy, Sz, Sm);

@@ -23,7 +23,7 @@ namespace nb {
 // coordinates on near tiles) or float (FP32 state, absolute coordinates).
 // MF selects the matrix-core far tiles (SPLIT, TPT 1, forces only).
 template <typename T, int TPT, bool POT, int MF = 0>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(3))) void k_force_f32(ForceArgs<T> a) {
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MF == 4 ? 4 : 3))) void k_force_f32(ForceArgs<T> a) {
     __shared__ BlockShared<T, MF> sh;
     const int tid = threadIdx.x;
     const int logical = xcd_remap(blockIdx.x, gridDim.x);
@@ -219,6 +219,12 @@ int launch_force_split(const ForceArgs<double> &a, int tpt, bool pot, hipStream_
         if (far_path == FAR_MFMA_R2) k_force_f32<double, 1, false, FAR_MFMA_R2><<<(unsigned)nb, BS, 0, s>>>(a);
         else k_force_f32<double, 1, false, FAR_MFMA_SUM><<<(unsigned)nb, BS, 0, s>>>(a);
         return check_launch("k_force_f32<mfma>");
+    }
+    if (far_path == FAR_MFMA_BF16 && tpt == 1 && !pot) {
+        const int64_t nb = (int64_t)a.nblk_i * a.nseg;
+        if (nb <= 0) return NB_OK;
+        k_force_f32<double, 1, false, FAR_MFMA_BF16><<<(unsigned)nb, BS, 0, s>>>(a);
+        return check_launch("k_force_f32<bf16 mfma>");
     }
     if (far_path == FAR_WEIGHTED) return launch_f32_impl<double, FAR_WEIGHTED>(a, tpt, pot, s);
     return launch_f32_impl<double, FAR_VALU>(a, tpt, pot, s);

@@ -53,6 +53,13 @@ struct ForceArgs {
     int accumulate;              // 1: add G*sum to the outputs instead of storing (nseg == 1)
 };
 
+// Far-tile paths (the far_path tuning value; nb_block.h describes each).
+constexpr int FAR_VALU = 0;       // 12 VALU + rsq per pair
+constexpr int FAR_MFMA_SUM = 1;   // sum on the matrix cores
+constexpr int FAR_MFMA_R2 = 2;    // r^2 and sum on the matrix cores (farthest tiles)
+constexpr int FAR_WEIGHTED = 3;   // weighted far tiles: 10 VALU + rsq per pair
+constexpr int FAR_MFMA_BF16 = 4;  // bf16-split r^2 on the matrix cores, sum on the VALU
+
 // Launchers (nb_force.hip)
 int launch_tile_boxes(const void *x, const void *y, const void *z, int is_double, int64_t n,
                       double *box, hipStream_t s);

@@ -94,9 +94,9 @@ int autotune_ctx(nb_ctx *c) {
     // A single evaluation is timed per candidate when it is long.
     // candidates (targets per thread, far path); the matrix-core and weighted
     // far paths exist in the SPLIT modes only
-    const int cand[8][2] = {{1, 0}, {2, 0}, {4, 0}, {1, 3}, {2, 3}, {4, 3}, {1, 1}, {1, 2}};
+    const int cand[9][2] = {{1, 0}, {2, 0}, {4, 0}, {1, 3}, {2, 3}, {4, 3}, {1, 4}, {1, 1}, {1, 2}};
     const bool split = c->mode == NB_MODE_SPLIT || c->mode == NB_MODE_SPLIT_CENTRAL;
-    for (int ti = 0; ti < (split ? 8 : 3); ++ti) {
+    for (int ti = 0; ti < (split ? 9 : 3); ++ti) {
         const int tpt = cand[ti][0];
         const int mf = cand[ti][1];
         if (c->mode == NB_MODE_FP64 && tpt != 1) continue;
@@ -106,7 +106,7 @@ int autotune_ctx(nb_ctx *c) {
             if (nseg > ntiles) break;
             if (nblk * nseg > (1 << 20)) break;
             if ((double)n * nseg * 32.0 > 4.0e9) break;  // partial-sum buffer bound
-            if (nseg > 1 && nblk * (nseg / 2) >= 8192) break;  // the grid was already full
+            if (nseg > 1 && nblk * (nseg / 2) >= 32768) break;  // the grid was already full
             if (nblk * nseg < 64 && nseg < ntiles && si + 1 < 8) continue;  // far too few blocks
             c->tun.targets_per_thread = tpt;
             c->tun.j_segments = nseg;
@@ -217,7 +217,7 @@ extern "C" int nb_tuner_load(const char *path) {
         json_int(obj, "sort_interval", &si);
         json_int(obj, "far_path", &mf);
         if ((tpt != 1 && tpt != 2 && tpt != 4) || nseg < 1 || nseg > 256 || mode < 0 || mode > 3 ||
-            si < 0 || mf < 0 || mf > 3 || ((mf == 1 || mf == 2) && tpt != 1)) {
+            si < 0 || mf < 0 || mf > 4 || ((mf == 1 || mf == 2 || mf == 4) && tpt != 1)) {
             nb::set_error("%s: entry out of range", path);
             return NB_ERR_INVALID;
         }

@@ -1,0 +1,53 @@
+"""Regenerate the launch-configuration table that the package loads at import
+(tuning/mi355x.json): autotune one seeded system per size and precision mode
+and save what the autotuner measured.
+
+    python -m astronbodysim.jl_amd.tuning.make_table [out.json]
+
+(or run this file directly).  Sizes 2^12 .. 2^20 in the split mode on
+two-Plummer collisions (the benchmark's geometry), and 2^12 .. 2^16 in the
+fp32 and fp64 modes on Plummer spheres.  Needs the GPU; a few minutes.
+"""
+from __future__ import annotations
+
+import sys
+import time
+from pathlib import Path
+
+
+def _package():
+    here = Path(__file__).resolve().parents[2]
+    if str(here) not in sys.path:
+        sys.path.insert(0, str(here))
+    import __graft_entry__ as ge
+    return ge.load_package()
+
+
+def main(out=None):
+    pkg = _package()
+    nat = pkg._native
+    out = out or str(nat.DEFAULT_TUNING)
+    nat.tuner_clear()
+    plan = [("split", k) for k in range(12, 21)] + [("fp32", k) for k in range(12, 17)] + \
+           [("fp64", k) for k in range(12, 16)]
+    for mode, k in plan:
+        n = 1 << k
+        if mode == "split":
+            pos, vel, mass = pkg.ics.collision(n // 2)
+            h = 0.002
+        else:
+            pos, vel, mass = pkg.ics.plummer(n, seed=2)
+            h = 0.005
+        ctx = nat.Context(n, nat.MODES[mode])
+        ctx.set_params(1.0, [h] * 6)
+        ctx.upload(pos, vel, mass)
+        t0 = time.time()
+        ctx.autotune()
+        print(f"{mode} N=2^{k}: {ctx.get_tuning()} ({time.time() - t0:.1f} s)", flush=True)
+        ctx.close()
+    nat.tuner_save(out)
+    print("saved", out, flush=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else None)

@@ -47,14 +47,20 @@ def parse():
     return p.parse_args()
 
 
-def traffic_from_profile():
+def traffic_from_profile(tuning):
+    """HBM bytes per force launch from the committed counter summary, when it
+    was taken with the launch configuration this run uses."""
     f = ROOT / "profiles" / "force_c3_traffic.json"
-    if f.exists():
-        try:
-            return json.loads(f.read_text()).get("bytes_per_launch")
-        except Exception:
-            return None
-    return None
+    if not f.exists():
+        return None
+    try:
+        prof = json.loads(f.read_text())
+    except ValueError:
+        return None
+    keys = ("targets_per_thread", "j_segments", "far_path")
+    if any(prof.get("tuning", {}).get(k) != tuning.get(k) for k in keys):
+        return None
+    return prof.get("bytes_per_launch")
 
 
 def progress(msg):
@@ -326,7 +332,7 @@ def main():
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "frac_unpacked": achieved / PEAK_FP32_UNPACKED_TFLOPS,
-                     "traffic": traffic_from_profile(),
+                     "traffic": traffic_from_profile(tuning),
                      "kernel": "k_force_f32<double,%d,false,%s>" % (
                          tuning["targets_per_thread"], tuning["far_path"]),
                      "flops_per_pair": FLOPS_PER_PAIR,

@@ -79,7 +79,9 @@ typedef struct nb_tuning {
     int32_t far_path;           /* far-tile arithmetic (SPLIT modes; forces only, potentials take
                                    0): 0 FP32 VALU, 12 ops + rsq per pair; 3 weighted VALU,
                                    10 ops + rsq on the farthest tiles; 1 sum on the matrix
-                                   cores, 2 also r^2 there (1 and 2: targets_per_thread 1) */
+                                   cores, 2 also r^2 there; 4 r^2 from bf16-split products
+                                   on the matrix cores beside a VALU sum (1, 2 and 4:
+                                   targets_per_thread 1) */
 } nb_tuning;
 
 /* ---- library / device ------------------------------------------------- */

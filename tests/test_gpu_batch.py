@@ -41,6 +41,7 @@ def make(pkg, ens, mode="split", dt=1e-3, **tun):
                                                ("split", GATE, 4, 0), ("split", GATE, 1, 1),
                                                ("split", GATE, 1, 2), ("split", GATE, 1, 3),
                                                ("split", GATE, 2, 3), ("split", GATE, 4, 3),
+                                               ("split", GATE, 1, 4),
                                                ("fp64", 1e-12, 1, 0)])
 def test_ragged_force(pkg, orc, clusters, mode, tol, tpt, far):
     ref, _ = oracle_forces(orc, clusters)

@@ -64,7 +64,7 @@ def test_potential_plummer4096(pkg, orc, plummer4096, mode, tol):
 @pytest.mark.parametrize("tpt,nseg,sort,far", [
     (1, 1, 0, 0), (1, 7, 0, 0), (2, 1, 0, 0), (2, 3, 0, 0), (4, 1, 0, 0), (4, 16, 0, 0),
     (1, 1, 1, 1), (1, 5, 1, 1), (1, 1, 0, 1), (2, 2, 1, 0), (1, 1, 1, 2), (1, 3, 0, 2), (1, 1, 1, 3), (2, 3, 1, 3),
-    (4, 2, 1, 3)])
+    (4, 2, 1, 3), (1, 1, 1, 4), (1, 3, 0, 4)])
 def test_launch_configurations_agree(pkg, orc, plummer4096, tpt, nseg, sort, far):
     pos, vel, mass, ref, _ = plummer4096
     with make_ctx(pkg, pos, vel, mass, [0.01] * 6, "split", tuning=(tpt, nseg, sort, far)) as ctx:
@@ -189,7 +189,7 @@ def test_graph_replay_matches_eager(pkg):
         assert np.array_equal(out[0]["vel"], out[k]["vel"])
 
 
-@pytest.mark.parametrize("far", [0, 1, 2, 3])
+@pytest.mark.parametrize("far", [0, 1, 2, 3, 4])
 def test_collision_far_tiles(pkg, orc, far):
     """Two separated Plummer spheres sorted into Morton order: most tiles are
     far (pure Newtonian on hi coordinates, or on the matrix cores)."""
