@@ -190,13 +190,14 @@ constexpr double FARW_REACH = 2.0;
 
 // With FAR_MFMA_BF16 the squared distances of the tiles past FARW_REACH * Re
 // come from bf16 matrix cores (v_mfma_f32_16x16x32_bf16), which run alongside
-// the VALU, and the sum stays on the VALU in the S form above with
-// f = m_j rsq(r^2)^3: rsq, three multiplies, three FMAs and an add per pair.
-// Every FP32 coordinate is split into three bf16 (x = x0 + x1 + x2 exactly);
-// the K = 32 product holds the cross terms x_i^a x_j^b with a + b <= 3 of
-// each axis (the dropped x_i^2 x_j^2 terms are below 2^-32 |x_i||x_j|),
-// |x'_j|^2 and |x'_i|^2, accumulated in FP32, so r^2 carries the few-eps
-// (|x'_i| + |x'_j|)^2 error of the weighted expansion.
+// the VALU, and the sum stays on the VALU in the S form above.  As in the
+// weighted path, the source rows carry w_j = m_j^(-2/3), so the product is
+// w_j r_ij^2 and f = rsq(w r^2)^3: rsq, two multiplies, three FMAs and an
+// add per pair.  Every FP32 operand is split into three bf16 (x = x0 + x1 +
+// x2 exactly); the K = 32 product holds the cross terms of split orders
+// a + b <= 2 of each axis (the dropped ones are below 2^-23 |x_i||x_j|),
+// w |x'_j|^2 and w |x'_i|^2, accumulated in FP32, so w r^2 carries the
+// few-eps (|x'_i| + |x'_j|)^2 error of the weighted expansion.
 
 typedef __bf16 nb_bf8 __attribute__((ext_vector_type(8)));
 typedef unsigned nb_u4v __attribute__((ext_vector_type(4)));
@@ -340,37 +341,70 @@ __device__ __forceinline__ void split_bf16(float x, __bf16 (&p)[3]) {
     p[2] = (__bf16)(r - (float)p[1]);
 }
 
-// K layout of the squared-distance product (A = source row, B = target column):
-//   k = 8d + c, d = x, y, z, c = 0..7:  A = s_d^MF3_IA[c],  B = -2 t_d^MF3_IB[c]
-//   k = 24..26:                         A = split of |x'_j|^2, B = 1
-//   k = 27..29:                         A = 1, B = split of |x'_i|^2
-//   k = 30, 31:                         0
-__device__ constexpr int MF3_IA[8] = {0, 0, 1, 0, 2, 1, 1, 2};
-__device__ constexpr int MF3_IB[8] = {0, 1, 0, 2, 0, 1, 2, 1};
+// K layout of the weighted squared-distance product (A = source row, B =
+// target column), w_j = m_j^(-2/3):
+//   k = 6d + c (d = x, y, z; c = 0..5): A = (w x'_j)_d^RA[c], B = -2 (x'_i)_d^RB[c]
+//   k = 18..20:                         A = split of w |x'_j|^2, B = 1
+//   k = 21..26:                         A = w^RA[c], B = (|x'_i|^2)^RB[c]
+//   k = 27..31:                         0
+// with (RA, RB) = (0,0) (0,1) (1,0) (0,2) (2,0) (1,1): every cross term of
+// split orders a + b <= 2, the dropped ones being below 2^-23 of the product.
+// The product is w_j r_ij^2, so f_ij = m_j r_ij^-3 = rsq(w_j r_ij^2)^3.
+__device__ constexpr int RA[6] = {0, 0, 1, 0, 2, 1};
+__device__ constexpr int RB[6] = {0, 1, 0, 2, 0, 1};
 
-__device__ __forceinline__ void mf3_rows(float x, float y, float z, float q, bool source,
-                                         nb_bf8 (&out)[4]) {
-    __bf16 sp[3][3], qs[3];
-    split_bf16(x, sp[0]);
-    split_bf16(y, sp[1]);
-    split_bf16(z, sp[2]);
-    split_bf16(q, qs);
+__device__ __forceinline__ void set_k(nb_bf8 (&out)[4], int k, __bf16 v) { out[k >> 3][k & 7] = v; }
+
+// A row of a source with weight w > 0: (wx, wy, wz) = w x'_j, wq = w |x'_j|^2.
+__device__ __forceinline__ void bf16_source_row(float wx, float wy, float wz, float wq, float w,
+                                                nb_bf8 (&out)[4]) {
+    __bf16 sp[3][3], qs[3], ws[3];
+    split_bf16(wx, sp[0]);
+    split_bf16(wy, sp[1]);
+    split_bf16(wz, sp[2]);
+    split_bf16(wq, qs);
+    split_bf16(w, ws);
     const __bf16 one = (__bf16)1.0f, zero = (__bf16)0.0f;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
+    for (int d = 0; d < 3; ++d)
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const int k = 8 * d + c;
-            out[k >> 3][k & 7] = source ? sp[d][MF3_IA[c]] : (__bf16)(-2.0f * (float)sp[d][MF3_IB[c]]);
-        }
-    }
+        for (int c = 0; c < 6; ++c) set_k(out, 6 * d + c, sp[d][RA[c]]);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        out[3][a] = source ? qs[a] : one;
-        out[3][3 + a] = source ? one : qs[a];
-    }
-    out[3][6] = zero;
-    out[3][7] = zero;
+    for (int a = 0; a < 3; ++a) set_k(out, 18 + a, qs[a]);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) set_k(out, 21 + c, ws[RA[c]]);
+#pragma unroll
+    for (int k = 27; k < 32; ++k) set_k(out, k, zero);
+    (void)one;
+}
+
+// A row of a massless (or padding) source: w r^2 = 2^120, so f = 0 exactly.
+__device__ __forceinline__ void bf16_massless_row(nb_bf8 (&out)[4]) {
+    const __bf16 zero = (__bf16)0.0f;
+#pragma unroll
+    for (int k = 0; k < 32; ++k) set_k(out, k, zero);
+    set_k(out, 18, (__bf16)FARW_MASSLESS);
+}
+
+// B column of a target at x' with c = |x'|^2.
+__device__ __forceinline__ void bf16_target_col(float x, float y, float z, float c,
+                                                nb_bf8 (&out)[4]) {
+    __bf16 tp[3][3], cs[3];
+    split_bf16(x, tp[0]);
+    split_bf16(y, tp[1]);
+    split_bf16(z, tp[2]);
+    split_bf16(c, cs);
+    const __bf16 one = (__bf16)1.0f, zero = (__bf16)0.0f;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int c6 = 0; c6 < 6; ++c6) set_k(out, 6 * d + c6, (__bf16)(-2.0f * (float)tp[d][RB[c6]]));
+#pragma unroll
+    for (int a = 0; a < 3; ++a) set_k(out, 18 + a, one);
+#pragma unroll
+    for (int c6 = 0; c6 < 6; ++c6) set_k(out, 21 + c6, cs[RB[c6]]);
+#pragma unroll
+    for (int k = 27; k < 32; ++k) set_k(out, k, zero);
 }
 
 __device__ __forceinline__ nb_bf8 shfl_bf8(nb_bf8 v, int src) {
@@ -381,7 +415,7 @@ __device__ __forceinline__ nb_bf8 shfl_bf8(nb_bf8 v, int src) {
 }
 
 // Farthest tiles with FAR_MFMA_BF16.  Chunk of 16 sources jc..jc+15: MFMA g gives
-// lane l the r^2 of targets 16g + (l & 15) (the B columns) against sources
+// lane l the w r^2 of targets 16g + (l & 15) (the B columns) against sources
 // jc + 4(l >> 4) + v in element v (the A rows); the lane then accumulates
 // S[g] for its four targets over its four sources.
 __device__ __forceinline__ void tile_mf3(const float4 *__restrict__ s, const nb_bf8 (*__restrict__ sa)[4],
@@ -399,8 +433,8 @@ __device__ __forceinline__ void tile_mf3(const float4 *__restrict__ s, const nb_
             const float4 p = s[jc + 4 * grp + v];
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
-                const float ri = __builtin_amdgcn_rsqf(D[g][v]);
-                const float f = (p.w * ri) * (ri * ri);
+                const float q = __builtin_amdgcn_rsqf(D[g][v]);
+                const float f = (q * q) * q;
                 S[g][0] = __builtin_fmaf(f, p.x, S[g][0]);
                 S[g][1] = __builtin_fmaf(f, p.y, S[g][1]);
                 S[g][2] = __builtin_fmaf(f, p.z, S[g][2]);
@@ -558,7 +592,7 @@ __device__ __forceinline__ void force_block(
         const int lane = tid & 63;
         const double dx = px[0] - ox, dy = py[0] - oy, dz = pz[0] - oz;
         nb_bf8 mine[4];
-        mf3_rows(xi[0], yi[0], zi[0], (float)(dx * dx + dy * dy + dz * dz), false, mine);
+        bf16_target_col(xi[0], yi[0], zi[0], (float)(dx * dx + dy * dy + dz * dz), mine);
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
 #pragma unroll
@@ -629,9 +663,16 @@ __device__ __forceinline__ void force_block(
                 sh.sB[buf][3][tid] = (float)md;
             }
             if constexpr (MF == 2) sh.sQ[MF == 2 ? buf : 0][MF == 2 ? tid : 0] = (float)(dx * dx + dy * dy + dz * dz);
-            if constexpr (ROWS)
-                mf3_rows(hx, hy, hz, (float)(dx * dx + dy * dy + dz * dz), true,
-                         sh.sA[0][ROWS ? tid : 0]);
+            if constexpr (ROWS) {
+                if (rm > 0) {
+                    const double wd = 1.0 / cbrt((double)rm * (double)rm);
+                    const double q2 = (double)hx * hx + (double)hy * hy + (double)hz * hz;
+                    bf16_source_row((float)(wd * hx), (float)(wd * hy), (float)(wd * hz),
+                                    (float)(wd * q2), (float)wd, sh.sA[0][ROWS ? tid : 0]);
+                } else {
+                    bf16_massless_row(sh.sA[0][ROWS ? tid : 0]);
+                }
+            }
             if constexpr (WFAR) {
                 const double q2 = (double)hx * hx + (double)hy * hy + (double)hz * hz;
                 float4 W;
@@ -658,9 +699,7 @@ __device__ __forceinline__ void force_block(
             }
             // |PAD|^2: r^2 ~ 3e36 keeps r^-3 at zero (and finite) for padding
             if constexpr (MF == 2) sh.sQ[MF == 2 ? buf : 0][MF == 2 ? tid : 0] = 3.0f * PAD_COORD * PAD_COORD;
-            if constexpr (ROWS)
-                mf3_rows(PAD_COORD, PAD_COORD, PAD_COORD, 3.0f * PAD_COORD * PAD_COORD, true,
-                         sh.sA[0][ROWS ? tid : 0]);
+            if constexpr (ROWS) bf16_massless_row(sh.sA[0][ROWS ? tid : 0]);
             if constexpr (WFAR) {
                 sh.sW[WFAR ? buf : 0][WFAR ? tid : 0] = make_float4(0.0f, 0.0f, 0.0f, FARW_MASSLESS);
                 sh.sw[WFAR ? buf : 0][WFAR ? tid : 0] = 0.0f;

@@ -25,21 +25,37 @@ def split3(x):
     return [h, m, bf16(np.float32(r - m))]
 
 
-# K layout of the bf16 product (A = source row, B = target column)
-IA = [0, 0, 1, 0, 2, 1, 1, 2]
-IB = [0, 1, 0, 2, 0, 1, 2, 1]
+# K layout of the weighted bf16 product (A = source row scaled by w = m^(-2/3),
+# B = target column): cross terms of split orders a + b <= 2 per axis,
+# w |x_j|^2 against 1, and w against |x_i|^2.
+RA = [0, 0, 1, 0, 2, 1]
+RB = [0, 1, 0, 2, 0, 1]
 
 
-def rows(p, q, source):
-    sp = [split3(v) for v in p]
-    qs = split3(q)
+def source_row(xj, w):
+    wx = [split3(np.float32(np.float64(w) * np.float64(v))) for v in xj]
+    wq = split3(np.float32(np.float64(w) * np.dot(xj.astype(np.float64), xj)))
+    ws = split3(w)
     out = np.zeros(32, np.float32)
     for d in range(3):
-        for c in range(8):
-            out[8 * d + c] = sp[d][IA[c]] if source else bf16(np.float32(-2.0) * sp[d][IB[c]])
-    for a in range(3):
-        out[24 + a] = qs[a] if source else 1.0
-        out[27 + a] = 1.0 if source else qs[a]
+        for c in range(6):
+            out[6 * d + c] = wx[d][RA[c]]
+    out[18:21] = wq
+    for c in range(6):
+        out[21 + c] = ws[RA[c]]
+    return out
+
+
+def target_col(xi):
+    tp = [split3(v) for v in xi]
+    cs = split3(np.float32(np.dot(xi.astype(np.float64), xi)))
+    out = np.zeros(32, np.float32)
+    for d in range(3):
+        for c in range(6):
+            out[6 * d + c] = bf16(np.float32(-2.0) * tp[d][RB[c]])
+    out[18:21] = 1.0
+    for c in range(6):
+        out[21 + c] = cs[RB[c]]
     return out
 
 
@@ -63,15 +79,16 @@ def test_bf16_split_is_exact():
 
 @pytest.mark.parametrize("reach", [2.0, 4.0])
 def test_bf16_product_r2_within_budget(reach):
-    """Far path 4: 30 exact bf16 products accumulated in FP32."""
+    """Far path 4: 27 exact bf16 products accumulated in FP32 give w r^2."""
+    rng = np.random.default_rng(7)
     errs = []
     for xi, xj in samples(reach):
-        a = rows(xj, np.float32(np.dot(xj.astype(np.float64), xj)), True)
-        b = rows(xi, np.float32(np.dot(xi.astype(np.float64), xi)), False)
+        w = np.float32(1.0 / np.cbrt(rng.uniform(0.1, 10.0) ** 2))
+        a, b = source_row(xj, w), target_col(xi)
         acc = np.float32(0.0)
         for k in range(32):
             acc = np.float32(acc + np.float32(a[k] * b[k]))
-        exact = float(np.sum((xj.astype(np.float64) - xi.astype(np.float64)) ** 2))
+        exact = float(w) * float(np.sum((xj.astype(np.float64) - xi.astype(np.float64)) ** 2))
         errs.append(abs(float(acc) - exact) / exact)
     errs = np.array(errs)
     assert errs.max() < 12 * EPS32 and np.median(errs) < 2 * EPS32
