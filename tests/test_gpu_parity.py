@@ -238,6 +238,29 @@ def test_central_body_fp64_terms(pkg, orc):
     assert np.median(e) < 1e-9
 
 
+@pytest.mark.parametrize("far", [0, 3, 4])
+def test_far_paths_with_central_and_massless_sources(pkg, orc, far):
+    """A star, massive bodies and massless test particles in split_central
+    mode: the weighted and matrix-core far paths give central and massless
+    sources exactly zero weight (the star comes from the FP64 pass)."""
+    rng = np.random.default_rng(11)
+    n = 3000
+    pos = rng.normal(size=(n, 3)) * np.where(np.arange(n) < 1500, 1.0, 8.0)[:, None]
+    pos[0] = 0.0
+    vel = np.zeros((n, 3))
+    mass = np.where(np.arange(n) % 3 == 0, 0.0, rng.uniform(1e-4, 1e-3, n))
+    mass[0] = 1.0
+    soft = [1e-3] * 6
+    ref = orc.compute_force(pos, mass, soft)
+    with make_ctx(pkg, pos, vel, mass, soft, "split_central", tuning=(1, 2, 16, far)) as ctx:
+        ctx.set_central(1)
+        ctx.compute_force()
+        acc = ctx.download(pos=False, vel=False)["acc"]
+    e = orc.rel_err(acc, ref)
+    print(f"far_path={far}: median {np.median(e):.3e} max {e.max():.3e}")
+    assert np.median(e) <= GATE and e.max() < 1e-5
+
+
 def test_background_point_mass(pkg, orc):
     pos, vel, mass = pkg.ics.plummer(500, seed=7)
     ref = orc.compute_force(pos, mass, [0.01] * 6)
