@@ -209,6 +209,10 @@ int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = tru
         a.part = c->part;
         a.skip_src = (c->mode == NB_MODE_SPLIT_CENTRAL && c->ncentral > 0) ? c->central : nullptr;
         a.accumulate = 0;
+        if (!f64 && !with_pot && far_path_weighted(c->tun.far_path)) {
+            NB_TRY(launch_source_weights(c->m, true, n, c->wsrc, s));
+            a.sw = c->wsrc;
+        }
         if (f64) NB_TRY(launch_force_f64(a, with_pot, s));
         else NB_TRY(launch_force_split(a, tpt, with_pot, s, c->tun.far_path));
     } else {
@@ -374,6 +378,7 @@ int nb_create(nb_ctx **out, int64_t n, int32_t precision_mode) {
             (rc = dalloc(&c->pot, sizeof(double) * nn)) ||
             (rc = dalloc(&c->tile_box, sizeof(double) * 8 * (size_t)((n + BS - 1) / BS + 1))) ||
             (rc = dalloc(&c->stage, sizeof(double) * 3 * nn)) ||
+            (rc = dalloc(&c->wsrc, sizeof(float) * nn)) ||
             (rc = dalloc(&c->red, sizeof(double) * (ENERGY_BLOCKS * 8 + 8))) ||
             (rc = dalloc(&c->time, sizeof(double) * 2)) || (rc = dalloc(&c->flag, sizeof(unsigned))) ||
             (rc = dalloc(&c->central, nn)))
@@ -421,7 +426,7 @@ int nb_destroy(nb_ctx *c) {
     dfree(c->m); dfree(c->h); dfree(c->coll);
     dfree(c->ax); dfree(c->ay); dfree(c->az); dfree(c->pot);
     dfree(c->perm);
-    dfree(c->tile_box); dfree(c->part); dfree(c->stage); dfree(c->red);
+    dfree(c->tile_box); dfree(c->part); dfree(c->stage); dfree(c->red); dfree(c->wsrc);
     dfree(c->time); dfree(c->flag); dfree(c->central);
     dfree(c->cx); dfree(c->cy); dfree(c->cz); dfree(c->cm); dfree(c->cidx);
     dfree(c->ti_end); dfree(c->ti_beg);

@@ -39,6 +39,7 @@ struct nb_batch {
     int items_tpt = 0;
     unsigned *counter = nullptr;  // [2]: next item, finished blocks
     double *stage = nullptr;      // 3n doubles
+    float *wsrc = nullptr;        // source weights m^(-2/3) (weighted far paths)
     double *red = nullptr;        // per-system reductions [nsys][8]
     // Morton order within systems
     void *sort_tmp = nullptr;

@@ -90,6 +90,7 @@ struct BatchArgs {
     const double *tile_box;
     const double *Gs;
     const double *x, *y, *z, *m, *h;
+    const float *w;               // source weights (weighted far paths) or nullptr
     const uint8_t *skip_src;
     double *ax, *ay, *az, *pot;
     const int64_t *items;
@@ -121,7 +122,7 @@ __global__ __launch_bounds__(BS) void k_batch_force(BatchArgs a) {
         bool valid[TPT];
         // force_block starts with a barrier, so item_sh is read by every
         // thread before thread 0 overwrites it in the next iteration.
-        force_block<double, TPT, POT, MF>(sh, a.x, a.y, a.z, a.h, 0.0, i0, o1, a.x, a.y, a.z, a.m, o0,
+        force_block<double, TPT, POT, MF>(sh, a.x, a.y, a.z, a.h, 0.0, i0, o1, a.x, a.y, a.z, a.m, a.w, o0,
                                       o1, a.tile_box + a.tile_base[s] * 8, a.skip_src, true, Ax,
                                       Ay, Az, Aq, valid);
         const double G = a.Gs[s];
@@ -634,6 +635,10 @@ static int bforce(nb_batch *b, bool with_pot, bool allow_timing) {
         k_batch_tile_boxes<<<(unsigned)b->ntiles, BS, 0, s>>>(b->x, b->y, b->z, b->off,
                                                               b->tile_base, b->tile_sys, b->tile_box);
         NB_TRY(launch_ok("k_batch_tile_boxes"));
+        if (!with_pot && far_path_weighted(b->tun.far_path)) {
+            NB_TRY(launch_source_weights(b->m, true, b->n, b->wsrc, s));
+            a.w = b->wsrc;
+        }
         int nblk = b->tun.blocks > 0 ? b->tun.blocks : 8 * b->cu_count;
         nblk = (int)std::max<int64_t>(1, std::min<int64_t>(nblk, b->nitems));
 #define NB_BL(T_, P_) k_batch_force<T_, P_><<<nblk, BS, 0, s>>>(a)
@@ -864,6 +869,7 @@ int nb_batch_create(nb_batch **out, int32_t nsys, const int64_t *offsets, int32_
             (rc = dalloc(&b->tile_box, sizeof(double) * 8 * (size_t)std::max<int64_t>(b->ntiles, 1))) ||
             (rc = dalloc(&b->counter, sizeof(unsigned) * 2)) ||
             (rc = dalloc(&b->stage, sizeof(double) * 3 * n)) ||
+            (rc = dalloc(&b->wsrc, sizeof(float) * std::max<int64_t>(n, 1))) ||
             (rc = dalloc(&b->red, sizeof(double) * 8 * std::max<size_t>(ns, 1))))
             break;
         e = hipMemcpy(b->off, offsets, sizeof(int64_t) * (ns + 1), hipMemcpyHostToDevice);
@@ -908,7 +914,7 @@ int nb_batch_destroy(nb_batch *b) {
     dfree(b->m); dfree(b->h); dfree(b->ax); dfree(b->ay); dfree(b->az); dfree(b->pot);
     dfree(b->perm); dfree(b->central); dfree(b->cidx);
     dfree(b->tile_base); dfree(b->tile_sys); dfree(b->tile_box);
-    dfree(b->items); dfree(b->counter); dfree(b->stage); dfree(b->red);
+    dfree(b->items); dfree(b->counter); dfree(b->stage); dfree(b->red); dfree(b->wsrc);
     dfree(b->sort_tmp); dfree(b->keys); dfree(b->idx); dfree(b->sysbox); dfree(b->spare);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;

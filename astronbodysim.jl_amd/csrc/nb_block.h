@@ -532,7 +532,7 @@ __device__ __forceinline__ void force_block(
     BlockShared<T, MF> &sh, const T *__restrict__ tx, const T *__restrict__ ty,
     const T *__restrict__ tz, const double *__restrict__ th, double th_const, int64_t i0,
     int64_t i_end, const T *__restrict__ sx, const T *__restrict__ sy, const T *__restrict__ sz,
-    const T *__restrict__ sm, int64_t j0, int64_t j1, const double *__restrict__ tile_box,
+    const T *__restrict__ sm, const float *__restrict__ sw, int64_t j0, int64_t j1, const double *__restrict__ tile_box,
     const uint8_t *__restrict__ skip_src, bool use_origin, double (&Ax)[TPT], double (&Ay)[TPT],
     double (&Az)[TPT], double (&Aq)[TPT], bool (&valid)[TPT]) {
     constexpr bool HILO = BlockShared<T, MF>::HILO;
@@ -634,13 +634,19 @@ __device__ __forceinline__ void force_block(
         const float c = (float)re2;
         ci = nb_f4v{c, c, c, c};
     }
-    // Register prefetch of the first tile.
+    // Register prefetch of the first tile (with the source weight when the
+    // weighted far paths are compiled in).
+    constexpr bool WEIGHTS = WFAR || ROWS;
     T rx = 0, ry = 0, rz = 0, rm = 0;
+    float rw = 0.0f;
     bool rv = false;
     {
         const int64_t j = j0 + tid;
         rv = j < j1 && !(skip_src && skip_src[j]);
-        if (rv) { rx = sx[j]; ry = sy[j]; rz = sz[j]; rm = sm[j]; }
+        if (rv) {
+            rx = sx[j]; ry = sy[j]; rz = sz[j]; rm = sm[j];
+            if (WEIGHTS) rw = sw ? sw[j] : (rm > 0 ? (float)(1.0 / cbrt((double)rm * (double)rm)) : 0.0f);
+        }
     }
     int it = 0;
     for (int64_t base = j0; base < j1; base += BS, ++it) {
@@ -664,8 +670,8 @@ __device__ __forceinline__ void force_block(
             }
             if constexpr (MF == 2) sh.sQ[MF == 2 ? buf : 0][MF == 2 ? tid : 0] = (float)(dx * dx + dy * dy + dz * dz);
             if constexpr (ROWS) {
-                if (rm > 0) {
-                    const double wd = 1.0 / cbrt((double)rm * (double)rm);
+                if (rw > 0.0f) {
+                    const double wd = (double)rw;
                     const double q2 = (double)hx * hx + (double)hy * hy + (double)hz * hz;
                     bf16_source_row((float)(wd * hx), (float)(wd * hy), (float)(wd * hz),
                                     (float)(wd * q2), (float)wd, sh.sA[0][ROWS ? tid : 0]);
@@ -677,10 +683,10 @@ __device__ __forceinline__ void force_block(
                 const double q2 = (double)hx * hx + (double)hy * hy + (double)hz * hz;
                 float4 W;
                 float w;
-                if (rm > 0) {
-                    const double wd = 1.0 / cbrt((double)rm * (double)rm);
+                if (rw > 0.0f) {
+                    const double wd = (double)rw;
                     W = make_float4((float)(wd * hx), (float)(wd * hy), (float)(wd * hz), (float)(wd * q2));
-                    w = (float)wd;
+                    w = rw;
                 } else {
                     W = make_float4(0.0f, 0.0f, 0.0f, FARW_MASSLESS);
                     w = 0.0f;
@@ -709,7 +715,10 @@ __device__ __forceinline__ void force_block(
         {
             const int64_t j = base + BS + tid;
             rv = j < j1 && !(skip_src && skip_src[j]);
-            if (rv) { rx = sx[j]; ry = sy[j]; rz = sz[j]; rm = sm[j]; }
+            if (rv) {
+                rx = sx[j]; ry = sy[j]; rz = sz[j]; rm = sm[j];
+                if (WEIGHTS) rw = sw ? sw[j] : (rm > 0 ? (float)(1.0 / cbrt((double)rm * (double)rm)) : 0.0f);
+            }
         }
         const double d2 = tile_gap2(tile_box + ((base - j0) / BS) * 8, wg);
         if (d2 <= soft_d2) {

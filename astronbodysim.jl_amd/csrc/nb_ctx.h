@@ -68,6 +68,7 @@ struct nb_ctx {
     double *part = nullptr;
     int part_cap = 0;             // segments the partial buffer holds
     double *stage = nullptr;      // 3n doubles
+    float *wsrc = nullptr;        // source weights m^(-2/3) (weighted far paths), n floats
     double *red = nullptr;        // energy scratch
     double *time = nullptr;       // two FP64 time slots for device-side drift
     unsigned *flag = nullptr;

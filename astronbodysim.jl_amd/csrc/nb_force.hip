@@ -35,7 +35,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MF == 4 ? 4 
     double Ax[TPT], Ay[TPT], Az[TPT], Aq[TPT];
     bool valid[TPT];
     force_block<T, TPT, POT, MF>(sh, a.tx, a.ty, a.tz, a.th, a.th_const, i0, a.nt, a.sx, a.sy, a.sz,
-                             a.sm, j0, j1, a.tile_box + (j0 / BS) * 8, a.skip_src,
+                             a.sm, a.sw, j0, j1, a.tile_box + (j0 / BS) * 8, a.skip_src,
                              a.use_origin != 0, Ax, Ay, Az, Aq, valid);
 #pragma unroll
     for (int k = 0; k < TPT; ++k) {
@@ -228,6 +228,22 @@ int launch_force_split(const ForceArgs<double> &a, int tpt, bool pot, hipStream_
     }
     if (far_path == FAR_WEIGHTED) return launch_f32_impl<double, FAR_WEIGHTED>(a, tpt, pot, s);
     return launch_f32_impl<double, FAR_VALU>(a, tpt, pot, s);
+}
+
+template <typename T>
+__global__ void k_source_weights(const T *__restrict__ m, int64_t n, float *__restrict__ w) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double mi = (double)m[i];
+    w[i] = mi > 0.0 ? (float)(1.0 / cbrt(mi * mi)) : 0.0f;
+}
+
+int launch_source_weights(const void *m, bool is_double, int64_t n, float *w, hipStream_t s) {
+    if (n <= 0) return NB_OK;
+    const unsigned g = (unsigned)((n + 255) / 256);
+    if (is_double) k_source_weights<double><<<g, 256, 0, s>>>((const double *)m, n, w);
+    else k_source_weights<float><<<g, 256, 0, s>>>((const float *)m, n, w);
+    return check_launch("k_source_weights");
 }
 
 int launch_force_f64(const ForceArgs<double> &a, bool pot, hipStream_t s) {

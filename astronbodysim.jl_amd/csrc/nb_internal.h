@@ -51,6 +51,7 @@ struct ForceArgs {
     double *part;                // partials [nseg][4][nt] (nseg > 1)
     const uint8_t *skip_src;     // sources excluded from the FP32 sum (central bodies) or nullptr
     int accumulate;              // 1: add G*sum to the outputs instead of storing (nseg == 1)
+    const float *sw;             // source weights m^(-2/3) (far paths 3 and 4) or nullptr
 };
 
 // Far-tile paths (the far_path tuning value; nb_block.h describes each).
@@ -67,6 +68,9 @@ int launch_force_f32(const ForceArgs<float> &a, int tpt, bool pot, hipStream_t s
 int launch_force_split(const ForceArgs<double> &a, int tpt, bool pot, hipStream_t s,
                        int far_path = 0);
 int launch_force_f64(const ForceArgs<double> &a, bool pot, hipStream_t s);
+// w_i = m_i^(-2/3) (0 for massless particles) for the weighted far paths.
+int launch_source_weights(const void *m, bool is_double, int64_t n, float *w, hipStream_t s);
+inline bool far_path_weighted(int far_path) { return far_path == FAR_WEIGHTED || far_path == FAR_MFMA_BF16; }
 int launch_finalize(const double *part, int nseg, int64_t n, double G, double *ax, double *ay,
                     double *az, double *pot, bool with_pot, hipStream_t s);
 

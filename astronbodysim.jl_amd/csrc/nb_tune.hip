@@ -106,7 +106,7 @@ int autotune_ctx(nb_ctx *c) {
             if (nseg > ntiles) break;
             if (nblk * nseg > (1 << 20)) break;
             if ((double)n * nseg * 32.0 > 4.0e9) break;  // partial-sum buffer bound
-            if (nseg > 1 && nblk * (nseg / 2) >= 32768) break;  // the grid was already full
+            if (nseg > 1 && nblk * (nseg / 2) >= 65536) break;  // the grid was already full
             if (nblk * nseg < 64 && nseg < ntiles && si + 1 < 8) continue;  // far too few blocks
             c->tun.targets_per_thread = tpt;
             c->tun.j_segments = nseg;
