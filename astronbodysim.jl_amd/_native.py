@@ -348,8 +348,8 @@ class Context:
 
     def add_background(self, kind: int, params=()):
         p = np.ascontiguousarray(np.asarray(params, dtype=np.float64).reshape(-1))
-        if p.size < 5:
-            p = np.concatenate([p, np.zeros(5 - p.size)])
+        if p.size < 8:
+            p = np.concatenate([p, np.zeros(8 - p.size)])
         call("nb_add_background", self.handle, int(kind), dptr(p))
 
     def set_mond(self, acc0: float, nu_index: float = 2.0):

@@ -830,7 +830,7 @@ int nb_add_background(nb_ctx *c, int32_t kind, const double *params) {
         c->bg.clear();
         return NB_OK;
     }
-    if ((kind != 1 && kind != 2) || !params) {
+    if (kind < 1 || kind > 4 || !params) {
         set_error("unknown background field kind %d", kind);
         return NB_ERR_INVALID;
     }
@@ -840,7 +840,7 @@ int nb_add_background(nb_ctx *c, int32_t kind, const double *params) {
     }
     BgField f{};
     f.kind = kind;
-    const int np = kind == 1 ? 4 : 5;
+    const int np = kind == 1 ? 4 : kind == 2 ? 5 : kind == 3 ? 7 : 3;
     for (int k = 0; k < np; ++k) {
         if (!isfinite(params[k])) {
             set_error("non-finite background parameter %d", k);

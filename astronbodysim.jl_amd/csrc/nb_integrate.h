@@ -10,8 +10,8 @@ struct SoftTable {
 };
 
 struct BgField {
-    int kind;      // 1 point mass, 2 Paczynski-Wiita
-    double p[5];   // GM, cx, cy, cz, r_g
+    int kind;      // 1 point mass, 2 Paczynski-Wiita, 3 radial polynomial, 4 uniform
+    double p[8];   // see nb_add_background
 };
 
 constexpr int ENERGY_BLOCKS = 512;

@@ -189,6 +189,16 @@ __global__ void k_background(const T *__restrict__ x, const T *__restrict__ y,
                              double *__restrict__ az, double *__restrict__ pot) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (f.kind == 4) {
+        // uniform field g: a = g, phi = -g . x
+        if (ax) {
+            ax[i] += f.p[0];
+            ay[i] += f.p[1];
+            az[i] += f.p[2];
+        }
+        if (pot) pot[i] -= f.p[0] * (double)x[i] + f.p[1] * (double)y[i] + f.p[2] * (double)z[i];
+        return;
+    }
     const double dx = (double)x[i] - f.p[1], dy = (double)y[i] - f.p[2], dz = (double)z[i] - f.p[3];
     const double r2 = dx * dx + dy * dy + dz * dz;
     const double r = sqrt(r2);
@@ -197,10 +207,17 @@ __global__ void k_background(const T *__restrict__ x, const T *__restrict__ y,
     if (f.kind == 1) {
         fac = -GM / (r2 * r);
         phi = -GM / r;
-    } else {
+    } else if (f.kind == 2) {
         const double rr = r - f.p[4];
         fac = -GM / (r * rr * rr);
         phi = -GM / rr;
+    } else {
+        // radial polynomial: |a| = c0 + c1 r + c2 r^2 + c3 r^3 toward the
+        // centre, phi = c0 r + c1 r^2/2 + c2 r^3/3 + c3 r^4/4 (zero at r = 0)
+        const double c0 = f.p[0], c1 = f.p[4], c2 = f.p[5], c3 = f.p[6];
+        const double mag = c0 + r * (c1 + r * (c2 + r * c3));
+        fac = r > 0.0 ? -mag / r : 0.0;
+        phi = r * (c0 + r * (0.5 * c1 + r * (c2 / 3.0 + r * 0.25 * c3)));
     }
     if (ax) {
         ax[i] += fac * dx;

@@ -185,7 +185,10 @@ NB_API int nb_set_central(nb_ctx *ctx, int64_t count);
  * apply_background_potential  src/base/PostProcessing/Potential.jl:1-7 with a
  * fixed menu.  kind 0 clears all fields; kind 1 = point mass
  * params = {GM, cx, cy, cz}; kind 2 = Paczynski-Wiita pseudo-Newtonian mass
- * params = {GM, cx, cy, cz, r_g}  (src/tools/BlackHole.jl:48-60). */
+ * params = {GM, cx, cy, cz, r_g}  (src/tools/BlackHole.jl:48-60); kind 3 =
+ * radial polynomial about a centre, |a| = c0 + c1 r + c2 r^2 + c3 r^3 toward
+ * it, phi = c0 r + c1 r^2/2 + c2 r^3/3 + c3 r^4/4, params = {c0, cx, cy, cz,
+ * c1, c2, c3}; kind 4 = uniform field, params = {gx, gy, gz}, phi = -g.x. */
 NB_API int nb_add_background(nb_ctx *ctx, int32_t kind, const double *params);
 
 /* MOND: after the background fields, Acc <- Acc * nu(|Acc|/acc0, nu_index) with

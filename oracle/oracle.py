@@ -186,6 +186,35 @@ def background_point_mass(pos, GM, center, acc=None, pot=None, G=1.0):
                                  _p(acc), _p(pot))
 
 
+def background_field(pos, kind, params):
+    """Analytic background fields (acc, pot) at `pos` in FP64.
+    kind 1: point mass {GM, cx, cy, cz}; kind 2: Paczynski-Wiita
+    pseudo-Newtonian mass {GM, cx, cy, cz, r_g}, a = -GM/(R - r_g)^2 n,
+    phi = -GM/(R - r_g) (src/tools/BlackHole.jl:18-60); kind 3: radial
+    polynomial {c0, cx, cy, cz, c1, c2, c3}, |a| = c0 + c1 R + c2 R^2 + c3 R^3
+    toward the centre; kind 4: uniform {gx, gy, gz}, phi = -g.x."""
+    pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
+    p = list(params) + [0.0] * (8 - len(params))
+    if kind == 4:
+        g = np.array(p[:3])
+        return np.broadcast_to(g, pos.shape).copy(), -(pos @ g)
+    d = pos - np.array(p[1:4])
+    R = np.sqrt(np.sum(d * d, axis=1))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        n = np.where(R[:, None] > 0, d / R[:, None], 0.0)
+    if kind == 1:
+        mag, phi = p[0] / R ** 2, -p[0] / R
+    elif kind == 2:
+        mag, phi = p[0] / (R - p[4]) ** 2, -p[0] / (R - p[4])
+    elif kind == 3:
+        c0, c1, c2, c3 = p[0], p[4], p[5], p[6]
+        mag = c0 + c1 * R + c2 * R ** 2 + c3 * R ** 3
+        phi = c0 * R + c1 * R ** 2 / 2 + c2 * R ** 3 / 3 + c3 * R ** 4 / 4
+    else:
+        raise ValueError(f"unknown background kind {kind}")
+    return -mag[:, None] * n, phi
+
+
 def mond_nu(y, n=2.0):
     """Generalised MOND interpolation nu(y, n) (src/mond/milgrom1983.jl:6-8)."""
     y = np.asarray(y, dtype=np.float64)

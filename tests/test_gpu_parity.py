@@ -273,6 +273,30 @@ def test_background_point_mass(pkg, orc):
     assert np.max(orc.rel_err(acc, ref)) < 1e-12
 
 
+@pytest.mark.parametrize("kind,params", [
+    (1, [3.0, 5.0, 0.0, 0.0]), (2, [3.0, 5.0, 0.0, 0.0, 0.4]),
+    (3, [0.1, 0.5, -1.0, 0.2, 0.7, -0.2, 0.05]), (4, [0.3, -0.2, 0.9])])
+@pytest.mark.parametrize("mode", ["fp64", "split"])
+def test_background_field_menu(pkg, orc, kind, params, mode):
+    """Every analytic field of nb_add_background, acceleration and potential,
+    against the oracle's restatement (src/tools/BlackHole.jl:18-60 for kind 2)."""
+    pos, vel, mass = pkg.ics.plummer(700, seed=17)
+    soft = [0.01] * 6
+    bacc, bpot = orc.background_field(pos, kind, params)
+    ref = orc.compute_force(pos, mass, soft) + bacc
+    refp = orc.compute_potential(pos, mass, soft) + bpot
+    with make_ctx(pkg, pos, vel, mass, soft, mode) as ctx:
+        ctx.add_background(kind, params)
+        ctx.compute_force()
+        acc = ctx.download(pos=False, vel=False)["acc"]
+        ctx.compute_potential()
+        pot = ctx.download(pos=False, vel=False, acc=False, pot=True)["pot"]
+    tol = 1e-12 if mode == "fp64" else 1e-5
+    assert np.median(orc.rel_err(acc, ref)) <= (1e-13 if mode == "fp64" else GATE)
+    assert np.max(orc.rel_err(acc, ref)) < tol
+    assert np.max(np.abs(pot - refp)) / np.max(np.abs(refp)) < tol
+
+
 def test_timing_counters(pkg):
     pos, vel, mass = pkg.ics.plummer(2048, seed=3)
     with make_ctx(pkg, pos, vel, mass, [0.01] * 6, "split") as ctx:

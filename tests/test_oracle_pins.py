@@ -429,3 +429,36 @@ def test_mond_point_mass_flat_rotation(orc):
     assert np.all(am[0] == 0.0)                  # massless sources: the centre feels nothing
     v4 = (np.linalg.norm(am[1:], axis=1) * r) ** 2
     assert np.allclose(v4, G * M * a0, rtol=1e-3)
+
+
+@pytest.mark.parametrize("kind,params", [
+    (1, [2.0, 0.5, -1.0, 0.2]), (2, [2.0, 0.5, -1.0, 0.2, 0.3]),
+    (3, [0.1, 0.5, -1.0, 0.2, 0.7, -0.2, 0.05]), (4, [0.3, -0.2, 0.9])])
+def test_background_fields_are_gradients(orc, kind, params):
+    """a = -grad(phi) by central differences, for every field of the menu."""
+    rng = np.random.default_rng(kind)
+    pos = rng.normal(size=(50, 3)) * 2.0 + np.array([3.0, 0.0, 0.0])
+    acc, _ = orc.background_field(pos, kind, params)
+    hstep = 1e-5
+    num = np.zeros_like(pos)
+    for d in range(3):
+        e = np.zeros(3)
+        e[d] = hstep
+        num[:, d] = -(orc.background_field(pos + e, kind, params)[1]
+                      - orc.background_field(pos - e, kind, params)[1]) / (2 * hstep)
+    assert np.allclose(acc, num, rtol=1e-6, atol=1e-9)
+
+
+def test_background_fields_closed_forms(orc):
+    rng = np.random.default_rng(4)
+    pos = rng.normal(size=(40, 3)) * 3.0
+    # Paczynski-Wiita with r_g = 0 is the point mass of the C oracle
+    acc_pm, pot_pm = np.zeros((40, 3)), np.zeros(40)
+    orc.background_point_mass(pos, 1.5, np.array([0.2, 0.1, -0.3]), acc=acc_pm, pot=pot_pm)
+    a2, p2 = orc.background_field(pos, 2, [1.5, 0.2, 0.1, -0.3, 0.0])
+    assert np.allclose(a2, acc_pm, rtol=1e-13) and np.allclose(p2, pot_pm, rtol=1e-13)
+    # radial polynomial with only c1 = k is the harmonic field a = -k (x - c)
+    a3, p3 = orc.background_field(pos, 3, [0.0, 1.0, 2.0, 3.0, 0.7, 0.0, 0.0])
+    d = pos - np.array([1.0, 2.0, 3.0])
+    assert np.allclose(a3, -0.7 * d, rtol=1e-13)
+    assert np.allclose(p3, 0.35 * np.sum(d * d, axis=1), rtol=1e-13)
