@@ -88,6 +88,37 @@ def test_ragged_sizes(pkg, orc, n):
             assert np.median(orc.rel_err(acc, ref)) <= tol, mode
 
 
+@pytest.mark.parametrize("mode", ["fp64", "split", "fp32"])
+def test_empty_system(pkg, mode):
+    """N = 0: every entry point is a no-op that succeeds."""
+    z3, z1 = np.zeros((0, 3)), np.zeros(0)
+    with make_ctx(pkg, z3, z3, z1, [0.01] * 6, mode) as ctx:
+        ctx.compute_force()
+        ctx.compute_potential()
+        ctx.kdk_steps(4, 1e-3, 0.0)
+        d = ctx.download(pos=True, vel=True, acc=True, pot=True)
+        assert all(v.size == 0 for v in d.values())
+        e = ctx.energy()
+        assert e["K"] == 0.0 and e["W"] == 0.0
+
+
+@pytest.mark.parametrize("mode", ["fp64", "split"])
+def test_coincident_particles(pkg, orc, mode):
+    """Distinct particles at one position: with h > 0 the spline gives a
+    zero pair force at r = 0 (src/directsum/gravity.jl:8-28), as for the self
+    pair, and the rest of the system is unaffected."""
+    pos, vel, mass = pkg.ics.plummer(600, seed=19)
+    pos[10] = pos[11] = pos[12]
+    soft = [0.01] * 6
+    ref = orc.compute_force(pos, mass, soft)
+    with make_ctx(pkg, pos, vel, mass, soft, mode) as ctx:
+        ctx.compute_force()
+        acc = ctx.download(pos=False, vel=False)["acc"]
+    assert np.all(np.isfinite(acc))
+    tol = 1e-12 if mode == "fp64" else 1e-5
+    assert np.max(orc.rel_err(acc, ref)) < tol
+
+
 def test_binary_closed_forms(pkg, orc):
     G, m, D, h = 1.0, 1.0e10, 2.0, 1e-4
     pos, vel, mass = pkg.ics.binary(m, D, G)
