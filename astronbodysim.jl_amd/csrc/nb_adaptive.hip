@@ -17,6 +17,7 @@
 // keep their accelerations (the reference CPU path does; its GPU path zeroes
 // them, src/directsumgpu/gravity.jl:43).
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cstring>
 #include <math.h>
 #include <rocprim/device/device_select.hpp>
@@ -356,8 +357,15 @@ int nb_adaptive_compute_force(nb_ctx *c, int64_t t_int, int64_t *nactive) {
             (const float *)c->x, (const float *)c->y, (const float *)c->z, c->h, c->ad.idx,
             c->ad.cnt, (float *)gx, (float *)gy, (float *)gz, gh);
     NB_TRY(launch_ok("k_gather_targets"));
+    // Few active targets still fill the GPU: split the sources into segments
+    // (partials reduced in a fixed order) for about 4096 blocks.
     const int64_t ntiles = (n + BS - 1) / BS;
     const int64_t nblk = (na + BS - 1) / BS;
+    int64_t want = (4096 + nblk - 1) / nblk;
+    want = std::max<int64_t>(1, std::min<int64_t>(want, std::min<int64_t>(ntiles, 64)));
+    const int64_t tiles_per_seg = (ntiles + want - 1) / want;
+    const int nseg = (int)((ntiles + tiles_per_seg - 1) / tiles_per_seg);
+    NB_TRY(ctx_ensure_part(c, nseg));
     if (c->is_double) {
         ForceArgs<double> a{};
         a.tx = (const double *)gx; a.ty = (const double *)gy; a.tz = (const double *)gz;
@@ -365,16 +373,18 @@ int nb_adaptive_compute_force(nb_ctx *c, int64_t t_int, int64_t *nactive) {
         a.sx = (const double *)c->x; a.sy = (const double *)c->y; a.sz = (const double *)c->z;
         a.sm = (const double *)c->m; a.ns = n;
         a.tile_box = c->tile_box; a.use_origin = 1;
-        a.nseg = 1; a.seg_len = ntiles * BS; a.nblk_i = (int)nblk; a.G = c->G;
-        a.ax = gax; a.ay = gay; a.az = gaz; a.pot = nullptr;
+        a.nseg = nseg; a.seg_len = tiles_per_seg * BS; a.nblk_i = (int)nblk; a.G = c->G;
+        a.ax = gax; a.ay = gay; a.az = gaz; a.pot = nullptr; a.part = c->part;
         a.skip_src = (c->mode == NB_MODE_SPLIT_CENTRAL && c->ncentral > 0) ? c->central : nullptr;
         if (c->mode == NB_MODE_FP64) NB_TRY(launch_force_f64(a, false, s));
         else NB_TRY(launch_force_split(a, 1, false, s));
+        if (nseg > 1) NB_TRY(launch_finalize(c->part, nseg, na, c->G, gax, gay, gaz, nullptr, false, s));
         if (c->mode == NB_MODE_SPLIT_CENTRAL && c->ncentral > 0) {
             // FP64 central terms for the active targets
             ForceArgs<double> b = a;
             b.sx = c->cx; b.sy = c->cy; b.sz = c->cz; b.sm = c->cm; b.ns = c->ncentral;
             b.skip_src = nullptr; b.tile_box = nullptr; b.accumulate = 1;
+            b.nseg = 1; b.part = nullptr;
             b.seg_len = ((c->ncentral + BS - 1) / BS) * BS;
             NB_TRY(ctx_gather_central(c));
             NB_TRY(launch_force_f64(b, false, s));
@@ -386,9 +396,10 @@ int nb_adaptive_compute_force(nb_ctx *c, int64_t t_int, int64_t *nactive) {
         a.sx = (const float *)c->x; a.sy = (const float *)c->y; a.sz = (const float *)c->z;
         a.sm = (const float *)c->m; a.ns = n;
         a.tile_box = c->tile_box; a.use_origin = 0;
-        a.nseg = 1; a.seg_len = ntiles * BS; a.nblk_i = (int)nblk; a.G = c->G;
-        a.ax = gax; a.ay = gay; a.az = gaz; a.pot = nullptr;
+        a.nseg = nseg; a.seg_len = tiles_per_seg * BS; a.nblk_i = (int)nblk; a.G = c->G;
+        a.ax = gax; a.ay = gay; a.az = gaz; a.pot = nullptr; a.part = c->part;
         NB_TRY(launch_force_f32(a, 1, false, s));
+        if (nseg > 1) NB_TRY(launch_finalize(c->part, nseg, na, c->G, gax, gay, gaz, nullptr, false, s));
     }
     // post-processing of the active targets, in the order of
     // src/base/PostProcessing/Force.jl:33-46: background fields, then MOND

@@ -292,6 +292,7 @@ int ctx_run_force(nb_ctx *c, bool with_pot, bool allow_timing) {
 int ctx_rebuild_central(nb_ctx *c) { return rebuild_central(c); }
 int ctx_gather_central(nb_ctx *c) { return gather_central(c); }
 void ctx_drop_graph(nb_ctx *c) { drop_graph(c); }
+int ctx_ensure_part(nb_ctx *c, int nseg) { return ensure_part(c, nseg); }
 }  // namespace nb
 
 // ===========================================================================

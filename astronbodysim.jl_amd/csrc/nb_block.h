@@ -333,13 +333,6 @@ __device__ __forceinline__ double uniform_f64(double v) {
 }
 
 // ---- bf16-split squared distances on the matrix cores (FAR_MFMA_BF16) ----
-// Three bf16 whose sum is x: each remainder is exact in FP32.
-__device__ __forceinline__ void split_bf16(float x, __bf16 (&p)[3]) {
-    p[0] = (__bf16)x;
-    const float r = x - (float)p[0];
-    p[1] = (__bf16)r;
-    p[2] = (__bf16)(r - (float)p[1]);
-}
 
 // K layout of the weighted squared-distance product (A = source row, B =
 // target column), w_j = m_j^(-2/3):
@@ -347,64 +340,91 @@ __device__ __forceinline__ void split_bf16(float x, __bf16 (&p)[3]) {
 //   k = 18..20:                         A = split of w |x'_j|^2, B = 1
 //   k = 21..26:                         A = w^RA[c], B = (|x'_i|^2)^RB[c]
 //   k = 27..31:                         0
-// with (RA, RB) = (0,0) (0,1) (1,0) (0,2) (2,0) (1,1): every cross term of
+// with (RA[c], RB[c]) = (0,0) (0,1) (1,0) (0,2) (2,0) (1,1): every cross term of
 // split orders a + b <= 2, the dropped ones being below 2^-23 of the product.
 // The product is w_j r_ij^2, so f_ij = m_j r_ij^-3 = rsq(w_j r_ij^2)^3.
-__device__ constexpr int RA[6] = {0, 0, 1, 0, 2, 1};
-__device__ constexpr int RB[6] = {0, 1, 0, 2, 0, 1};
 
-__device__ __forceinline__ void set_k(nb_bf8 (&out)[4], int k, __bf16 v) { out[k >> 3][k & 7] = v; }
+typedef float nb_f2v __attribute__((ext_vector_type(2)));
+typedef __bf16 nb_bf2 __attribute__((ext_vector_type(2)));
+
+// Two FP32 values (each exactly a bf16) packed into one dword (v_cvt_pk_bf16_f32).
+__device__ __forceinline__ unsigned pk_bf16(float a, float b) {
+    const nb_f2v v = {a, b};
+    return __builtin_bit_cast(unsigned, __builtin_convertvector(v, nb_bf2));
+}
+
+// The three-bf16 splits of two values at once, as FP32 (each exactly a bf16).
+__device__ __forceinline__ void split2(float x0, float x1, float (&s0)[3], float (&s1)[3]) {
+    const nb_f2v v = {x0, x1};
+    const nb_f2v h = __builtin_convertvector(__builtin_convertvector(v, nb_bf2), nb_f2v);
+    const nb_f2v r = v - h;
+    const nb_f2v m = __builtin_convertvector(__builtin_convertvector(r, nb_bf2), nb_f2v);
+    const nb_f2v l = r - m;
+    s0[0] = h.x; s0[1] = m.x; s0[2] = l.x;
+    s1[0] = h.y; s1[1] = m.y; s1[2] = l.y;
+}
+
+__device__ __forceinline__ void store_words(const unsigned (&w)[16], nb_bf8 (&out)[4]) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const nb_u4v u = {w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]};
+        out[q] = __builtin_bit_cast(nb_bf8, u);
+    }
+}
 
 // A row of a source with weight w > 0: (wx, wy, wz) = w x'_j, wq = w |x'_j|^2.
+// Pairs of K entries are packed from their FP32 splits: entries 6d..6d+5 are
+// (h, h, m, h, l, m) of axis d (RA order), 18..20 (h, m, l) of wq, 21..26 the
+// RA order of w, then zeros.
 __device__ __forceinline__ void bf16_source_row(float wx, float wy, float wz, float wq, float w,
                                                 nb_bf8 (&out)[4]) {
-    __bf16 sp[3][3], qs[3], ws[3];
-    split_bf16(wx, sp[0]);
-    split_bf16(wy, sp[1]);
-    split_bf16(wz, sp[2]);
-    split_bf16(wq, qs);
-    split_bf16(w, ws);
-    const __bf16 one = (__bf16)1.0f, zero = (__bf16)0.0f;
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-#pragma unroll
-        for (int c = 0; c < 6; ++c) set_k(out, 6 * d + c, sp[d][RA[c]]);
-#pragma unroll
-    for (int a = 0; a < 3; ++a) set_k(out, 18 + a, qs[a]);
-#pragma unroll
-    for (int c = 0; c < 6; ++c) set_k(out, 21 + c, ws[RA[c]]);
-#pragma unroll
-    for (int k = 27; k < 32; ++k) set_k(out, k, zero);
-    (void)one;
+    float sx[3], sy[3], sz[3], sq[3], sw[3], s0[3];
+    split2(wx, wy, sx, sy);
+    split2(wz, wq, sz, sq);
+    split2(w, 0.0f, sw, s0);
+    unsigned d[16];
+    d[0] = pk_bf16(sx[0], sx[0]); d[1] = pk_bf16(sx[1], sx[0]); d[2] = pk_bf16(sx[2], sx[1]);
+    d[3] = pk_bf16(sy[0], sy[0]); d[4] = pk_bf16(sy[1], sy[0]); d[5] = pk_bf16(sy[2], sy[1]);
+    d[6] = pk_bf16(sz[0], sz[0]); d[7] = pk_bf16(sz[1], sz[0]); d[8] = pk_bf16(sz[2], sz[1]);
+    d[9] = pk_bf16(sq[0], sq[1]); d[10] = pk_bf16(sq[2], sw[0]); d[11] = pk_bf16(sw[0], sw[1]);
+    d[12] = pk_bf16(sw[0], sw[2]); d[13] = pk_bf16(sw[1], 0.0f); d[14] = 0u; d[15] = 0u;
+    store_words(d, out);
 }
 
 // A row of a massless (or padding) source: w r^2 = 2^120, so f = 0 exactly.
 __device__ __forceinline__ void bf16_massless_row(nb_bf8 (&out)[4]) {
-    const __bf16 zero = (__bf16)0.0f;
+    unsigned d[16];
 #pragma unroll
-    for (int k = 0; k < 32; ++k) set_k(out, k, zero);
-    set_k(out, 18, (__bf16)FARW_MASSLESS);
+    for (int q = 0; q < 16; ++q) d[q] = 0u;
+    d[9] = pk_bf16(FARW_MASSLESS, 0.0f);
+    store_words(d, out);
 }
 
-// B column of a target at x' with c = |x'|^2.
+// B column of a target at x' with c = |x'|^2: entries 6d..6d+5 are -2 times
+// (h, m, h, l, h, m) of axis d (RB order), 18..20 ones, 21..26 the RB order
+// of c, then zeros.
 __device__ __forceinline__ void bf16_target_col(float x, float y, float z, float c,
                                                 nb_bf8 (&out)[4]) {
-    __bf16 tp[3][3], cs[3];
-    split_bf16(x, tp[0]);
-    split_bf16(y, tp[1]);
-    split_bf16(z, tp[2]);
-    split_bf16(c, cs);
-    const __bf16 one = (__bf16)1.0f, zero = (__bf16)0.0f;
+    float tx[3], ty[3], tz[3], tc[3];
+    split2(x, y, tx, ty);
+    split2(z, c, tz, tc);
+    float (*t[3])[3] = {&tx, &ty, &tz};
+    unsigned d[16];
 #pragma unroll
-    for (int d = 0; d < 3; ++d)
-#pragma unroll
-        for (int c6 = 0; c6 < 6; ++c6) set_k(out, 6 * d + c6, (__bf16)(-2.0f * (float)tp[d][RB[c6]]));
-#pragma unroll
-    for (int a = 0; a < 3; ++a) set_k(out, 18 + a, one);
-#pragma unroll
-    for (int c6 = 0; c6 < 6; ++c6) set_k(out, 21 + c6, cs[RB[c6]]);
-#pragma unroll
-    for (int k = 27; k < 32; ++k) set_k(out, k, zero);
+    for (int a = 0; a < 3; ++a) {
+        const float h = -2.0f * (*t[a])[0], m = -2.0f * (*t[a])[1], l = -2.0f * (*t[a])[2];
+        d[3 * a] = pk_bf16(h, m);
+        d[3 * a + 1] = pk_bf16(h, l);
+        d[3 * a + 2] = pk_bf16(h, m);
+    }
+    d[9] = pk_bf16(1.0f, 1.0f);
+    d[10] = pk_bf16(1.0f, tc[0]);
+    d[11] = pk_bf16(tc[1], tc[0]);
+    d[12] = pk_bf16(tc[2], tc[0]);
+    d[13] = pk_bf16(tc[1], 0.0f);
+    d[14] = 0u;
+    d[15] = 0u;
+    store_words(d, out);
 }
 
 __device__ __forceinline__ nb_bf8 shfl_bf8(nb_bf8 v, int src) {
@@ -671,10 +691,9 @@ __device__ __forceinline__ void force_block(
             if constexpr (MF == 2) sh.sQ[MF == 2 ? buf : 0][MF == 2 ? tid : 0] = (float)(dx * dx + dy * dy + dz * dz);
             if constexpr (ROWS) {
                 if (rw > 0.0f) {
-                    const double wd = (double)rw;
                     const double q2 = (double)hx * hx + (double)hy * hy + (double)hz * hz;
-                    bf16_source_row((float)(wd * hx), (float)(wd * hy), (float)(wd * hz),
-                                    (float)(wd * q2), (float)wd, sh.sA[0][ROWS ? tid : 0]);
+                    bf16_source_row(rw * hx, rw * hy, rw * hz, (float)((double)rw * q2), rw,
+                                    sh.sA[0][ROWS ? tid : 0]);
                 } else {
                     bf16_massless_row(sh.sA[0][ROWS ? tid : 0]);
                 }

@@ -131,6 +131,7 @@ int ctx_run_force(nb_ctx *c, bool with_pot, bool allow_timing);
 int ctx_rebuild_central(nb_ctx *c);
 int ctx_gather_central(nb_ctx *c);
 void ctx_drop_graph(nb_ctx *c);
+int ctx_ensure_part(nb_ctx *c, int nseg);
 // nb_snapshot.hip
 void snapshot_free(nb_ctx *c);
 }  // namespace nb
