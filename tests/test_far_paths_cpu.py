@@ -77,7 +77,7 @@ def test_bf16_split_is_exact():
         assert float(h) + float(m) + float(l) == float(x)
 
 
-@pytest.mark.parametrize("reach", [2.0, 4.0])
+@pytest.mark.parametrize("reach", [1.5, 2.0, 4.0])
 def test_bf16_product_r2_within_budget(reach):
     """Far path 4: 27 exact bf16 products accumulated in FP32 give w r^2."""
     rng = np.random.default_rng(7)
