@@ -752,21 +752,28 @@ __device__ __forceinline__ void force_block(
                                            yil, zil, h2c, h2, hinv, hinv3, ax, ay, az, aq);
         } else if (ROWS && d2 > wg.farw_d2) {
             tile_mf3(sh.sj[buf], sh.sA[0], tid & 63, Bg, S3);
-            // Target 16g + c sits in lanes c, 16 + c, 32 + c, 48 + c (slot g);
-            // after the butterfly each of them holds the tile's sums and lane
-            // l keeps slot l >> 4, its own target.
-            const int lane = tid & 63;
-            float col[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            // Target 16g + c sits in lanes c, 16 + c, 32 + c, 48 + c (slot g)
+            // and lane l keeps slot l >> 4, its own target.  permlane32_swap
+            // of slots g and g + 2 followed by an add leaves lanes 0-31 with
+            // slot g and lanes 32-63 with slot g + 2, each summed over lane
+            // bit 5; permlane16_swap of the two results does the same for
+            // lane bit 4 and slot bit 0.
+            float col[4];
 #pragma unroll
-            for (int g = 0; g < 4; ++g) {
+            for (int c = 0; c < 4; ++c) {
+                float t1[2];
 #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    float t = S3[g][c];
-                    t += __shfl_xor(t, 16, 64);
-                    t += __shfl_xor(t, 32, 64);
-                    if (g == (lane >> 4)) col[c] = t;
-                    S3[g][c] = 0.0f;
+                for (int g0 = 0; g0 < 2; ++g0) {
+                    const auto r = __builtin_amdgcn_permlane32_swap(
+                        __builtin_bit_cast(unsigned, S3[g0][c]), __builtin_bit_cast(unsigned, S3[g0 + 2][c]),
+                        false, false);
+                    t1[g0] = __builtin_bit_cast(float, (unsigned)r[0]) + __builtin_bit_cast(float, (unsigned)r[1]);
                 }
+                const auto r = __builtin_amdgcn_permlane16_swap(__builtin_bit_cast(unsigned, t1[0]),
+                                                                __builtin_bit_cast(unsigned, t1[1]), false, false);
+                col[c] = __builtin_bit_cast(float, (unsigned)r[0]) + __builtin_bit_cast(float, (unsigned)r[1]);
+#pragma unroll
+                for (int g = 0; g < 4; ++g) S3[g][c] = 0.0f;
             }
             const double sm = (double)col[3];
             Ax[0] += (double)col[0] - (px[0] - ox) * sm;
