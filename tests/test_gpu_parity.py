@@ -235,6 +235,22 @@ def test_collision_far_tiles(pkg, orc, far):
     assert np.median(e) <= 2e-7 and np.percentile(e, 99) <= 2e-6
 
 
+@pytest.mark.parametrize("far", [3, 4])
+def test_collision_far_tiles_ragged(pkg, orc, far):
+    """As above with a last source tile that is only partly filled (8000
+    particles) and four source segments: padded rows carry zero weight."""
+    pos, vel, mass = pkg.ics.collision(4000, seeds=(5, 6))
+    assert len(mass) % 256 != 0
+    soft = [0.002] * 6
+    ref = orc.compute_force(pos, mass, soft)
+    with make_ctx(pkg, pos, vel, mass, soft, "split", tuning=(1, 4, 16, far)) as ctx:
+        ctx.compute_force()
+        acc = ctx.download(pos=False, vel=False)["acc"]
+    e = orc.rel_err(acc, ref)
+    print(f"far_path={far}: median {np.median(e):.3e} p99 {np.percentile(e, 99):.3e} max {e.max():.3e}")
+    assert np.median(e) <= 2e-7 and np.percentile(e, 99) <= 2e-6
+
+
 def test_morton_sort_preserves_order_and_values(pkg, orc):
     pos, vel, mass = pkg.ics.collision(3000, seeds=(3, 4))
     soft = [0.002] * 6
