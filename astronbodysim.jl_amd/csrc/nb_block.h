@@ -183,10 +183,14 @@ constexpr double MF2_REACH = 3.0;
 // (-2x'_i, -2y'_i, -2z'_i, |x'_i|^2), with the sum kept as
 //   S = sum_j f_ij (x'_j, y'_j, z'_j, 1),   a_i = S_xyz - x'_i S_m:
 // ten VALU and one v_rsq_f32 per pair against twelve and one.  The expansion
-// loses about 3 eps (|x'_i| + |x'_j|)^2 / r^2 <= 3 eps (1 + 2 / 1.5)^2 = 16 eps
-// relative in w r^2 for r >= 1.5 Re, since |x'_j| <= r + Re.  Massless sources
-// get w = 0 and w |x'|^2 = 2^120, so their f is exactly zero.
-constexpr double FARW_REACH = 1.5;
+// loses about 3 eps (|x'_i| + |x'_j|)^2 / r^2 <= 3 eps (1 + 2 / 0.5)^2 = 75 eps
+// relative in w r^2 for r >= 0.5 Re, since |x'_i| <= Re and |x'_j| <= r + Re;
+// sampled pairs at that reach stay below 41 eps (median 1 eps), and the
+// forces of the two-sphere benchmark keep a median error of 3.5e-8 against
+// FP64 (3.2e-8 with the tiles between 0.5 Re and 1.5 Re on the exact loop).
+// Tiles nearer than R/2 take the hi/lo loop anyway.  Massless sources get
+// w = 0 and w |x'|^2 = 2^120, so their f is exactly zero.
+constexpr double FARW_REACH = 0.5;
 
 // With FAR_MFMA_BF16 the squared distances of the tiles past FARW_REACH * Re
 // come from bf16 matrix cores (v_mfma_f32_16x16x32_bf16), which run alongside

@@ -1,6 +1,6 @@
 """CPU emulation of the far-tile squared-distance expansions used by the
 all-pairs kernel (far paths 3 and 4): the precision budget that lets tiles
-past 2 Re (Re = the wave's largest distance from the block origin) use
+past 0.5 Re (Re = the wave's largest distance from the block origin) use
 r^2 = |x_i|^2 + |x_j|^2 - 2 x_i . x_j.  The GPU parity tests check the
 kernels themselves; these check the arithmetic scheme, in numpy with FP32
 rounding, against the exact squared distance of the same FP32 inputs."""
@@ -77,9 +77,11 @@ def test_bf16_split_is_exact():
         assert float(h) + float(m) + float(l) == float(x)
 
 
-@pytest.mark.parametrize("reach", [1.5, 2.0, 4.0])
-def test_bf16_product_r2_within_budget(reach):
-    """Far path 4: 27 exact bf16 products accumulated in FP32 give w r^2."""
+@pytest.mark.parametrize("reach,budget", [(0.5, 48), (1.0, 16), (1.5, 12), (4.0, 12)])
+def test_bf16_product_r2_within_budget(reach, budget):
+    """Far path 4: 27 exact bf16 products accumulated in FP32 give w r^2,
+    within about 3 eps (1 + 2 / reach)^2 of the exact value (75 eps at the
+    kernel's reach of 0.5 Re; the sampled pairs stay near half of that)."""
     rng = np.random.default_rng(7)
     errs = []
     for xi, xj in samples(reach):
@@ -91,14 +93,15 @@ def test_bf16_product_r2_within_budget(reach):
         exact = float(w) * float(np.sum((xj.astype(np.float64) - xi.astype(np.float64)) ** 2))
         errs.append(abs(float(acc) - exact) / exact)
     errs = np.array(errs)
-    assert errs.max() < 12 * EPS32 and np.median(errs) < 2 * EPS32
+    assert errs.max() < budget * EPS32 and np.median(errs) < 2 * EPS32
 
 
-def test_weighted_expansion_within_budget():
+@pytest.mark.parametrize("reach,budget", [(0.5, 24), (2.0, 16)])
+def test_weighted_expansion_within_budget(reach, budget):
     """Far path 3: w r^2 from four FP32 FMAs, w = m^(-2/3)."""
     rng = np.random.default_rng(5)
     errs = []
-    for xi, xj in samples(2.0):
+    for xi, xj in samples(reach):
         m = rng.uniform(0.1, 10.0)
         w = np.float32(1.0 / np.cbrt(m * m))
         W = [np.float32(w * xj[0]), np.float32(w * xj[1]), np.float32(w * xj[2]),
@@ -109,4 +112,4 @@ def test_weighted_expansion_within_budget():
         r2w = fma(W[0], b[0], fma(W[1], b[1], fma(W[2], b[2], fma(w, ci, W[3]))))
         exact = float(w) * float(np.sum((xj.astype(np.float64) - xi.astype(np.float64)) ** 2))
         errs.append(abs(float(r2w) - exact) / exact)
-    assert max(errs) < 16 * EPS32
+    assert max(errs) < budget * EPS32
