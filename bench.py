@@ -296,10 +296,21 @@ def main():
         e = orc.rel_err(acc0[sample], ref)
         err = {"median": float(np.median(e)), "p99": float(np.percentile(e, 99)),
                "max": float(np.max(e)), "targets": int(k)}
+        # the faithful variant (per-target scratch array + pairwise sum, as
+        # src/directsum/gravity.jl:40-45) on a smaller sample of the same targets
+        progress("CPU oracle sample (faithful variant)")
+        kf = int(min(k, max(32, k // 8)))
+        t1 = time.perf_counter()
+        ref_f = orc.force_at_points(pos[sample[:kf]], h, pos, mass, variant="faithful")
+        faithful_s = time.perf_counter() - t1
         cpu = {"value": float(k) * n / cpu_s, "unit": "pair interactions/s", "cores": threads,
                "kind": "port",
                "sample": f"{k} seeded targets x all {n} sources of the initial state, FP64 "
-                         f"C restatement of src/directsum (SIMD variant, OpenMP {threads} threads)"}
+                         f"C restatement of src/directsum (SIMD variant, OpenMP {threads} threads)",
+               "variants": {"simd": float(k) * n / cpu_s,
+                            "faithful": float(kf) * n / faithful_s,
+                            "faithful_sample": f"{kf} of the same targets",
+                            "faithful_vs_simd_max_rel": float(np.max(orc.rel_err(ref_f, ref[:kf])))}}
 
     batch = None
     if not args.no_batch:
