@@ -357,6 +357,12 @@ __device__ __forceinline__ unsigned pk_bf16(float a, float b) {
     return __builtin_bit_cast(unsigned, __builtin_convertvector(v, nb_bf2));
 }
 
+// Two FP32 values that are exactly bf16 (low 16 bits zero) packed into one
+// dword by a byte permute: the same bits as pk_bf16, without the conversion.
+__device__ __forceinline__ unsigned pk_exact(float a, float b) {
+    return __builtin_amdgcn_perm(__builtin_bit_cast(unsigned, b), __builtin_bit_cast(unsigned, a), 0x07060302u);
+}
+
 // The three-bf16 splits of two values at once, as FP32 (each exactly a bf16).
 __device__ __forceinline__ void split2(float x0, float x1, float (&s0)[3], float (&s1)[3]) {
     const nb_f2v v = {x0, x1};
@@ -387,11 +393,11 @@ __device__ __forceinline__ void bf16_source_row(float wx, float wy, float wz, fl
     split2(wz, wq, sz, sq);
     split2(w, 0.0f, sw, s0);
     unsigned d[16];
-    d[0] = pk_bf16(sx[0], sx[0]); d[1] = pk_bf16(sx[1], sx[0]); d[2] = pk_bf16(sx[2], sx[1]);
-    d[3] = pk_bf16(sy[0], sy[0]); d[4] = pk_bf16(sy[1], sy[0]); d[5] = pk_bf16(sy[2], sy[1]);
-    d[6] = pk_bf16(sz[0], sz[0]); d[7] = pk_bf16(sz[1], sz[0]); d[8] = pk_bf16(sz[2], sz[1]);
-    d[9] = pk_bf16(sq[0], sq[1]); d[10] = pk_bf16(sq[2], sw[0]); d[11] = pk_bf16(sw[0], sw[1]);
-    d[12] = pk_bf16(sw[0], sw[2]); d[13] = pk_bf16(sw[1], 0.0f); d[14] = 0u; d[15] = 0u;
+    d[0] = pk_exact(sx[0], sx[0]); d[1] = pk_exact(sx[1], sx[0]); d[2] = pk_exact(sx[2], sx[1]);
+    d[3] = pk_exact(sy[0], sy[0]); d[4] = pk_exact(sy[1], sy[0]); d[5] = pk_exact(sy[2], sy[1]);
+    d[6] = pk_exact(sz[0], sz[0]); d[7] = pk_exact(sz[1], sz[0]); d[8] = pk_exact(sz[2], sz[1]);
+    d[9] = pk_exact(sq[0], sq[1]); d[10] = pk_exact(sq[2], sw[0]); d[11] = pk_exact(sw[0], sw[1]);
+    d[12] = pk_exact(sw[0], sw[2]); d[13] = pk_exact(sw[1], 0.0f); d[14] = 0u; d[15] = 0u;
     store_words(d, out);
 }
 
@@ -442,23 +448,26 @@ __device__ __forceinline__ nb_bf8 shfl_bf8(nb_bf8 v, int src) {
 // lane l the w r^2 of targets 16g + (l & 15) (the B columns) against sources
 // jc + 4(l >> 4) + v in element v (the A rows); the lane then accumulates
 // S[g] for its four targets over its four sources.
-__device__ __forceinline__ void tile_mf3(const float4 *__restrict__ s, const nb_bf8 (*__restrict__ sa)[4],
-                                         int lane, const nb_bf8 (&Bg)[4], float (&S)[4][4]) {
-    const int c16 = lane & 15, grp = lane >> 4;
+template <bool FIRST>
+__device__ __forceinline__ void mf3_chunk(const float4 *__restrict__ s, const nb_bf8 a, int grp,
+                                          const nb_bf8 (&Bg)[4], float (&S)[4][4]) {
     const nb_f4v zero = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll 1
-    for (int jc = 0; jc < BS; jc += 16) {
-        const nb_bf8 a = sa[jc + c16][grp];
-        nb_f4v D[4];
+    nb_f4v D[4];
 #pragma unroll
-        for (int g = 0; g < 4; ++g) D[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, Bg[g], zero, 0, 0, 0);
+    for (int g = 0; g < 4; ++g) D[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, Bg[g], zero, 0, 0, 0);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const float4 p = s[jc + 4 * grp + v];
+    for (int v = 0; v < 4; ++v) {
+        const float4 p = s[4 * grp + v];
 #pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                const float q = __builtin_amdgcn_rsqf(D[g][v]);
-                const float f = (q * q) * q;
+        for (int g = 0; g < 4; ++g) {
+            const float q = __builtin_amdgcn_rsqf(D[g][v]);
+            const float f = (q * q) * q;
+            if (FIRST && v == 0) {
+                S[g][0] = f * p.x;
+                S[g][1] = f * p.y;
+                S[g][2] = f * p.z;
+                S[g][3] = f;
+            } else {
                 S[g][0] = __builtin_fmaf(f, p.x, S[g][0]);
                 S[g][1] = __builtin_fmaf(f, p.y, S[g][1]);
                 S[g][2] = __builtin_fmaf(f, p.z, S[g][2]);
@@ -466,6 +475,45 @@ __device__ __forceinline__ void tile_mf3(const float4 *__restrict__ s, const nb_
             }
         }
     }
+}
+
+// A whole tile; S is written (not accumulated into), so it lives only
+// within the tile.
+__device__ __forceinline__ void tile_mf3(const float4 *__restrict__ s, const nb_bf8 (*__restrict__ sa)[4],
+                                         int lane, const nb_bf8 (&Bg)[4], float (&S)[4][4]) {
+    const int c16 = lane & 15, grp = lane >> 4;
+    mf3_chunk<true>(s, sa[c16][grp], grp, Bg, S);
+#pragma unroll 1
+    for (int jc = 16; jc < BS; jc += 16) mf3_chunk<false>(s + jc, sa[jc + c16][grp], grp, Bg, S);
+}
+
+// FAR_MFMA_BF16 fold: target 16g + c sits in lanes c, 16 + c, 32 + c, 48 + c
+// (slot g) and lane l keeps slot l >> 4, its own target.  permlane32_swap of
+// slots g and g + 2 followed by an add leaves lanes 0-31 with slot g and lanes
+// 32-63 with slot g + 2, each summed over lane bit 5; permlane16_swap of the
+// two results does the same for lane bit 4 and slot bit 0.  The lane's four
+// sums (S_xyz, S_m) go to FP64 as S_xyz - x'_i S_m.
+__device__ __forceinline__ void fold_mf3(const float (&S)[4][4], double xd, double yd, double zd,
+                                         double &Ax, double &Ay, double &Az) {
+    float col[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        float t1[2];
+#pragma unroll
+        for (int g0 = 0; g0 < 2; ++g0) {
+            const auto r = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(unsigned, S[g0][c]),
+                                                            __builtin_bit_cast(unsigned, S[g0 + 2][c]), false,
+                                                            false);
+            t1[g0] = __builtin_bit_cast(float, (unsigned)r[0]) + __builtin_bit_cast(float, (unsigned)r[1]);
+        }
+        const auto r = __builtin_amdgcn_permlane16_swap(__builtin_bit_cast(unsigned, t1[0]),
+                                                        __builtin_bit_cast(unsigned, t1[1]), false, false);
+        col[c] = __builtin_bit_cast(float, (unsigned)r[0]) + __builtin_bit_cast(float, (unsigned)r[1]);
+    }
+    const double sm = (double)col[3];
+    Ax += (double)col[0] - xd * sm;
+    Ay += (double)col[1] - yd * sm;
+    Az += (double)col[2] - zd * sm;
 }
 
 // Per-wave tile classes.  Each wave uses the box of its own targets, their
@@ -608,10 +656,10 @@ __device__ __forceinline__ void force_block(
     const double soft_d2 = wg.soft_d2, close_d2 = wg.close_d2;
     const double close_d = sqrt(close_d2);
     // FAR_MFMA_BF16: lane l holds, for target group g, the K slice (l >> 4) of
-    // the B column of target 16g + (l & 15); S3 accumulates those targets.
+    // the B column of target 16g + (l & 15).
     constexpr bool ROWS = BlockShared<T, MF>::ROWS;
     nb_bf8 Bg[4];
-    float S3[4][4];
+    uint64_t m_soft = 0, m_close = 0, m_far = 0;   // classes of tiles 64k..64k+63
     if constexpr (ROWS) {
         const int lane = tid & 63;
         const double dx = px[0] - ox, dy = py[0] - oy, dz = pz[0] - oz;
@@ -625,10 +673,6 @@ __device__ __forceinline__ void force_block(
                 if (sl == (lane >> 4)) Bg[g] = t;
             }
         }
-#pragma unroll
-        for (int g = 0; g < 4; ++g)
-#pragma unroll
-            for (int c = 0; c < 4; ++c) S3[g][c] = 0.0f;
     }
     // weighted far tiles: per-target operands and S-form sums
     constexpr bool WFAR = BlockShared<T, MF>::WFAR && !POT;
@@ -743,47 +787,38 @@ __device__ __forceinline__ void force_block(
                 if (WEIGHTS) rw = sw ? sw[j] : (rm > 0 ? (float)(1.0 / cbrt((double)rm * (double)rm)) : 0.0f);
             }
         }
-        const double d2 = tile_gap2(tile_box + ((base - j0) / BS) * 8, wg);
-        if (d2 <= soft_d2) {
+        // Tile classes of the next 64 tiles, one per lane, as wave-wide bit
+        // masks (the per-tile test is then scalar).
+        if ((it & 63) == 0) {
+            const int64_t t = it + (tid & 63);
+            bool c_soft = false, c_close = false, c_far = false;
+            if (j0 + t * BS < j1) {
+                const double g2 = tile_gap2(tile_box + t * 8, wg);
+                c_soft = g2 <= soft_d2;
+                c_close = !c_soft && g2 <= close_d2;
+                c_far = g2 > wg.farw_d2;
+            }
+            m_soft = __builtin_amdgcn_ballot_w64(c_soft);
+            m_close = __builtin_amdgcn_ballot_w64(c_close);
+            m_far = __builtin_amdgcn_ballot_w64(c_far);
+        }
+        const uint64_t tbit = 1ull << (it & 63);
+        if (m_soft & tbit) {
             if (HILO)
                 tile_f32_hilo<TPT, true, POT>(sh.sj[buf], sh.sjl[HILO ? buf : 0], xi, yi, zi, xil,
                                               yil, zil, h2c, h2, hinv, hinv3, ax, ay, az, aq);
             else
                 tile_f32<TPT, true, POT>(sh.sj[buf], xi, yi, zi, h2c, h2, hinv, hinv3, ax, ay, az,
                                          aq);
-        } else if (HILO && d2 <= close_d2) {
+        } else if (HILO && (m_close & tbit)) {
             tile_f32_hilo<TPT, false, POT>(sh.sj[buf], sh.sjl[HILO ? buf : 0], xi, yi, zi, xil,
                                            yil, zil, h2c, h2, hinv, hinv3, ax, ay, az, aq);
-        } else if (ROWS && d2 > wg.farw_d2) {
+        } else if (ROWS && (m_far & tbit)) {
+            float S3[4][4];
             tile_mf3(sh.sj[buf], sh.sA[0], tid & 63, Bg, S3);
-            // Target 16g + c sits in lanes c, 16 + c, 32 + c, 48 + c (slot g)
-            // and lane l keeps slot l >> 4, its own target.  permlane32_swap
-            // of slots g and g + 2 followed by an add leaves lanes 0-31 with
-            // slot g and lanes 32-63 with slot g + 2, each summed over lane
-            // bit 5; permlane16_swap of the two results does the same for
-            // lane bit 4 and slot bit 0.
-            float col[4];
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                float t1[2];
-#pragma unroll
-                for (int g0 = 0; g0 < 2; ++g0) {
-                    const auto r = __builtin_amdgcn_permlane32_swap(
-                        __builtin_bit_cast(unsigned, S3[g0][c]), __builtin_bit_cast(unsigned, S3[g0 + 2][c]),
-                        false, false);
-                    t1[g0] = __builtin_bit_cast(float, (unsigned)r[0]) + __builtin_bit_cast(float, (unsigned)r[1]);
-                }
-                const auto r = __builtin_amdgcn_permlane16_swap(__builtin_bit_cast(unsigned, t1[0]),
-                                                                __builtin_bit_cast(unsigned, t1[1]), false, false);
-                col[c] = __builtin_bit_cast(float, (unsigned)r[0]) + __builtin_bit_cast(float, (unsigned)r[1]);
-#pragma unroll
-                for (int g = 0; g < 4; ++g) S3[g][c] = 0.0f;
-            }
-            const double sm = (double)col[3];
-            Ax[0] += (double)col[0] - (px[0] - ox) * sm;
-            Ay[0] += (double)col[1] - (py[0] - oy) * sm;
-            Az[0] += (double)col[2] - (pz[0] - oz) * sm;
-        } else if (WFAR && d2 > wg.farw_d2) {
+            fold_mf3(S3, px[0] - ox, py[0] - oy, pz[0] - oz, Ax[0], Ay[0], Az[0]);
+            continue;   // no FP32 VALU partials to flush
+        } else if (WFAR && (m_far & tbit)) {
             tile_w<TPT>(sh.sj[buf], sh.sW[WFAR ? buf : 0], sh.sw[WFAR ? buf : 0], wbx, wby, wbz, wci,
                         Sx, Sy, Sz, Sm);
 #pragma unroll
@@ -795,7 +830,7 @@ __device__ __forceinline__ void force_block(
                 Sx[k] = Sy[k] = Sz[k] = Sm[k] = 0.0f;
             }
         } else if constexpr (MF == 1 || MF == 2) {
-            if (MF == 2 && d2 > mf2_d2)
+            if (MF == 2 && tile_gap2(tile_box + ((base - j0) / BS) * 8, wg) > mf2_d2)
                 tile_mfma2(sh.sj[buf], &sh.sQ[MF == 2 ? buf : 0][0], &sh.sB[buf][0][0], tid & 3, b2x,
                            b2y, b2z, ci, S);
             else
