@@ -629,7 +629,8 @@ __device__ __forceinline__ void force_block(
     }
     BlockStats st;
     block_stats<TPT>(px, py, pz, ph, valid, use_origin, sh.red, st);
-    const double ox = st.ox, oy = st.oy, oz = st.oz;
+    // block-uniform: kept in scalar registers
+    const double ox = uniform_f64(st.ox), oy = uniform_f64(st.oy), oz = uniform_f64(st.oz);
 
     float xi[TPT], yi[TPT], zi[TPT], xil[TPT], yil[TPT], zil[TPT];
     float h2c[TPT], h2[TPT], hinv[TPT], hinv3[TPT];
