@@ -251,6 +251,23 @@ def test_collision_far_tiles_ragged(pkg, orc, far):
     assert np.median(e) <= 2e-7 and np.percentile(e, 99) <= 2e-6
 
 
+@pytest.mark.parametrize("far", [0, 4])
+def test_segment_longer_than_one_class_group(pkg, orc, far):
+    """One source segment of 79 tiles: the kernel classifies tiles 64 at a
+    time (one lane each), so this covers a full group of 64 and a partial
+    second group of 15."""
+    pos, vel, mass = pkg.ics.collision(10000, seeds=(7, 8))
+    assert (len(mass) + 255) // 256 == 79
+    soft = [0.002] * 6
+    ref = orc.compute_force(pos, mass, soft)
+    with make_ctx(pkg, pos, vel, mass, soft, "split", tuning=(1, 1, 16, far)) as ctx:
+        ctx.compute_force()
+        acc = ctx.download(pos=False, vel=False)["acc"]
+    e = orc.rel_err(acc, ref)
+    print(f"far_path={far}: median {np.median(e):.3e} p99 {np.percentile(e, 99):.3e} max {e.max():.3e}")
+    assert np.median(e) <= 2e-7 and np.percentile(e, 99) <= 2e-6
+
+
 def test_morton_sort_preserves_order_and_values(pkg, orc):
     pos, vel, mass = pkg.ics.collision(3000, seeds=(3, 4))
     soft = [0.002] * 6
