@@ -267,6 +267,14 @@ def main():
     nforce, force_ms = ctx.get_timing()
     ctx.set_timing(False)
     tuning = ctx.get_tuning()
+    # the same force evaluation with the boundary's host copies: upload of
+    # positions, velocities and masses, the force, download of the accelerations
+    ctx.sync()
+    t1 = time.perf_counter()
+    ctx.upload(pos, vel, mass)
+    ctx.compute_force()
+    _ = ctx.download(pos=False, vel=False, acc=True)["acc"]
+    pcie_s = time.perf_counter() - t1
     ctx.close()
 
     pairs_per_step = float(n) * float(n)
@@ -335,6 +343,9 @@ def main():
                    "n": n, "softening": h, "dt": dt, "precision": "split",
                    "tuning": tuning},
         "force_rel_err": err,
+        # one force evaluation including the host-buffer boundary (PCIe): not `value`
+        "host_inclusive": {"pairs_per_s": float(n) * float(n) / pcie_s, "s": pcie_s,
+                           "what": "nb_upload (pos, vel, mass) + nb_compute_force + nb_download (acc)"},
         # compute-bound: priced against MI355X's FP32 peak (157.3 TFLOP/s, the
         # figure shared by FP32 MFMA and packed FP32 vector issue); the pair
         # loop issues unpacked FP32 VALU + v_rsq_f32, and with far path 4 the
