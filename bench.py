@@ -265,6 +265,14 @@ def main():
     ctx.sync()
     elapsed = time.perf_counter() - t0
     nforce, force_ms = ctx.get_timing()
+    # five separate force evaluations, each timed with HIP events: median
+    singles = []
+    for _ in range(5):
+        c0, m0 = ctx.get_timing()
+        ctx.compute_force()
+        ctx.sync()
+        c1, m1 = ctx.get_timing()
+        singles.append((m1 - m0) / max(c1 - c0, 1))
     ctx.set_timing(False)
     tuning = ctx.get_tuning()
     # the same force evaluation with the boundary's host copies: upload of
@@ -359,7 +367,8 @@ def main():
                      "kernel": "k_force_f32<double,%d,false,%s>" % (
                          tuning["targets_per_thread"], tuning["far_path"]),
                      "flops_per_pair": FLOPS_PER_PAIR,
-                     "avg_kernel_ms": avg_force_s * 1e3, "launches": nforce},
+                     "avg_kernel_ms": avg_force_s * 1e3, "launches": nforce,
+                     "median_single_force_ms": float(np.median(singles)), "single_forces": len(singles)},
         "cpu_baseline": cpu,
         "batched": batch,
         "device": {"name": dev["name"], "arch": dev["arch"], "cu": dev["compute_units"]},
