@@ -376,8 +376,18 @@ int nb_adaptive_compute_force(nb_ctx *c, int64_t t_int, int64_t *nactive) {
         a.nseg = nseg; a.seg_len = tiles_per_seg * BS; a.nblk_i = (int)nblk; a.G = c->G;
         a.ax = gax; a.ay = gay; a.az = gaz; a.pot = nullptr; a.part = c->part;
         a.skip_src = (c->mode == NB_MODE_SPLIT_CENTRAL && c->ncentral > 0) ? c->central : nullptr;
-        if (c->mode == NB_MODE_FP64) NB_TRY(launch_force_f64(a, false, s));
-        else NB_TRY(launch_force_split(a, 1, false, s));
+        if (c->mode == NB_MODE_FP64) {
+            NB_TRY(launch_force_f64(a, false, s));
+        } else {
+            // the context's far path (one target per lane here), with the
+            // source weights that the weighted and matrix-core paths read
+            const int fp = c->tun.far_path;
+            if (far_path_weighted(fp)) {
+                NB_TRY(launch_source_weights(c->m, true, n, c->wsrc, s));
+                a.sw = c->wsrc;
+            }
+            NB_TRY(launch_force_split(a, 1, false, s, fp));
+        }
         if (nseg > 1) NB_TRY(launch_finalize(c->part, nseg, na, c->G, gax, gay, gaz, nullptr, false, s));
         if (c->mode == NB_MODE_SPLIT_CENTRAL && c->ncentral > 0) {
             // FP64 central terms for the active targets

@@ -268,6 +268,36 @@ def test_segment_longer_than_one_class_group(pkg, orc, far):
     assert np.median(e) <= 2e-7 and np.percentile(e, 99) <= 2e-6
 
 
+@pytest.mark.parametrize("far", [0, 4])
+def test_active_subset_force_far_paths(pkg, orc, far):
+    """Adaptive steps: the active-subset force (every third particle active)
+    takes the context's far path; inactive particles keep their values
+    (src/directsum/gravity.jl:94-98)."""
+    import ctypes as C
+    call = pkg._native.call
+    pos, vel, mass = pkg.ics.collision(10000, seeds=(7, 8))
+    n = len(mass)
+    soft = [0.002] * 6
+    ref = orc.compute_force(pos, mass, soft)
+    active = np.arange(n) % 3 == 0
+    with make_ctx(pkg, pos, vel, mass, soft, "split", tuning=(1, 4, 0, far)) as ctx:
+        ctx.compute_force()
+        call("nb_adaptive_init", ctx.handle, 1e-3, 1 << 10, 1e-2, 0.0, 1.0)
+        ti_end = np.where(active, 64, 128).astype(np.int64)
+        ti_beg = np.zeros(n, dtype=np.int64)
+        call("nb_adaptive_set_steps", ctx.handle, ti_end.ctypes.data_as(C.POINTER(C.c_int64)),
+             ti_beg.ctypes.data_as(C.POINTER(C.c_int64)))
+        ctx.upload_acc(np.full((n, 3), 7.0))
+        na = C.c_int64()
+        call("nb_adaptive_compute_force", ctx.handle, 64, C.byref(na))
+        acc = ctx.download(pos=False, vel=False)["acc"]
+    assert na.value == int(active.sum())
+    assert np.all(acc[~active] == 7.0)
+    e = orc.rel_err(acc[active], ref[active])
+    print(f"far_path={far}: median {np.median(e):.3e} p99 {np.percentile(e, 99):.3e} max {e.max():.3e}")
+    assert np.median(e) <= 2e-7 and np.percentile(e, 99) <= 2e-6
+
+
 def test_morton_sort_preserves_order_and_values(pkg, orc):
     pos, vel, mass = pkg.ics.collision(3000, seeds=(3, 4))
     soft = [0.002] * 6
