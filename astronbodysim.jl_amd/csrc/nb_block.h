@@ -519,9 +519,9 @@ __device__ __forceinline__ void fold_mf3(const float (&S)[4][4], double xd, doub
 // Per-wave tile classes.  Each wave uses the box of its own targets, their
 // largest h, their largest distance R from the block origin per axis and
 // their largest Euclidean distance Re from it.  A margin keeps every far pair
-// at FP32-resolvable, nonzero separation.  The main kernel and the matrix-core
-// far-tile kernel both decide with these two functions, evaluated without FMA
-// contraction, so that each (wave, tile) pair is summed by exactly one of them.
+// at FP32-resolvable, nonzero separation.  The two functions are evaluated
+// without FMA contraction, so the classes do not depend on how the compiler
+// schedules them (one lane per tile, 64 tiles at a time, in force_block).
 struct WaveGeom {
     double lo[3], hi[3];
     double soft_d2, close_d2, farw_d2;
