@@ -11,7 +11,10 @@ property that holds at any size:
   time reversal of the KDK step (2 steps forward, velocities flipped, 2
   steps back: positions within 1e-9, velocities within 1e-6, absolute);
 * Plummer sphere, N = 65536, FP32 state (the FP32 configuration): a sample of
-  2048 targets against the oracle on the FP32-rounded inputs (median <= 1e-6).
+  2048 targets against the oracle on the FP32-rounded inputs (median <= 1e-6);
+* the two ensembles at their full sizes: eight sampled clusters of the
+  512-cluster ensemble, and all 2048 planetary systems (the oracle sums each
+  73-body system whole).
 """
 import numpy as np
 import pytest
@@ -100,3 +103,51 @@ def test_plummer_full_size_fp32_sampled_parity(pkg, orc):
     e = orc.rel_err(acc[sample], ref)
     print(f"N=65536 fp32: median {np.median(e):.3e} p99 {np.percentile(e, 99):.3e} max {e.max():.3e}")
     assert np.median(e) <= 1e-6
+
+
+def test_cluster_ensemble_full_size_sampled_parity(pkg, orc):
+    """512 clusters (N log-uniform in [256, 16384], about 2e6 particles) in
+    one ragged launch: eight seeded clusters against the oracle, each with
+    its own softening (median <= 1e-6, max <= 1e-4)."""
+    ens = pkg.ics.cluster_ensemble(nsys=512, seed=5)
+    off = ens["offsets"]
+    e = pkg.Ensemble(off, "split")
+    try:
+        e.set_params(1.0, ens["h"], 1e-3)
+        e.upload(ens["pos"], ens["vel"], ens["mass"])
+        e.compute_force()
+        acc = e.download(pos=False, vel=False)["acc"]
+    finally:
+        e.close()
+    rng = np.random.default_rng(23)
+    errs = []
+    for s in np.sort(rng.choice(512, 8, replace=False)):
+        sl = slice(off[s], off[s + 1])
+        ref = orc.compute_force(ens["pos"][sl], ens["mass"][sl], [ens["h"][s]] * 6)
+        errs.append(orc.rel_err(acc[sl], ref))
+    err = np.concatenate(errs)
+    print(f"clusters: median {np.median(err):.3e} max {err.max():.3e} over {len(err)} particles")
+    assert np.median(err) <= 1e-6 and err.max() <= 1e-4
+
+
+def test_planetary_ensemble_full_size_parity(pkg, orc):
+    """2048 planetary systems x 73 bodies, eps = 0, central star in FP64:
+    every system against the oracle (self pair skipped)."""
+    ens = pkg.ics.planetary_ensemble(nsys=2048, seed=6)
+    off = ens["offsets"]
+    e = pkg.Ensemble(off, "split_central")
+    try:
+        e.set_params(1.0, 0.0, 1e-3)
+        e.upload(ens["pos"], ens["vel"], ens["mass"])
+        e.set_central(1)
+        e.compute_force()
+        acc = e.download(pos=False, vel=False)["acc"]
+    finally:
+        e.close()
+    ref = np.empty_like(acc)
+    for s in range(2048):
+        sl = slice(off[s], off[s + 1])
+        ref[sl] = orc.compute_force(ens["pos"][sl], ens["mass"][sl], [0.0] * 6, skip_self=True)
+    err = orc.rel_err(acc, ref)
+    print(f"planetary: median {np.median(err):.3e} p99 {np.percentile(err, 99):.3e} max {err.max():.3e}")
+    assert np.median(err) <= 1e-9 and err.max() <= 1e-5
