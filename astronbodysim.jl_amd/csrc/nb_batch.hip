@@ -101,8 +101,14 @@ struct BatchArgs {
 // Persistent ragged all-pairs kernel: blocks pull (system, target tile) items
 // from an atomic queue until it is empty; the last block to finish resets the
 // queue for the next launch (so the launch can be replayed from a graph).
+// The bf16 matrix-core far path is held to 4 waves per SIMD, as in the
+// one-system kernel: at the 2 waves its 169 VGPRs would give, an ensemble of
+// 8 Morton-sorted 131072-particle Plummer spheres took 48.4 ms per force;
+// at 4 waves (a few values spilled outside the pair loops) 43.2 ms, against
+// 44.3 ms for far path 0.
 template <int TPT, bool POT, int MF = 0>
-__global__ __launch_bounds__(BS) void k_batch_force(BatchArgs a) {
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MF == 4 ? 4 : 1))) void k_batch_force(
+    BatchArgs a) {
     __shared__ BlockShared<double, MF> sh;
     __shared__ int64_t item_sh;
     const int tid = threadIdx.x;
