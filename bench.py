@@ -40,7 +40,10 @@ def parse():
                    help="target duration of the CPU-oracle sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-batch", action="store_true")
-    p.add_argument("--no-tune", action="store_true")
+    p.add_argument("--tune", action="store_true",
+                   help="autotune on this device instead of using the package's tuning table")
+    p.add_argument("--no-tune", action="store_true",
+                   help="use the package's tuning table (the default)")
     p.add_argument("--save-tuning", default=None, help="write the autotuner's table here")
     p.add_argument("--tuning-file", default=None, help="load a tuning table before creating contexts")
     return p.parse_args()
@@ -241,7 +244,7 @@ def main():
     ctx = nat.Context(n, nat.NB_MODE_SPLIT)
     ctx.set_params(1.0, [h] * 6)
     ctx.upload(pos, vel, mass)
-    if not args.no_tune:
+    if args.tune and not args.no_tune:
         progress("autotuning the all-pairs launch configuration")
         ctx.autotune()
         if args.save_tuning:
