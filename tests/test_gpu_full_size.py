@@ -9,7 +9,8 @@ property that holds at any size:
   particle), momentum balance sum_i m_i a_i ~ 0 (equal softening, so the
   pair forces are antisymmetric; relative to sum_i m_i |a_i|, <= 1e-6), and
   time reversal of the KDK step (2 steps forward, velocities flipped, 2
-  steps back: positions within 1e-9, velocities within 1e-6, absolute);
+  steps back: positions within 1e-9, velocities within 1e-6, absolute), and
+  the potential of 512 sampled targets (median <= 1e-6);
 * Plummer sphere, N = 65536, FP32 state (the FP32 configuration): a sample of
   2048 targets against the oracle on the FP32-rounded inputs (median <= 1e-6);
 * the two ensembles at their full sizes: eight sampled clusters of the
@@ -151,3 +152,22 @@ def test_planetary_ensemble_full_size_parity(pkg, orc):
     err = orc.rel_err(acc, ref)
     print(f"planetary: median {np.median(err):.3e} p99 {np.percentile(err, 99):.3e} max {err.max():.3e}")
     assert np.median(err) <= 1e-9 and err.max() <= 1e-5
+
+
+def test_collision_full_size_sampled_potential(pkg, orc, c3):
+    """The potential of the N = 2^20 collision (FP32 pairs, FP64 sums): 512
+    sampled targets against the oracle (median <= 1e-6, max <= 1e-4)."""
+    pos, vel, mass = c3
+    n, h = len(mass), 0.002
+    ctx = _ctx(pkg, pos, vel, mass, h, "split")
+    try:
+        ctx.compute_potential(False)
+        pot = ctx.download(pos=False, vel=False, acc=False, pot=True)["pot"]
+    finally:
+        ctx.close()
+    rng = np.random.default_rng(24)
+    sample = np.sort(rng.choice(n, 512, replace=False))
+    ref = orc.potential_at_points(pos[sample], h, pos, mass)
+    e = np.abs(pot[sample] - ref) / np.abs(ref)
+    print(f"N=2^20 potential: median {np.median(e):.3e} max {e.max():.3e}")
+    assert np.median(e) <= 1e-6 and e.max() <= 1e-4
