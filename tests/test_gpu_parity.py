@@ -298,6 +298,26 @@ def test_active_subset_force_far_paths(pkg, orc, far):
     assert np.median(e) <= 2e-7 and np.percentile(e, 99) <= 2e-6
 
 
+@pytest.mark.parametrize("far", [0, 4])
+def test_far_outlier_particle(pkg, orc, far):
+    """Two spheres plus one particle 1e5 away: the outlier's Morton
+    neighbours share a block with it (a huge origin offset and box), which
+    the tile classes must keep on the hi/lo loops."""
+    pos, vel, mass = pkg.ics.collision(6000, seeds=(9, 10))
+    pos = np.vstack([pos, [[1.0e5, 3.0, -2.0]]])
+    vel = np.vstack([vel, [[0.0, 0.0, 0.0]]])
+    mass = np.append(mass, 1e-4)
+    soft = [0.002] * 6
+    ref = orc.compute_force(pos, mass, soft)
+    with make_ctx(pkg, pos, vel, mass, soft, "split", tuning=(1, 4, 16, far)) as ctx:
+        ctx.compute_force()
+        acc = ctx.download(pos=False, vel=False)["acc"]
+    e = orc.rel_err(acc, ref)
+    print(f"far_path={far}: median {np.median(e):.3e} p99 {np.percentile(e, 99):.3e} "
+          f"max {e.max():.3e} outlier {e[-1]:.3e}")
+    assert np.median(e) <= 2e-7 and np.percentile(e, 99) <= 2e-6 and e[-1] <= 1e-6
+
+
 def test_morton_sort_preserves_order_and_values(pkg, orc):
     pos, vel, mass = pkg.ics.collision(3000, seeds=(3, 4))
     soft = [0.002] * 6
