@@ -181,7 +181,10 @@ int maybe_sort(nb_ctx *c, int64_t upcoming) {
     return NB_OK;
 }
 
-int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = true) {
+// pot_only: the FP32-pair kernels write the potential alone (the caller keeps
+// the accelerations; the FP64 kernels and passes still write both).
+int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = true,
+              bool pot_only = false) {
     hipStream_t s = c->stream;
     const int64_t n = c->n;
     if (n == 0) return NB_OK;
@@ -209,12 +212,12 @@ int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = tru
         a.part = c->part;
         a.skip_src = (c->mode == NB_MODE_SPLIT_CENTRAL && c->ncentral > 0) ? c->central : nullptr;
         a.accumulate = 0;
-        if (!f64 && !with_pot && far_path_weighted(c->tun.far_path)) {
+        if (!f64 && (!with_pot || pot_only) && far_path_weighted(c->tun.far_path)) {
             NB_TRY(launch_source_weights(c->m, true, n, c->wsrc, s));
             a.sw = c->wsrc;
         }
         if (f64) NB_TRY(launch_force_f64(a, with_pot, s));
-        else NB_TRY(launch_force_split(a, tpt, with_pot, s, c->tun.far_path));
+        else NB_TRY(launch_force_split(a, tpt, with_pot, s, c->tun.far_path, !(with_pot && pot_only)));
     } else {
         ForceArgs<float> a{};
         a.tx = (const float *)c->x; a.ty = (const float *)c->y; a.tz = (const float *)c->z;
@@ -229,11 +232,12 @@ int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = tru
         a.part = c->part;
         a.skip_src = nullptr;
         a.accumulate = 0;
-        NB_TRY(launch_force_f32(a, tpt, with_pot, s));
+        NB_TRY(launch_force_f32(a, tpt, with_pot, s, !(with_pot && pot_only)));
     }
     if (allow_timing) NB_TRY(record_stop(c, e0));
     if (g.nseg > 1)
-        NB_TRY(launch_finalize(c->part, g.nseg, n, c->G, c->ax, c->ay, c->az, c->pot, with_pot, s));
+        NB_TRY(launch_finalize(c->part, g.nseg, n, c->G, c->ax, c->ay, c->az, c->pot, with_pot, s,
+                               !(with_pot && pot_only) || f64));
     // FP64 central-body terms: every target, sources = the central particles.
     if (c->mode == NB_MODE_SPLIT_CENTRAL && c->ncentral > 0) {
         const int64_t k = c->ncentral;
@@ -576,7 +580,7 @@ int nb_compute_potential(nb_ctx *c, int32_t add_self_term) {
         NB_HIP(hipMemcpyAsync(c->stage + c->n, c->ay, nb3, hipMemcpyDeviceToDevice, s));
         NB_HIP(hipMemcpyAsync(c->stage + 2 * c->n, c->az, nb3, hipMemcpyDeviceToDevice, s));
     }
-    NB_TRY(run_force(c, true, false, false));
+    NB_TRY(run_force(c, true, false, false, true));
     if (c->n > 0) {
         NB_HIP(hipMemcpyAsync(c->ax, c->stage, nb3, hipMemcpyDeviceToDevice, s));
         NB_HIP(hipMemcpyAsync(c->ay, c->stage + c->n, nb3, hipMemcpyDeviceToDevice, s));

@@ -487,6 +487,37 @@ __device__ __forceinline__ void tile_mf3(const float4 *__restrict__ s, const nb_
     for (int jc = 16; jc < BS; jc += 16) mf3_chunk<false>(s + jc, sa[jc + c16][grp], grp, Bg, S);
 }
 
+// Potential only (no accelerations): with q = rsq(w_j r^2) = m_j^(1/3) / r
+// from the same product, m_j / r = q c_j where c_j = m_j^(2/3) = 1 / w_j
+// (0 for massless sources) rides in the w slot of the lo coordinates: rsq and
+// one FMA per pair.
+template <bool FIRST>
+__device__ __forceinline__ void mf3_pot_chunk(const float4 *__restrict__ sl, const nb_bf8 a, int grp,
+                                              const nb_bf8 (&Bg)[4], float (&S)[4]) {
+    const nb_f4v zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    nb_f4v D[4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) D[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, Bg[g], zero, 0, 0, 0);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const float c = sl[4 * grp + v].w;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const float q = __builtin_amdgcn_rsqf(D[g][v]);
+            S[g] = (FIRST && v == 0) ? q * c : __builtin_fmaf(q, c, S[g]);
+        }
+    }
+}
+
+__device__ __forceinline__ void tile_mf3_pot(const float4 *__restrict__ sl,
+                                             const nb_bf8 (*__restrict__ sa)[4], int lane,
+                                             const nb_bf8 (&Bg)[4], float (&S)[4]) {
+    const int c16 = lane & 15, grp = lane >> 4;
+    mf3_pot_chunk<true>(sl, sa[c16][grp], grp, Bg, S);
+#pragma unroll 1
+    for (int jc = 16; jc < BS; jc += 16) mf3_pot_chunk<false>(sl + jc, sa[jc + c16][grp], grp, Bg, S);
+}
+
 // FAR_MFMA_BF16 fold: target 16g + c sits in lanes c, 16 + c, 32 + c, 48 + c
 // (slot g) and lane l keeps slot l >> 4, its own target.  permlane32_swap of
 // slots g and g + 2 followed by an add leaves lanes 0-31 with slot g and lanes
@@ -514,6 +545,20 @@ __device__ __forceinline__ void fold_mf3(const float (&S)[4][4], double xd, doub
     Ax += (double)col[0] - xd * sm;
     Ay += (double)col[1] - yd * sm;
     Az += (double)col[2] - zd * sm;
+}
+
+// The same lane fold for one sum per target (the potential-only tiles).
+__device__ __forceinline__ float fold_mf3_one(const float (&S)[4]) {
+    float t1[2];
+#pragma unroll
+    for (int g0 = 0; g0 < 2; ++g0) {
+        const auto r = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(unsigned, S[g0]),
+                                                        __builtin_bit_cast(unsigned, S[g0 + 2]), false, false);
+        t1[g0] = __builtin_bit_cast(float, (unsigned)r[0]) + __builtin_bit_cast(float, (unsigned)r[1]);
+    }
+    const auto r = __builtin_amdgcn_permlane16_swap(__builtin_bit_cast(unsigned, t1[0]),
+                                                    __builtin_bit_cast(unsigned, t1[1]), false, false);
+    return __builtin_bit_cast(float, (unsigned)r[0]) + __builtin_bit_cast(float, (unsigned)r[1]);
 }
 
 // Per-wave tile classes.  Each wave uses the box of its own targets, their
@@ -598,8 +643,10 @@ __device__ __forceinline__ int64_t target_of(int64_t i0, int k, int tid) {
 // One block's share of an all-pairs sum: targets target_of(i0, k, tid) (k < TPT,
 // i < i_end) against the sources [j0, j1).  tile_box points at the box of
 // the BS-source tile that starts at j0 (tiles are BS-aligned from j0).
-// Returns FP64 sums (without G) in Ax..Aq and the target validity.
-template <typename T, int TPT, bool POT, int MF = 0>
+// Returns FP64 sums (without G) in Ax..Aq and the target validity.  With
+// ACC = false (and POT) only Aq is meaningful: the kernel writes the
+// potential alone, so the acceleration arithmetic is dead and removed.
+template <typename T, int TPT, bool POT, int MF = 0, bool ACC = true>
 __device__ __forceinline__ void force_block(
     BlockShared<T, MF> &sh, const T *__restrict__ tx, const T *__restrict__ ty,
     const T *__restrict__ tz, const double *__restrict__ th, double th_const, int64_t i0,
@@ -611,8 +658,9 @@ __device__ __forceinline__ void force_block(
     static_assert(!(MF == FAR_MFMA_SUM || MF == FAR_MFMA_R2) || (HILO && TPT == 1 && !POT),
                   "MFMA far tiles: SPLIT, 1 target/lane, acc only");
     static_assert(MF != FAR_WEIGHTED || !POT, "weighted far tiles: acc only");
-    static_assert(MF != FAR_MFMA_BF16 || (HILO && TPT == 1 && !POT),
-                  "bf16 matrix-core far tiles: SPLIT, 1 target/lane, acc only");
+    static_assert(MF != FAR_MFMA_BF16 || (HILO && TPT == 1 && (!POT || !ACC)),
+                  "bf16 matrix-core far tiles: SPLIT, 1 target/lane, acc or potential alone");
+    static_assert(ACC || POT, "nothing to compute");
     const int tid = threadIdx.x;
     __syncthreads();  // the previous call's readers of sh are done
     double S[4] = {0.0, 0.0, 0.0, 0.0};
@@ -729,7 +777,8 @@ __device__ __forceinline__ void force_block(
             if (HILO)
                 sh.sjl[HILO ? buf : 0][HILO ? tid : 0] =
                     make_float4((float)(dx - (double)hx), (float)(dy - (double)hy),
-                                (float)(dz - (double)hz), 0.0f);
+                                (float)(dz - (double)hz),
+                                (ROWS && !ACC && rw > 0.0f) ? __builtin_amdgcn_rcpf(rw) : 0.0f);
             if constexpr (MF == 1 || MF == 2) {
                 const double md = (double)rm;
                 sh.sB[buf][0][tid] = (float)(md * dx);
@@ -815,9 +864,15 @@ __device__ __forceinline__ void force_block(
             tile_f32_hilo<TPT, false, POT>(sh.sj[buf], sh.sjl[HILO ? buf : 0], xi, yi, zi, xil,
                                            yil, zil, h2c, h2, hinv, hinv3, ax, ay, az, aq);
         } else if (ROWS && (m_far & tbit)) {
-            float S3[4][4];
-            tile_mf3(sh.sj[buf], sh.sA[0], tid & 63, Bg, S3);
-            fold_mf3(S3, px[0] - ox, py[0] - oy, pz[0] - oz, Ax[0], Ay[0], Az[0]);
+            if constexpr (!ACC) {
+                float Sq[4];
+                tile_mf3_pot(sh.sjl[HILO ? buf : 0], sh.sA[0], tid & 63, Bg, Sq);
+                Aq[0] += (double)fold_mf3_one(Sq);
+            } else {
+                float S3[4][4];
+                tile_mf3(sh.sj[buf], sh.sA[0], tid & 63, Bg, S3);
+                fold_mf3(S3, px[0] - ox, py[0] - oy, pz[0] - oz, Ax[0], Ay[0], Az[0]);
+            }
             continue;   // no FP32 VALU partials to flush
         } else if (WFAR && (m_far & tbit)) {
             tile_w<TPT>(sh.sj[buf], sh.sW[WFAR ? buf : 0], sh.sw[WFAR ? buf : 0], wbx, wby, wbz, wci,

@@ -21,8 +21,9 @@ namespace nb {
 
 // FP32-pair kernel.  T = double (SPLIT: FP64 state, per-block origin, hi/lo
 // coordinates on near tiles) or float (FP32 state, absolute coordinates).
-// MF selects the matrix-core far tiles (SPLIT, TPT 1, forces only).
-template <typename T, int TPT, bool POT, int MF = 0>
+// MF selects the matrix-core far tiles (SPLIT, TPT 1, forces or potentials
+// alone).  ACC = false writes the potential only.
+template <typename T, int TPT, bool POT, int MF = 0, bool ACC = true>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MF == 4 ? 4 : 3))) void k_force_f32(ForceArgs<T> a) {
     __shared__ BlockShared<T, MF> sh;
     const int tid = threadIdx.x;
@@ -34,7 +35,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MF == 4 ? 4 
     const int64_t j1 = (j0 + a.seg_len < a.ns) ? j0 + a.seg_len : a.ns;
     double Ax[TPT], Ay[TPT], Az[TPT], Aq[TPT];
     bool valid[TPT];
-    force_block<T, TPT, POT, MF>(sh, a.tx, a.ty, a.tz, a.th, a.th_const, i0, a.nt, a.sx, a.sy, a.sz,
+    force_block<T, TPT, POT, MF, ACC>(sh, a.tx, a.ty, a.tz, a.th, a.th_const, i0, a.nt, a.sx, a.sy, a.sz,
                              a.sm, a.sw, j0, j1, a.tile_box + (j0 / BS) * 8, a.skip_src,
                              a.use_origin != 0, Ax, Ay, Az, Aq, valid);
 #pragma unroll
@@ -42,15 +43,19 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MF == 4 ? 4 
         if (!valid[k]) continue;
         const int64_t i = target_of<TPT>(i0, k, tid);
         if (a.nseg == 1) {
-            a.ax[i] = a.G * Ax[k];
-            a.ay[i] = a.G * Ay[k];
-            a.az[i] = a.G * Az[k];
+            if (ACC) {
+                a.ax[i] = a.G * Ax[k];
+                a.ay[i] = a.G * Ay[k];
+                a.az[i] = a.G * Az[k];
+            }
             if (POT) a.pot[i] = -a.G * Aq[k];
         } else {
             double *p = a.part + (int64_t)seg * 4 * a.nt;
-            p[i] = Ax[k];
-            p[a.nt + i] = Ay[k];
-            p[2 * a.nt + i] = Az[k];
+            if (ACC) {
+                p[i] = Ax[k];
+                p[a.nt + i] = Ay[k];
+                p[2 * a.nt + i] = Az[k];
+            }
             if (POT) p[3 * a.nt + i] = Aq[k];
         }
     }
@@ -113,7 +118,8 @@ __global__ __launch_bounds__(BS) void k_force_f64(ForceArgs<double> a) {
 // Sum the per-segment partials in fixed segment order and apply G.
 __global__ void k_finalize(const double *__restrict__ part, int nseg, int64_t n, double G,
                            double *__restrict__ ax, double *__restrict__ ay,
-                           double *__restrict__ az, double *__restrict__ pot, int with_pot) {
+                           double *__restrict__ az, double *__restrict__ pot, int with_pot,
+                           int with_acc) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double sx = 0.0, sy = 0.0, sz = 0.0, sq = 0.0;
@@ -124,9 +130,11 @@ __global__ void k_finalize(const double *__restrict__ part, int nseg, int64_t n,
         sz += p[2 * n + i];
         if (with_pot) sq += p[3 * n + i];
     }
-    ax[i] = G * sx;
-    ay[i] = G * sy;
-    az[i] = G * sz;
+    if (with_acc) {
+        ax[i] = G * sx;
+        ay[i] = G * sy;
+        az[i] = G * sz;
+    }
     if (with_pot) pot[i] = -G * sq;
 }
 
@@ -189,7 +197,7 @@ int launch_tile_boxes(const void *x, const void *y, const void *z, int is_double
 }
 
 template <typename T, int MF>
-static int launch_f32_impl(const ForceArgs<T> &a, int tpt, bool pot, hipStream_t s) {
+static int launch_f32_impl(const ForceArgs<T> &a, int tpt, bool pot, hipStream_t s, bool acc) {
     const int64_t nb = (int64_t)a.nblk_i * a.nseg;
     if (nb <= 0) return NB_OK;
     if (nb > INT32_MAX) {
@@ -198,21 +206,31 @@ static int launch_f32_impl(const ForceArgs<T> &a, int tpt, bool pot, hipStream_t
     }
     dim3 g((unsigned)nb), b(BS);
 #define NB_L(TPT_, POT_) k_force_f32<T, TPT_, POT_, (POT_ ? 0 : MF)><<<g, b, 0, s>>>(a)
-    if (tpt == 1) { if (pot) NB_L(1, true); else NB_L(1, false); }
-    else if (tpt == 2) { if (pot) NB_L(2, true); else NB_L(2, false); }
-    else if (tpt == 4) { if (pot) NB_L(4, true); else NB_L(4, false); }
+#define NB_LP(TPT_) k_force_f32<T, TPT_, true, 0, false><<<g, b, 0, s>>>(a)
+    if (tpt == 1) { if (pot && !acc) NB_LP(1); else if (pot) NB_L(1, true); else NB_L(1, false); }
+    else if (tpt == 2) { if (pot && !acc) NB_LP(2); else if (pot) NB_L(2, true); else NB_L(2, false); }
+    else if (tpt == 4) { if (pot && !acc) NB_LP(4); else if (pot) NB_L(4, true); else NB_L(4, false); }
     else {
         set_error("unsupported targets_per_thread %d", tpt);
         return NB_ERR_INVALID;
     }
 #undef NB_L
+#undef NB_LP
     return check_launch("k_force_f32");
 }
 
-int launch_force_f32(const ForceArgs<float> &a, int tpt, bool pot, hipStream_t s) {
-    return launch_f32_impl<float, FAR_VALU>(a, tpt, pot, s);
+int launch_force_f32(const ForceArgs<float> &a, int tpt, bool pot, hipStream_t s, bool acc) {
+    return launch_f32_impl<float, FAR_VALU>(a, tpt, pot, s, acc || !pot);
 }
-int launch_force_split(const ForceArgs<double> &a, int tpt, bool pot, hipStream_t s, int far_path) {
+int launch_force_split(const ForceArgs<double> &a, int tpt, bool pot, hipStream_t s, int far_path,
+                       bool acc) {
+    if (!acc && !pot) acc = true;
+    if (far_path == FAR_MFMA_BF16 && tpt == 1 && pot && !acc) {
+        const int64_t nb = (int64_t)a.nblk_i * a.nseg;
+        if (nb <= 0) return NB_OK;
+        k_force_f32<double, 1, true, FAR_MFMA_BF16, false><<<(unsigned)nb, BS, 0, s>>>(a);
+        return check_launch("k_force_f32<bf16 mfma, potential>");
+    }
     if ((far_path == FAR_MFMA_SUM || far_path == FAR_MFMA_R2) && tpt == 1 && !pot) {
         const int64_t nb = (int64_t)a.nblk_i * a.nseg;
         if (nb <= 0) return NB_OK;
@@ -226,8 +244,8 @@ int launch_force_split(const ForceArgs<double> &a, int tpt, bool pot, hipStream_
         k_force_f32<double, 1, false, FAR_MFMA_BF16><<<(unsigned)nb, BS, 0, s>>>(a);
         return check_launch("k_force_f32<bf16 mfma>");
     }
-    if (far_path == FAR_WEIGHTED) return launch_f32_impl<double, FAR_WEIGHTED>(a, tpt, pot, s);
-    return launch_f32_impl<double, FAR_VALU>(a, tpt, pot, s);
+    if (far_path == FAR_WEIGHTED) return launch_f32_impl<double, FAR_WEIGHTED>(a, tpt, pot, s, acc);
+    return launch_f32_impl<double, FAR_VALU>(a, tpt, pot, s, acc);
 }
 
 template <typename T>
@@ -255,10 +273,11 @@ int launch_force_f64(const ForceArgs<double> &a, bool pot, hipStream_t s) {
 }
 
 int launch_finalize(const double *part, int nseg, int64_t n, double G, double *ax, double *ay,
-                    double *az, double *pot, bool with_pot, hipStream_t s) {
+                    double *az, double *pot, bool with_pot, hipStream_t s, bool with_acc) {
     if (n <= 0) return NB_OK;
     const unsigned nb = (unsigned)((n + 255) / 256);
-    k_finalize<<<nb, 256, 0, s>>>(part, nseg, n, G, ax, ay, az, pot, with_pot ? 1 : 0);
+    k_finalize<<<nb, 256, 0, s>>>(part, nseg, n, G, ax, ay, az, pot, with_pot ? 1 : 0,
+                                  with_acc ? 1 : 0);
     return check_launch("k_finalize");
 }
 

@@ -64,14 +64,16 @@ constexpr int FAR_MFMA_BF16 = 4;  // bf16-split r^2 on the matrix cores, sum on 
 // Launchers (nb_force.hip)
 int launch_tile_boxes(const void *x, const void *y, const void *z, int is_double, int64_t n,
                       double *box, hipStream_t s);
-int launch_force_f32(const ForceArgs<float> &a, int tpt, bool pot, hipStream_t s);
+// acc = false (with pot) writes the potential alone.
+int launch_force_f32(const ForceArgs<float> &a, int tpt, bool pot, hipStream_t s, bool acc = true);
 int launch_force_split(const ForceArgs<double> &a, int tpt, bool pot, hipStream_t s,
-                       int far_path = 0);
+                       int far_path = 0, bool acc = true);
 int launch_force_f64(const ForceArgs<double> &a, bool pot, hipStream_t s);
 // w_i = m_i^(-2/3) (0 for massless particles) for the weighted far paths.
 int launch_source_weights(const void *m, bool is_double, int64_t n, float *w, hipStream_t s);
 inline bool far_path_weighted(int far_path) { return far_path == FAR_WEIGHTED || far_path == FAR_MFMA_BF16; }
 int launch_finalize(const double *part, int nseg, int64_t n, double G, double *ax, double *ay,
-                    double *az, double *pot, bool with_pot, hipStream_t s);
+                    double *az, double *pot, bool with_pot, hipStream_t s,
+                    bool with_acc = true);
 
 }  // namespace nb

@@ -318,6 +318,25 @@ def test_far_outlier_particle(pkg, orc, far):
     assert np.median(e) <= 2e-7 and np.percentile(e, 99) <= 2e-6 and e[-1] <= 1e-6
 
 
+@pytest.mark.parametrize("far", [0, 3, 4])
+def test_potential_alone_far_paths(pkg, orc, far):
+    """compute_potential runs potential-only kernels (with far path 4 on the
+    matrix cores: m/r = rsq(w r^2) m^(2/3)); the accelerations are left as
+    they were."""
+    pos, vel, mass = pkg.ics.collision(8192, seeds=(3, 4))
+    soft = [0.002] * 6
+    ref = orc.compute_potential(pos, mass, soft)
+    with make_ctx(pkg, pos, vel, mass, soft, "split", tuning=(1, 4, 0, far)) as ctx:
+        ctx.compute_force()
+        acc0 = ctx.download(pos=False, vel=False)["acc"]
+        ctx.compute_potential(self_term=False)
+        d = ctx.download(pos=False, vel=False, acc=True, pot=True)
+    e = np.abs(d["pot"] - ref) / np.abs(ref)
+    print(f"far_path={far}: potential median {np.median(e):.3e} max {e.max():.3e}")
+    assert np.median(e) <= 1e-7 and e.max() <= 1e-5
+    assert np.array_equal(d["acc"], acc0)
+
+
 def test_morton_sort_preserves_order_and_values(pkg, orc):
     pos, vel, mass = pkg.ics.collision(3000, seeds=(3, 4))
     soft = [0.002] * 6
