@@ -318,21 +318,27 @@ def test_far_outlier_particle(pkg, orc, far):
     assert np.median(e) <= 2e-7 and np.percentile(e, 99) <= 2e-6 and e[-1] <= 1e-6
 
 
-@pytest.mark.parametrize("far", [0, 3, 4])
-def test_potential_alone_far_paths(pkg, orc, far):
+@pytest.mark.parametrize("mode,tpt,far", [
+    ("split", 1, 0), ("split", 1, 3), ("split", 1, 4), ("split", 2, 0), ("split", 4, 3),
+    ("fp32", 1, 0), ("fp32", 2, 0), ("fp32", 4, 0)])
+def test_potential_alone_far_paths(pkg, orc, mode, tpt, far):
     """compute_potential runs potential-only kernels (with far path 4 on the
     matrix cores: m/r = rsq(w r^2) m^(2/3)); the accelerations are left as
     they were."""
     pos, vel, mass = pkg.ics.collision(8192, seeds=(3, 4))
     soft = [0.002] * 6
-    ref = orc.compute_potential(pos, mass, soft)
-    with make_ctx(pkg, pos, vel, mass, soft, "split", tuning=(1, 4, 0, far)) as ctx:
+    if mode == "fp32":   # the FP32 state rounds the inputs: compare on the rounded inputs
+        ref = orc.compute_potential(pos.astype(np.float32).astype(np.float64),
+                                    mass.astype(np.float32).astype(np.float64), soft)
+    else:
+        ref = orc.compute_potential(pos, mass, soft)
+    with make_ctx(pkg, pos, vel, mass, soft, mode, tuning=(tpt, 4, 0, far)) as ctx:
         ctx.compute_force()
         acc0 = ctx.download(pos=False, vel=False)["acc"]
         ctx.compute_potential(self_term=False)
         d = ctx.download(pos=False, vel=False, acc=True, pot=True)
     e = np.abs(d["pot"] - ref) / np.abs(ref)
-    print(f"far_path={far}: potential median {np.median(e):.3e} max {e.max():.3e}")
+    print(f"{mode} tpt={tpt} far_path={far}: potential median {np.median(e):.3e} max {e.max():.3e}")
     assert np.median(e) <= 1e-7 and e.max() <= 1e-5
     assert np.array_equal(d["acc"], acc0)
 
@@ -385,13 +391,19 @@ def test_far_paths_with_central_and_massless_sources(pkg, orc, far):
     mass[0] = 1.0
     soft = [1e-3] * 6
     ref = orc.compute_force(pos, mass, soft)
+    pref = orc.compute_potential(pos, mass, soft)
     with make_ctx(pkg, pos, vel, mass, soft, "split_central", tuning=(1, 2, 16, far)) as ctx:
         ctx.set_central(1)
         ctx.compute_force()
         acc = ctx.download(pos=False, vel=False)["acc"]
+        ctx.compute_potential(self_term=False)
+        pot = ctx.download(pos=False, vel=False, acc=False, pot=True)["pot"]
     e = orc.rel_err(acc, ref)
-    print(f"far_path={far}: median {np.median(e):.3e} max {e.max():.3e}")
+    ep = np.abs(pot - pref) / np.abs(pref)
+    print(f"far_path={far}: median {np.median(e):.3e} max {e.max():.3e}; "
+          f"potential median {np.median(ep):.3e} max {ep.max():.3e}")
     assert np.median(e) <= GATE and e.max() < 1e-5
+    assert np.median(ep) <= GATE and ep.max() < 1e-5
 
 
 def test_background_point_mass(pkg, orc):
