@@ -119,19 +119,61 @@ __global__ void k_adaptive_drift(T *__restrict__ x, T *__restrict__ y, T *__rest
     z[i] += vz[i] * d;
 }
 
-__global__ __launch_bounds__(256) void k_min_endstep(const int64_t *__restrict__ te, int64_t n,
-                                                     int64_t base, int64_t *__restrict__ out) {
-    __shared__ int64_t red[256];
-    int64_t m = base;
-    for (int64_t i = threadIdx.x; i < n; i += 256) m = te[i] < m ? te[i] : m;
-    red[threadIdx.x] = m;
+// Minimum end step and the number of particles at it: (m, k) pairs combine as
+// (min, sum of the counts of the operands at that minimum).  Start value
+// (time_base, 0), so an empty range leaves (time_base, 0).
+constexpr int MINRED_BLOCKS = 256;
+
+__device__ __forceinline__ void min_count_block(int64_t m, int64_t k, int64_t *sm, int64_t *sk) {
+    const int t = threadIdx.x;
+    sm[t] = m;
+    sk[t] = k;
     __syncthreads();
     for (int off = 128; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off && red[threadIdx.x + off] < red[threadIdx.x])
-            red[threadIdx.x] = red[threadIdx.x + off];
+        if (t < off) {
+            const int64_t m1 = sm[t], m2 = sm[t + off];
+            const int64_t mm = m2 < m1 ? m2 : m1;
+            sk[t] = (m1 == mm ? sk[t] : 0) + (m2 == mm ? sk[t + off] : 0);
+            sm[t] = mm;
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_min_count_part(const int64_t *__restrict__ te, int64_t n,
+                                                        int64_t base, int64_t *__restrict__ part) {
+    __shared__ int64_t sm[256], sk[256];
+    int64_t m = base, k = 0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int64_t v = te[i];
+        if (v < m) {
+            m = v;
+            k = 1;
+        } else if (v == m) {
+            ++k;
+        }
+    }
+    min_count_block(m, k, sm, sk);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sm[0];
+        part[2 * blockIdx.x + 1] = sk[0];
+    }
+}
+
+// out = (minimum, count, error flag of the last kernel that set it)
+__global__ __launch_bounds__(256) void k_min_count_final(const int64_t *__restrict__ part, int nb,
+                                                         int64_t base,
+                                                         const unsigned *__restrict__ flag,
+                                                         int64_t *__restrict__ out) {
+    __shared__ int64_t sm[256], sk[256];
+    const int t = threadIdx.x;
+    const bool in = t < nb;
+    min_count_block(in ? part[2 * t] : base, in ? part[2 * t + 1] : 0, sm, sk);
+    if (t == 0) {
+        out[0] = sm[0];
+        out[1] = sk[0];
+        out[2] = (int64_t)*flag;
+    }
 }
 
 __global__ void k_active_flags(const int64_t *__restrict__ te, int64_t n, int64_t t_int,
@@ -212,16 +254,35 @@ int check_adaptive(nb_ctx *c) {
     return NB_OK;
 }
 
-int read_flag(nb_ctx *c) {
-    unsigned f = 0;
-    NB_HIP(hipMemcpyAsync(&f, c->flag, sizeof f, hipMemcpyDeviceToHost, c->stream));
-    NB_HIP(hipStreamSynchronize(c->stream));
+int flag_error(nb_ctx *c, unsigned f) {
     if (f) {
         NB_HIP(hipMemsetAsync(c->flag, 0, sizeof(unsigned), c->stream));
         NB_HIP(hipStreamSynchronize(c->stream));
         set_error(f & 1 ? "Timestep < MinStep" : "Wrong integer timestep");
         return NB_ERR_TIMESTEP;
     }
+    return NB_OK;
+}
+
+int read_flag(nb_ctx *c) {
+    unsigned f = 0;
+    NB_HIP(hipMemcpyAsync(&f, c->flag, sizeof f, hipMemcpyDeviceToHost, c->stream));
+    NB_HIP(hipStreamSynchronize(c->stream));
+    return flag_error(c, f);
+}
+
+// (minimum end step, particles at it, error flag) of the current timeline, in
+// one copy and one wait.
+int reduce_steps(nb_ctx *c, int64_t (&r)[3]) {
+    hipStream_t s = c->stream;
+    const int nb = (int)std::min<int64_t>(MINRED_BLOCKS, std::max<int64_t>(1, (c->n + 255) / 256));
+    k_min_count_part<<<nb, 256, 0, s>>>(c->ti_end, c->n, c->ad.time_base, c->ad.red);
+    NB_TRY(launch_ok("k_min_count_part"));
+    k_min_count_final<<<1, 256, 0, s>>>(c->ad.red, nb, c->ad.time_base, c->flag,
+                                        c->ad.red + 2 * MINRED_BLOCKS);
+    NB_TRY(launch_ok("k_min_count_final"));
+    NB_HIP(hipMemcpyAsync(r, c->ad.red + 2 * MINRED_BLOCKS, sizeof r, hipMemcpyDeviceToHost, s));
+    NB_HIP(hipStreamSynchronize(s));
     return NB_OK;
 }
 
@@ -246,6 +307,7 @@ int nb_adaptive_init(nb_ctx *c, double time_interval, int64_t time_base, double 
         set_error("invalid adaptive timestep parameters");
         return NB_ERR_INVALID;
     }
+    c->ad.next_valid = false;
     c->ad.time_interval = time_interval;
     c->ad.time_base = time_base;
     c->ad.err_tol = err_tol;
@@ -258,6 +320,7 @@ int nb_adaptive_init(nb_ctx *c, double time_interval, int64_t time_base, double 
         NB_TRY(dalloc(&c->ad.flags, n));
         NB_TRY(dalloc(&c->ad.idx, sizeof(int64_t) * n));
         NB_TRY(dalloc(&c->ad.cnt, sizeof(int64_t) * 2));
+        NB_TRY(dalloc(&c->ad.red, sizeof(int64_t) * (2 * MINRED_BLOCKS + 3)));
         NB_TRY(dalloc(&c->ad.buf, sizeof(double) * 7 * n));
         size_t bytes = 0;
         rocprim::counting_iterator<int64_t> it(0);
@@ -299,11 +362,15 @@ int nb_adaptive_min_endstep(nb_ctx *c, int64_t *min_endstep) {
         set_error("null output");
         return NB_ERR_INVALID;
     }
-    k_min_endstep<<<1, 256, 0, c->stream>>>(c->ti_end, c->n, c->ad.time_base, c->ad.cnt + 1);
-    NB_TRY(launch_ok("k_min_endstep"));
-    NB_HIP(hipMemcpyAsync(min_endstep, c->ad.cnt + 1, sizeof(int64_t), hipMemcpyDeviceToHost,
-                          c->stream));
-    NB_HIP(hipStreamSynchronize(c->stream));
+    // after a kick the reduction has already been read back with its flag
+    if (!c->ad.next_valid) {
+        int64_t r[3];
+        NB_TRY(reduce_steps(c, r));
+        c->ad.next_min = r[0];
+        c->ad.next_count = r[1];
+        c->ad.next_valid = true;
+    }
+    *min_endstep = c->ad.next_min;
     return NB_OK;
 }
 
@@ -336,9 +403,16 @@ int nb_adaptive_compute_force(nb_ctx *c, int64_t t_int, int64_t *nactive) {
     size_t bytes = c->ad.tmp_bytes;
     rocprim::counting_iterator<int64_t> it(0);
     NB_HIP(rocprim::select(c->ad.tmp, bytes, it, c->ad.flags, c->ad.idx, c->ad.cnt, (size_t)n, s));
+    // The count is known without a wait when t_int is the minimum end step
+    // reduced since Ti_endstep last changed (the usual block step); the
+    // kernels below read the device count, which equals it.
     int64_t na = 0;
-    NB_HIP(hipMemcpyAsync(&na, c->ad.cnt, sizeof na, hipMemcpyDeviceToHost, s));
-    NB_HIP(hipStreamSynchronize(s));
+    if (c->ad.next_valid && t_int == c->ad.next_min) {
+        na = c->ad.next_count;
+    } else {
+        NB_HIP(hipMemcpyAsync(&na, c->ad.cnt, sizeof na, hipMemcpyDeviceToHost, s));
+        NB_HIP(hipStreamSynchronize(s));
+    }
     if (nactive) *nactive = na;
     if (na == 0) return NB_OK;
     if (na == n) return ctx_run_force(c, false, true);   // everyone active: the plain path
@@ -424,6 +498,7 @@ int nb_adaptive_compute_force(nb_ctx *c, int64_t t_int, int64_t *nactive) {
 int nb_adaptive_kick(nb_ctx *c, int64_t t_int) {
     NB_TRY(check_adaptive(c));
     if (c->n == 0) return NB_OK;
+    c->ad.next_valid = false;
     if (c->is_double)
         k_adaptive_advance<double><<<gridn(c->n), 256, 0, c->stream>>>(
             (double *)c->vx, (double *)c->vy, (double *)c->vz, c->ax, c->ay, c->az, c->h, c->ti_end,
@@ -433,11 +508,19 @@ int nb_adaptive_kick(nb_ctx *c, int64_t t_int) {
             (float *)c->vx, (float *)c->vy, (float *)c->vz, c->ax, c->ay, c->az, c->h, c->ti_end,
             c->ti_beg, c->n, t_int, params_of(c), c->flag);
     NB_TRY(launch_ok("k_adaptive_advance"));
-    return read_flag(c);
+    // the new timeline's next sync point comes back with the error flag
+    int64_t r[3];
+    NB_TRY(reduce_steps(c, r));
+    NB_TRY(flag_error(c, (unsigned)r[2]));
+    c->ad.next_min = r[0];
+    c->ad.next_count = r[1];
+    c->ad.next_valid = true;
+    return NB_OK;
 }
 
 int nb_adaptive_set_steps(nb_ctx *c, const int64_t *endstep, const int64_t *begstep) {
     NB_TRY(check_adaptive(c));
+    c->ad.next_valid = false;
     const int64_t n = c->n;
     if (n == 0) return NB_OK;
     if (!endstep || !begstep) {

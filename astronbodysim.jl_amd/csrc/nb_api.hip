@@ -435,7 +435,8 @@ int nb_destroy(nb_ctx *c) {
     dfree(c->time); dfree(c->flag); dfree(c->central);
     dfree(c->cx); dfree(c->cy); dfree(c->cz); dfree(c->cm); dfree(c->cidx);
     dfree(c->ti_end); dfree(c->ti_beg);
-    dfree(c->ad.flags); dfree(c->ad.idx); dfree(c->ad.cnt); dfree(c->ad.buf); dfree(c->ad.tmp);
+    dfree(c->ad.flags); dfree(c->ad.idx); dfree(c->ad.cnt); dfree(c->ad.red); dfree(c->ad.buf);
+    dfree(c->ad.tmp);
     sort_free(c);
     snapshot_free(c);
     if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -30,7 +30,12 @@ struct AdaptState {
     double err_tol = 0.0, min_step = 0.0, max_step = 0.0;
     uint8_t *flags = nullptr;  // active flags
     int64_t *idx = nullptr;    // compacted active indices
-    int64_t *cnt = nullptr;    // [0] active count, [1] min end step
+    int64_t *cnt = nullptr;    // [0] active count
+    int64_t *red = nullptr;    // end-step reduction: block partials, then (min, count, flag)
+    // (minimum end step, particles at it) of the current Ti_endstep, when
+    // next_valid; read back together with the error flag after each kick
+    bool next_valid = false;
+    int64_t next_min = 0, next_count = 0;
     double *buf = nullptr;     // gathered targets and their accelerations (7n doubles)
     void *tmp = nullptr;       // rocPRIM select storage
     size_t tmp_bytes = 0;
