@@ -52,6 +52,37 @@ def test_invalid_arguments_rejected_without_device(pkg):
         nat.call("nb_kdk_steps", None, 1, 1e-3, 0.0, 0)
 
 
+def test_last_error_is_per_thread(pkg):
+    """nb_last_error belongs to the calling thread: an error raised on another
+    thread in between does not replace it."""
+    import threading
+    nat = pkg._native
+    step = threading.Barrier(2)
+    seen = {}
+
+    def first():
+        h = C.c_void_p()
+        assert nat.lib.nb_create(C.byref(h), 10, 7) == nat.NB_ERR_INVALID
+        step.wait()          # the other thread fails now
+        step.wait()
+        seen["first"] = nat.last_error()
+
+    def second():
+        step.wait()
+        h = C.c_void_p()
+        assert nat.lib.nb_create(C.byref(h), -1, 2) == nat.NB_ERR_INVALID
+        seen["second"] = nat.last_error()
+        step.wait()
+
+    ts = [threading.Thread(target=first), threading.Thread(target=second)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join(timeout=30)
+    assert "precision mode" in seen["first"]
+    assert seen["second"] and "precision mode" not in seen["second"]
+
+
 def test_tuner_table_roundtrip(pkg, tmp_path):
     nat = pkg._native
     p = tmp_path / "tune.json"

@@ -343,6 +343,47 @@ def test_potential_alone_far_paths(pkg, orc, mode, tpt, far):
     assert np.array_equal(d["acc"], acc0)
 
 
+def test_contexts_on_concurrent_threads(pkg):
+    """One context per host thread, each on its own stream: two systems
+    stepped at the same time from two threads (graph capture is thread-local)
+    give bit for bit what the same calls give one after the other."""
+    import threading
+    systems = [pkg.ics.plummer(8192, seed=1), pkg.ics.collision(8192, seeds=(3, 4))]
+    soft = [0.002] * 6
+
+    def run(pos, vel, mass, out, key, start=None):
+        with make_ctx(pkg, pos, vel, mass, soft, "split") as ctx:
+            if start is not None:
+                start.wait()
+            ctx.compute_force()
+            ctx.half_kick(+1, 1e-3)
+            ctx.kdk_steps(20, 1e-3, 0.0, use_graph=True)   # captured per thread
+            ctx.half_kick(-1, 1e-3)
+            out[key] = ctx.download(pos=True, vel=True, acc=True)
+
+    alone, together = {}, {}
+    for k, (pos, vel, mass) in enumerate(systems):
+        run(pos, vel, mass, alone, k)
+    start = threading.Barrier(2)
+    errors = []
+
+    def worker(k):
+        try:
+            run(*systems[k], together, k, start)
+        except Exception as exc:  # reported below
+            errors.append(exc)
+
+    ts = [threading.Thread(target=worker, args=(k,)) for k in range(2)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join(timeout=120)
+    assert not errors, errors
+    for k in range(2):
+        for f in ("pos", "vel", "acc"):
+            assert np.array_equal(together[k][f], alone[k][f]), (k, f)
+
+
 def test_morton_sort_preserves_order_and_values(pkg, orc):
     pos, vel, mass = pkg.ics.collision(3000, seeds=(3, 4))
     soft = [0.002] * 6
