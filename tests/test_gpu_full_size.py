@@ -11,6 +11,8 @@ property that holds at any size:
   time reversal of the KDK step (2 steps forward, velocities flipped, 2
   steps back: positions within 1e-9, velocities within 1e-6, absolute), and
   the potential of 512 sampled targets (median <= 1e-6);
+* the same collision at N = 2^22 (four times the benchmark): 256 sampled
+  targets and the momentum balance;
 * Plummer sphere, N = 65536, FP32 state (the FP32 configuration): a sample of
   2048 targets against the oracle on the FP32-rounded inputs (median <= 1e-6);
 * the two ensembles at their full sizes: eight sampled clusters of the
@@ -57,6 +59,32 @@ def test_collision_full_size_sampled_parity_and_momentum(pkg, orc, c3):
     ma = mass[:, None] * acc
     bal = np.linalg.norm(ma.sum(axis=0)) / np.linalg.norm(ma, axis=1).sum()
     print(f"momentum balance {bal:.3e}")
+    assert bal <= 1e-6
+
+
+def test_collision_4x_size_sampled_parity(pkg, orc):
+    """Four times the benchmark, N = 2^22 (no table entry: the default launch
+    of one source segment per target block, 16384 tiles each, far path 4):
+    256 sampled targets against every source, and the momentum balance."""
+    pos, vel, mass = pkg.ics.collision(1 << 21, seeds=(3, 4))
+    n, h = len(mass), 0.002
+    assert n == 1 << 22
+    ctx = _ctx(pkg, pos, vel, mass, h, "split")
+    try:
+        tuning = ctx.get_tuning()
+        ctx.compute_force()
+        acc = ctx.download(pos=False, vel=False)["acc"]
+    finally:
+        ctx.close()
+    assert np.isfinite(acc).all()
+    rng = np.random.default_rng(25)
+    sample = np.sort(rng.choice(n, 256, replace=False))
+    ref = orc.force_at_points(pos[sample], h, pos, mass, variant="simd")
+    e = orc.rel_err(acc[sample], ref)
+    ma = mass[:, None] * acc
+    bal = np.linalg.norm(ma.sum(axis=0)) / np.linalg.norm(ma, axis=1).sum()
+    print(f"N=2^22 {tuning}: median {np.median(e):.3e} max {e.max():.3e}; balance {bal:.3e}")
+    assert np.median(e) <= 2e-7 and e.max() <= 1e-5
     assert bal <= 1e-6
 
 
