@@ -63,9 +63,9 @@ def test_collision_full_size_sampled_parity_and_momentum(pkg, orc, c3):
 
 
 def test_collision_4x_size_sampled_parity(pkg, orc):
-    """Four times the benchmark, N = 2^22 (no table entry: the default launch
-    of one source segment per target block, 16384 tiles each, far path 4):
-    256 sampled targets against every source, and the momentum balance."""
+    """Four times the benchmark, N = 2^22 (no table entry: the default launch,
+    16 source segments of 1024 tiles, far path 4): 256 sampled targets
+    against every source, and the momentum balance."""
     pos, vel, mass = pkg.ics.collision(1 << 21, seeds=(3, 4))
     n, h = len(mass), 0.002
     assert n == 1 << 22
