@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import warnings
 from pathlib import Path
 
 import numpy as np
@@ -339,8 +340,14 @@ class Context:
         W = C.c_double()
         P = np.zeros(3)
         L = np.zeros(3)
-        call("nb_energy", self.handle, C.byref(K), C.byref(W) if potential else None,
-             dptr(P), dptr(L))
+        rc = lib.nb_energy(self.handle, C.byref(K), C.byref(W) if potential else None,
+                           dptr(P), dptr(L))
+        if rc == NB_ERR_NONFINITE:
+            # the reference returns NaN / Inf energies as they are: so does
+            # this host, with a warning
+            warnings.warn(last_error(), RuntimeWarning, stacklevel=2)
+        else:
+            check(rc, "nb_energy")
         return {"K": K.value, "W": W.value if potential else None, "P": P, "L": L}
 
     def set_central(self, count: int):

@@ -11,6 +11,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <string>
 #include <utility>
 #include <vector>
@@ -804,6 +805,19 @@ int nb_energy(nb_ctx *c, double *K, double *W, double P[3], double L[3]) {
     if (W) *W = 0.5 * host[1];
     if (P) { P[0] = host[2]; P[1] = host[3]; P[2] = host[4]; }
     if (L) { L[0] = host[5]; L[1] = host[6]; L[2] = host[7]; }
+    // the outputs stay written; a NaN or Inf among the requested ones is
+    // reported as NB_ERR_NONFINITE
+    const char *bad = nullptr;
+    if (K && !std::isfinite(host[0])) bad = "kinetic energy";
+    else if (W && !std::isfinite(host[1])) bad = "potential energy";
+    else if (P && !(std::isfinite(host[2]) && std::isfinite(host[3]) && std::isfinite(host[4])))
+        bad = "momentum";
+    else if (L && !(std::isfinite(host[5]) && std::isfinite(host[6]) && std::isfinite(host[7])))
+        bad = "angular momentum";
+    if (bad) {
+        set_error("non-finite %s (NaN or Inf in the particle state)", bad);
+        return NB_ERR_NONFINITE;
+    }
     return NB_OK;
 }
 

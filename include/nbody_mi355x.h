@@ -171,7 +171,9 @@ NB_API int nb_kdk_steps(nb_ctx *ctx, int64_t nsteps, double dt, double t_begin, 
  * P = sum m v, L = sum m x cross v; FP64, fixed-order reductions.  Replaces
  * total_kinetic / total_potential / total_momentum / total_angular_momentum
  * src/base/Energy.jl:13-100.  Any output pointer may be NULL; W == NULL skips
- * the potential evaluation. */
+ * the potential evaluation.  The outputs are always written; when one of the
+ * requested ones is NaN or Inf the call returns NB_ERR_NONFINITE (the
+ * reference returns such values silently). */
 NB_API int nb_energy(nb_ctx *ctx, double *K, double *W, double P[3], double L[3]);
 
 /* ---- hierarchical systems, background fields ---------------------------- */

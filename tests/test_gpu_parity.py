@@ -405,6 +405,29 @@ def test_energy_momentum_reductions(pkg, orc, plummer4096):
     assert np.allclose(e["P"], P, atol=1e-14) and np.allclose(e["L"], L, rtol=1e-12, atol=1e-15)
 
 
+def test_energy_reports_nonfinite_state(pkg):
+    """A NaN velocity: nb_energy still writes K, W, P, L and returns
+    NB_ERR_NONFINITE naming the first bad quantity; the Python host returns
+    the values (NaN where the reference would give NaN) with a warning."""
+    import ctypes as C
+    nat = pkg._native
+    pos, vel, mass = pkg.ics.plummer(1000, seed=3)
+    vel[17, 1] = np.nan
+    with make_ctx(pkg, pos, vel, mass, [0.01] * 6, "split") as ctx:
+        K, W = C.c_double(), C.c_double()
+        P, L = np.zeros(3), np.zeros(3)
+        rc = nat.lib.nb_energy(ctx.handle, C.byref(K), C.byref(W), nat.dptr(P), nat.dptr(L))
+        assert rc == nat.NB_ERR_NONFINITE
+        assert "kinetic" in nat.last_error()
+        assert np.isnan(K.value) and np.isfinite(W.value) and W.value < 0
+        with pytest.warns(RuntimeWarning, match="non-finite"):
+            e = ctx.energy()
+        assert np.isnan(e["K"]) and np.isnan(e["P"]).any() and np.isfinite(e["W"])
+        # potential alone is finite: no error
+        rc = nat.lib.nb_energy(ctx.handle, None, C.byref(W), None, None)
+        assert rc == nat.NB_OK
+
+
 def test_central_body_fp64_terms(pkg, orc):
     sysd = pkg.ics.planetary_ensemble(nsys=1, seed=6)
     pos, vel, mass = sysd["pos"], sysd["vel"], sysd["mass"]
