@@ -58,6 +58,25 @@ def test_config1_run_matches_oracle(pkg, orc, c1_inputs, c1_oracle, precision, t
     sim.close()
 
 
+def test_step_count_by_float_accumulation(pkg, orc):
+    """run(sim) counts steps as the reference does, by adding dt to the time
+    until it reaches TimeEnd (src/base/Timestep.jl:198, src/run.jl:158-160):
+    10^4 steps of 1e-3 up to 10.0 take 10001 steps, also when stretches of
+    steps are handed to nb_kdk_steps in one call."""
+    pos, vel, mass = pkg.ics.plummer(512, seed=9)
+    data = pkg.StarData(pos, Vel=vel, Mass=mass)
+    sim = pkg.Simulation(data, precision="split", TimeStep=1e-3, TimeEnd=10.0,
+                         TimeBetweenSnapshots=10.0, ForceSofteningTable=[0.05] * 6)
+    pkg.run(sim)
+    t, k = 0.0, 0
+    while t < 10.0:
+        t += 1e-3
+        k += 1
+    assert sim.timeinfo.stepcount == orc.count_steps(0.0, 10.0, 1e-3) == k == 10001
+    assert sim.timeinfo.system_time_float == t
+    sim.close()
+
+
 def test_snapshots_analysers_and_stop_file(pkg, orc, tmp_path):
     pos, vel, mass = pkg.ics.plummer(512, seed=3)
     soft = [0.02] * 6
