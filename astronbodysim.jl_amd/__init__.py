@@ -9,7 +9,7 @@ from .simulation import (
     MOND1983Milgrom, Newton, QUMOND, STAR,
     Simulation, StarData, Tiled, advance_and_find_timestep, compute_force, compute_potential,
     find_next_sync_point_and_drift, get_all_data, get_local_data, init_timesteps,
-    leapfrog_half_kick, run, set_softlen, set_softlen_, softlen, step, total_angular_momentum,
+    leapfrog_half_kick, numlocal, pack_pos, run, set_softlen, set_softlen_, softlen, step, total_angular_momentum,
     total_energy, total_kinetic, total_momentum, total_potential,
 )
 from . import ics
@@ -26,7 +26,7 @@ __all__ = [
     "Leapfrog", "Euler", "GAS", "HALO", "DISK", "BULGE", "STAR", "BLACKHOLE", "Simulation",
     "StarData", "run", "step", "compute_force", "compute_potential", "total_kinetic",
     "total_potential", "total_energy", "total_momentum", "total_angular_momentum",
-    "get_all_data", "get_local_data", "softlen", "set_softlen", "set_softlen_",
+    "get_all_data", "get_local_data", "numlocal", "pack_pos", "softlen", "set_softlen", "set_softlen_",
     "init_timesteps", "leapfrog_half_kick", "find_next_sync_point_and_drift",
     "advance_and_find_timestep", "ics", "Ensemble",
 ]

@@ -334,6 +334,19 @@ def get_all_data(sim: Simulation) -> StarData:
 get_local_data = get_all_data
 
 
+def numlocal(sim: Simulation) -> int:
+    """length(get_local_data(sim))  (src/base/Config.jl:1203-1205)"""
+    return len(sim)
+
+
+def pack_pos(sim: Simulation):
+    """x, y, z = pack_pos(sim)  (src/base/Plot.jl:1-5): the positions as three
+    arrays in the caller's order (unitless, as the library holds them),
+    downloaded without the other fields."""
+    pos = sim.ctx.download(pos=True, vel=False, acc=False)["pos"]
+    return pos[:, 0].copy(), pos[:, 1].copy(), pos[:, 2].copy()
+
+
 def softlen(c, sim: Simulation) -> float:
     """ForceSofteningTable[Int(c)]  (src/tools/SofteningLength.jl:10-24)"""
     return float(sim.config.grav.ForceSofteningTable[int(c) - 1])

@@ -117,6 +117,8 @@ def test_public_api_pieces(pkg, orc):
     pkg.compute_force(sim)
     d = pkg.get_all_data(sim)
     assert np.max(orc.rel_err(d.Acc, orc.compute_force(pos, mass, soft))) < 1e-12
+    x, y, z = pkg.pack_pos(sim)      # src/base/Plot.jl:1-5, caller order
+    assert np.array_equal(np.stack([x, y, z], axis=1), pos) and pkg.numlocal(sim) == 800
     pkg.compute_potential(sim)      # with the reference's self term
     d = pkg.get_all_data(sim)
     ref = orc.compute_potential(pos, mass, soft, self_term=True)
