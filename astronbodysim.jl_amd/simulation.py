@@ -334,6 +334,14 @@ def get_all_data(sim: Simulation) -> StarData:
 get_local_data = get_all_data
 
 
+def preprocessdata(sim: Simulation) -> None:
+    """preprocessdata(sim, ::DirectSum, ::GPU)  (src/run.jl:9-11) sets the
+    preferred units of the run.  This library holds unitless values (the
+    caller strips units when it builds the StarData), so there is nothing to
+    do; run(sim) needs no call to it."""
+    return None
+
+
 def numlocal(sim: Simulation) -> int:
     """length(get_local_data(sim))  (src/base/Config.jl:1203-1205)"""
     return len(sim)
