@@ -343,6 +343,27 @@ def test_potential_alone_far_paths(pkg, orc, mode, tpt, far):
     assert np.array_equal(d["acc"], acc0)
 
 
+@pytest.mark.parametrize("far", [0, 4])
+def test_translated_system_keeps_accuracy(pkg, orc, far):
+    """The collision moved 1e6 away from the coordinate origin: the
+    per-block FP64 origins keep the FP32 pair arithmetic as accurate as at
+    the origin (the oracle sums the same FP64 positions in each case)."""
+    pos0, vel, mass = pkg.ics.collision(8192, seeds=(3, 4))
+    soft = [0.002] * 6
+    med = {}
+    for shift in (0.0, 1.0e6):
+        pos = pos0 + np.array([1.0, -0.3, 0.2]) * shift
+        ref = orc.compute_force(pos, mass, soft)
+        with make_ctx(pkg, pos, vel, mass, soft, "split", tuning=(1, 4, 16, far)) as ctx:
+            ctx.compute_force()
+            acc = ctx.download(pos=False, vel=False)["acc"]
+        e = orc.rel_err(acc, ref)
+        med[shift] = np.median(e)
+        print(f"far_path={far}, offset {shift:g}: median {np.median(e):.3e} max {e.max():.3e}")
+        assert np.median(e) <= 2e-7 and e.max() <= 1e-5
+    assert med[1.0e6] <= 1.5 * med[0.0]
+
+
 def test_contexts_on_concurrent_threads(pkg):
     """One context per host thread, each on its own stream: two systems
     stepped at the same time from two threads (graph capture is thread-local)
