@@ -451,9 +451,12 @@ __device__ __forceinline__ uint64_t spread17(uint64_t v) {
     return v;
 }
 
+// Sort key: the system index above a Morton code of `mbits` bits per axis,
+// with 3 mbits + (bits of the largest system index) <= 64, so that no two
+// systems' keys interleave (mbits = 17 up to 8192 systems, fewer beyond).
 __global__ void k_batch_morton(const int32_t *__restrict__ sys_of, const double *__restrict__ x,
                                const double *__restrict__ y, const double *__restrict__ z,
-                               const double *__restrict__ box, int64_t n,
+                               const double *__restrict__ box, int64_t n, int mbits,
                                uint64_t *__restrict__ key, int64_t *__restrict__ idx) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -466,9 +469,10 @@ __global__ void k_batch_morton(const int32_t *__restrict__ sys_of, const double 
         const double ext = b[3 + d] - b[d];
         double u = ext > 0.0 ? (p[d] - b[d]) / ext : 0.0;
         u = fmin(fmax(u, 0.0), 1.0);
-        c[d] = (uint64_t)(u * 131071.0);
+        c[d] = (uint64_t)(u * (double)((1u << mbits) - 1u));
     }
-    key[i] = ((uint64_t)s << 51) | spread17(c[0]) | (spread17(c[1]) << 1) | (spread17(c[2]) << 2);
+    key[i] = ((uint64_t)s << (3 * mbits)) | spread17(c[0]) | (spread17(c[1]) << 1) |
+             (spread17(c[2]) << 2);
     idx[i] = i;
 }
 
@@ -586,7 +590,10 @@ static int batch_sort(nb_batch *b) {
     }
     k_sys_box<<<b->nsys, 256, 0, s>>>(b->off, b->x, b->y, b->z, b->sysbox);
     NB_TRY(launch_ok("k_sys_box"));
-    k_batch_morton<<<blocks_for(n), 256, 0, s>>>(b->sys_of, b->x, b->y, b->z, b->sysbox, n,
+    int sys_bits = 0;
+    while (sys_bits < 31 && ((int64_t)1 << sys_bits) < (int64_t)b->nsys) ++sys_bits;
+    const int mbits = std::min(17, (64 - sys_bits) / 3);
+    k_batch_morton<<<blocks_for(n), 256, 0, s>>>(b->sys_of, b->x, b->y, b->z, b->sysbox, n, mbits,
                                                  b->keys, b->idx);
     NB_TRY(launch_ok("k_batch_morton"));
     size_t bytes = b->sort_tmp_bytes;

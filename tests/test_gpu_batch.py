@@ -111,6 +111,70 @@ def test_batch_energy(pkg, orc, clusters):
         assert en["W"][s] == pytest.approx(W, rel=1e-6)
 
 
+def _degenerate(pkg, sizes, seed=31):
+    """An ensemble with empty, one- and two-particle systems among larger ones."""
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    pos = np.zeros((off[-1], 3))
+    vel = np.zeros((off[-1], 3))
+    mass = np.zeros(off[-1])
+    for s, n in enumerate(sizes):
+        if n:
+            p, v, m = pkg.ics.plummer(int(n), seed=seed + s)
+            pos[off[s]:off[s + 1]], vel[off[s]:off[s + 1]], mass[off[s]:off[s + 1]] = p, v, m
+    return {"offsets": off, "pos": pos, "vel": vel, "mass": mass, "h": np.full(len(sizes), 0.01)}
+
+
+@pytest.mark.parametrize("sizes,tun", [
+    ([0, 1, 2, 257, 0, 600, 1, 255], dict(resident=0)),
+    ([0, 1, 2, 257, 0, 600, 1, 255], dict(resident=0, targets_per_thread=2)),
+    ([0, 1, 2, 257, 0, 600, 1, 255], dict(resident=0, far_path=4)),
+    ([0, 1, 2, 255, 0, 73, 1], dict(resident=1))])
+def test_empty_and_tiny_systems(pkg, orc, sizes, tun):
+    """Systems of 0, 1 and 2 particles in the ragged and the resident launch:
+    forces match the oracle system by system, a lone particle feels nothing,
+    and KDK steps and energies stay finite."""
+    ens = _degenerate(pkg, sizes)
+    ref, _ = oracle_forces(orc, ens)
+    with make(pkg, ens, "split", **tun) as e:
+        e.compute_force()
+        acc = e.download(pos=False, vel=False)["acc"]
+        e.half_kick(+1)
+        e.kdk_steps(8)
+        e.half_kick(-1)
+        d = e.download(pos=True, vel=True, acc=True)
+        en = e.energy()
+    off = ens["offsets"]
+    for s, n in enumerate(sizes):
+        sl = slice(off[s], off[s + 1])
+        if n == 1:
+            assert np.all(acc[sl] == 0.0)
+        elif n > 1:
+            assert np.median(orc.rel_err(acc[sl], ref[sl])) <= GATE, s
+        if n == 0:
+            assert en["K"][s] == 0.0 and en["W"][s] == 0.0
+    assert all(np.isfinite(v).all() for v in d.values())
+    assert np.isfinite(en["K"]).all() and np.isfinite(en["W"]).all()
+
+
+def test_more_systems_than_the_sort_key_once_held(pkg, orc):
+    """9000 systems on the ragged (Morton-sorted) path: the sort key keeps
+    the system index above the Morton code for any number of systems, so
+    every particle stays in its own system (13 key bits held 8192)."""
+    rng = np.random.default_rng(41)
+    sizes = rng.integers(2, 41, 9000)
+    ens = _degenerate(pkg, list(sizes), seed=1000)
+    ref, _ = oracle_forces(orc, ens)
+    with make(pkg, ens, "split", resident=0, sort_interval=1) as e:
+        e.compute_force()
+        d = e.download(pos=True, vel=False, acc=True)
+    assert np.array_equal(d["pos"], ens["pos"])
+    off = ens["offsets"]
+    worst = max(np.median(orc.rel_err(d["acc"][off[s]:off[s + 1]], ref[off[s]:off[s + 1]]))
+                for s in range(len(sizes)))
+    print(f"9000 systems: worst per-system median {worst:.3e}")
+    assert worst <= GATE
+
+
 @pytest.fixture(scope="module")
 def planets(pkg):
     return pkg.ics.planetary_ensemble(nsys=6, seed=6)
