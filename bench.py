@@ -227,8 +227,9 @@ def bench_planetary(pkg, nat, args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding of "
-              "this path is described in DESIGN.md and not built", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); sharding this path over "
+              "several GPUs (targets split, sources all-gathered per step) is not built",
+              file=sys.stderr)
         sys.exit(2)
     import __graft_entry__ as ge
     pkg = ge.load_package()
