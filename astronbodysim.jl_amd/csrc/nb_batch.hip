@@ -106,7 +106,7 @@ struct BatchArgs {
 // 8 Morton-sorted 131072-particle Plummer spheres took 48.4 ms per force;
 // at 4 waves (a few values spilled outside the pair loops) 43.2 ms, against
 // 44.3 ms for far path 0.
-template <int TPT, bool POT, int MF = 0>
+template <int TPT, bool POT, int MF = 0, bool ACC = true>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MF == 4 ? 4 : 1))) void k_batch_force(
     BatchArgs a) {
     __shared__ BlockShared<double, MF> sh;
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MF == 4 ? 4 
         bool valid[TPT];
         // force_block starts with a barrier, so item_sh is read by every
         // thread before thread 0 overwrites it in the next iteration.
-        force_block<double, TPT, POT, MF>(sh, a.x, a.y, a.z, a.h, 0.0, i0, o1, a.x, a.y, a.z, a.m, a.w, o0,
+        force_block<double, TPT, POT, MF, ACC>(sh, a.x, a.y, a.z, a.h, 0.0, i0, o1, a.x, a.y, a.z, a.m, a.w, o0,
                                       o1, a.tile_box + a.tile_base[s] * 8, a.skip_src, true, Ax,
                                       Ay, Az, Aq, valid);
         const double G = a.Gs[s];
@@ -136,9 +136,11 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MF == 4 ? 4 
         for (int q = 0; q < TPT; ++q) {
             if (!valid[q]) continue;
             const int64_t i = target_of<TPT>(i0, q, tid);
-            a.ax[i] = G * Ax[q];
-            a.ay[i] = G * Ay[q];
-            a.az[i] = G * Az[q];
+            if (ACC) {
+                a.ax[i] = G * Ax[q];
+                a.ay[i] = G * Ay[q];
+                a.az[i] = G * Az[q];
+            }
             if (POT) a.pot[i] = -G * Aq[q];
         }
     }
@@ -187,7 +189,7 @@ __global__ __launch_bounds__(BS) void k_batch_force_f64(BatchArgs a) {
 }
 
 // FP64 terms of the central bodies of each system, added to acc (and pot).
-template <bool POT>
+template <bool POT, bool ACC = true>
 __global__ void k_batch_central(const int32_t *__restrict__ sys_of, const int64_t *__restrict__ cidx,
                                 int ncentral, const double *__restrict__ Gs,
                                 const double *__restrict__ x, const double *__restrict__ y,
@@ -207,9 +209,11 @@ __global__ void k_batch_central(const int32_t *__restrict__ sys_of, const int64_
         pair_f64<POT>(x[j], y[j], z[j], m[j], xi, yi, zi, hi, hinv, hinv3, Ax, Ay, Az, Aq);
     }
     const double G = Gs[s];
-    ax[i] += G * Ax;
-    ay[i] += G * Ay;
-    az[i] += G * Az;
+    if (ACC) {
+        ax[i] += G * Ax;
+        ay[i] += G * Ay;
+        az[i] += G * Az;
+    }
     if (POT) pot[i] -= G * Aq;
 }
 
@@ -622,7 +626,9 @@ static int bmaybe_sort(nb_batch *b, int64_t upcoming) {
     return NB_OK;
 }
 
-static int bforce(nb_batch *b, bool with_pot, bool allow_timing) {
+// pot_only (with with_pot): the FP32-pair kernels and the central pass write
+// the potential alone and leave the accelerations as they are.
+static int bforce(nb_batch *b, bool with_pot, bool allow_timing, bool pot_only = false) {
     if (b->n == 0) return NB_OK;
     hipStream_t s = b->stream;
     const bool f64 = b->mode == NB_MODE_FP64;
@@ -648,7 +654,10 @@ static int bforce(nb_batch *b, bool with_pot, bool allow_timing) {
         k_batch_tile_boxes<<<(unsigned)b->ntiles, BS, 0, s>>>(b->x, b->y, b->z, b->off,
                                                               b->tile_base, b->tile_sys, b->tile_box);
         NB_TRY(launch_ok("k_batch_tile_boxes"));
-        if (!with_pot && far_path_weighted(b->tun.far_path)) {
+        pot_only = pot_only && with_pot;
+        // potential alone on far path 4: m/r = rsq(w r^2) m^(2/3)
+        const bool pot_mf4 = pot_only && tpt == 1 && b->tun.far_path == FAR_MFMA_BF16;
+        if ((!with_pot && far_path_weighted(b->tun.far_path)) || pot_mf4) {
             NB_TRY(launch_source_weights(b->m, true, b->n, b->wsrc, s));
             a.w = b->wsrc;
         }
@@ -656,7 +665,11 @@ static int bforce(nb_batch *b, bool with_pot, bool allow_timing) {
         nblk = (int)std::max<int64_t>(1, std::min<int64_t>(nblk, b->nitems));
 #define NB_BL(T_, P_) k_batch_force<T_, P_><<<nblk, BS, 0, s>>>(a)
         const int fp = with_pot ? FAR_VALU : b->tun.far_path;
-        if (tpt == 1 && fp == FAR_MFMA_R2) k_batch_force<1, false, FAR_MFMA_R2><<<nblk, BS, 0, s>>>(a);
+        if (pot_mf4) k_batch_force<1, true, FAR_MFMA_BF16, false><<<nblk, BS, 0, s>>>(a);
+        else if (pot_only && tpt == 1) k_batch_force<1, true, FAR_VALU, false><<<nblk, BS, 0, s>>>(a);
+        else if (pot_only && tpt == 2) k_batch_force<2, true, FAR_VALU, false><<<nblk, BS, 0, s>>>(a);
+        else if (pot_only) k_batch_force<4, true, FAR_VALU, false><<<nblk, BS, 0, s>>>(a);
+        else if (tpt == 1 && fp == FAR_MFMA_R2) k_batch_force<1, false, FAR_MFMA_R2><<<nblk, BS, 0, s>>>(a);
         else if (tpt == 1 && fp == FAR_MFMA_SUM) k_batch_force<1, false, FAR_MFMA_SUM><<<nblk, BS, 0, s>>>(a);
         else if (tpt == 1 && fp == FAR_MFMA_BF16) k_batch_force<1, false, FAR_MFMA_BF16><<<nblk, BS, 0, s>>>(a);
         else if (fp == FAR_WEIGHTED && tpt == 1) k_batch_force<1, false, FAR_WEIGHTED><<<nblk, BS, 0, s>>>(a);
@@ -675,7 +688,11 @@ static int bforce(nb_batch *b, bool with_pot, bool allow_timing) {
         b->pending.emplace_back(e0, e1);
     }
     if (b->mode == NB_MODE_SPLIT_CENTRAL && b->ncentral > 0) {
-        if (with_pot)
+        if (with_pot && pot_only && !f64)
+            k_batch_central<true, false><<<blocks_for(b->n), 256, 0, s>>>(
+                b->sys_of, b->cidx, b->ncentral, b->Gs, b->x, b->y, b->z, b->m, b->h, b->n, b->ax,
+                b->ay, b->az, b->pot);
+        else if (with_pot)
             k_batch_central<true><<<blocks_for(b->n), 256, 0, s>>>(
                 b->sys_of, b->cidx, b->ncentral, b->Gs, b->x, b->y, b->z, b->m, b->h, b->n, b->ax,
                 b->ay, b->az, b->pot);
@@ -1063,7 +1080,7 @@ int nb_batch_compute_force(nb_batch *b) {
 int nb_batch_compute_potential(nb_batch *b) {
     NB_TRY(bcheck(b));
     NB_TRY(bmaybe_sort(b, 1));
-    NB_TRY(bforce(b, true, false));
+    NB_TRY(bforce(b, true, false, true));
     NB_HIP(hipStreamSynchronize(b->stream));
     return NB_OK;
 }
@@ -1104,7 +1121,7 @@ int nb_batch_kdk_steps_tracked(nb_batch *b, int64_t nsteps, int32_t every, doubl
         std::vector<double> host(ns * 8);
         for (int64_t k = 0; k < slots && rc == NB_OK; ++k) {
             rc = batch_steps(b, every, true);
-            if (rc == NB_OK) rc = bforce(b, true, false);   // potential (acc unchanged)
+            if (rc == NB_OK) rc = bforce(b, true, false, true);   // potential alone
             if (rc == NB_OK) {
                 k_batch_energy<<<b->nsys, 256, 0, b->stream>>>(
                     b->off, b->x, b->y, b->z, b->vx, b->vy, b->vz, b->ax, b->ay, b->az, b->m,
@@ -1147,7 +1164,7 @@ int nb_batch_energy(nb_batch *b, double *K, double *W, double *P, double *L) {
     NB_TRY(bcheck(b));
     if (W) {
         NB_TRY(bmaybe_sort(b, 1));
-        NB_TRY(bforce(b, true, false));
+        NB_TRY(bforce(b, true, false, true));
     }
     k_batch_energy<<<b->nsys, 256, 0, b->stream>>>(b->off, b->x, b->y, b->z, b->vx, b->vy, b->vz,
                                                    b->ax, b->ay, b->az, b->m, W ? b->pot : nullptr,

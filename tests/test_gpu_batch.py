@@ -156,6 +156,43 @@ def test_empty_and_tiny_systems(pkg, orc, sizes, tun):
     assert np.isfinite(en["K"]).all() and np.isfinite(en["W"]).all()
 
 
+@pytest.mark.parametrize("tpt,far", [(1, 0), (2, 0), (1, 4)])
+def test_potential_alone(pkg, orc, clusters, tpt, far):
+    """compute_potential writes the potential alone (with far path 4 on the
+    matrix cores: m/r = rsq(w r^2) m^(2/3)); the accelerations stay as the
+    last force left them."""
+    ref_acc, phi = oracle_forces(orc, clusters, pot=True)
+    with make(pkg, clusters, "split", resident=0, targets_per_thread=tpt, far_path=far) as e:
+        e.compute_force()
+        acc0 = e.download(pos=False, vel=False)["acc"]
+        e.compute_potential()
+        d = e.download(pos=False, vel=False, acc=True, pot=True)
+    assert np.array_equal(d["acc"], acc0)
+    off = clusters["offsets"]
+    for s in range(len(off) - 1):
+        sl = slice(off[s], off[s + 1])
+        ep = np.abs(d["pot"][sl] - phi[sl]) / np.abs(phi[sl])
+        assert np.median(ep) <= GATE and ep.max() <= 1e-5, s
+
+
+def test_potential_alone_with_central_bodies(pkg, orc):
+    """split_central ensembles: the FP64 central pass adds the star's
+    potential and leaves the accelerations alone."""
+    ens = pkg.ics.planetary_ensemble(nsys=6, seed=6)
+    _, phi = oracle_forces(orc, ens, skip_self=True, pot=True)
+    with make(pkg, ens, "split_central", resident=0) as e:
+        e.set_central(1)
+        e.compute_force()
+        acc0 = e.download(pos=False, vel=False)["acc"]
+        e.compute_potential()
+        d = e.download(pos=False, vel=False, acc=True, pot=True)
+    assert np.array_equal(d["acc"], acc0)
+    massive = np.abs(phi) > 0
+    ep = np.abs(d["pot"][massive] - phi[massive]) / np.abs(phi[massive])
+    print(f"planetary potential: median {np.median(ep):.3e} max {ep.max():.3e}")
+    assert np.median(ep) <= 1e-9 and ep.max() <= 1e-5
+
+
 def test_more_systems_than_the_sort_key_once_held(pkg, orc):
     """9000 systems on the ragged (Morton-sorted) path: the sort key keeps
     the system index above the Morton code for any number of systems, so
