@@ -139,7 +139,8 @@ NB_API int nb_compute_force(nb_ctx *ctx);
 /* Potential[i] = sum_j spline pair potential; replaces
  * compute_potential(sim, DirectSum, CPU)  src/directsum/potential.jl:142-148
  * (no GPU method exists in the reference).  add_self_term != 0 also applies
- * the public compute_potential(sim) self term  src/base/Potential.jl:5-24. */
+ * the public compute_potential(sim) self term  src/base/Potential.jl:5-24.
+ * The accelerations are left as they were. */
 NB_API int nb_compute_potential(nb_ctx *ctx, int32_t add_self_term);
 /* Force / potential at npts probe points with a caller-given softening h;
  * replaces compute_force(sim, pos, h, DirectSum, ...)  src/directsum/gravity.jl:171-194
@@ -282,6 +283,9 @@ NB_API int nb_batch_download(nb_batch *b, double *pos, double *vel, double *acc,
  * NB_MODE_SPLIT_CENTRAL). */
 NB_API int nb_batch_set_central(nb_batch *b, int32_t count);
 NB_API int nb_batch_compute_force(nb_batch *b);
+/* Potential of every particle within its own system; the accelerations are
+ * left as they were (nb_batch_energy's W and the tracked-energy steps run the
+ * same evaluation). */
 NB_API int nb_batch_compute_potential(nb_batch *b);
 /* Vel += Acc * forward * 0.5 * dt_s for every system s. */
 NB_API int nb_batch_half_kick(nb_batch *b, int32_t forward);
