@@ -19,6 +19,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <numeric>
 #include <rocprim/device/device_radix_sort.hpp>
@@ -1180,6 +1181,18 @@ int nb_batch_energy(nb_batch *b, double *K, double *W, double *P, double *L) {
         if (W) W[s] = 0.5 * r[1];
         if (P) { P[3 * s] = r[2]; P[3 * s + 1] = r[3]; P[3 * s + 2] = r[4]; }
         if (L) { L[3 * s] = r[5]; L[3 * s + 1] = r[6]; L[3 * s + 2] = r[7]; }
+    }
+    // the outputs stay written; the first system with a NaN or Inf among the
+    // requested quantities is reported as NB_ERR_NONFINITE
+    for (int32_t s = 0; s < b->nsys; ++s) {
+        const double *r = host.data() + (size_t)s * 8;
+        const bool bad = (K && !std::isfinite(r[0])) || (W && !std::isfinite(r[1])) ||
+                         (P && !(std::isfinite(r[2]) && std::isfinite(r[3]) && std::isfinite(r[4]))) ||
+                         (L && !(std::isfinite(r[5]) && std::isfinite(r[6]) && std::isfinite(r[7])));
+        if (bad) {
+            set_error("non-finite energy or momentum in system %d (NaN or Inf in its state)", s);
+            return NB_ERR_NONFINITE;
+        }
     }
     return NB_OK;
 }

@@ -9,6 +9,7 @@ ensemble object, so this module has no reference counterpart beyond that.
 from __future__ import annotations
 
 import ctypes as C
+import warnings
 
 import numpy as np
 
@@ -138,7 +139,12 @@ class Ensemble:
         W = np.zeros(self.nsys) if potential else None
         P = np.zeros((self.nsys, 3))
         L = np.zeros((self.nsys, 3))
-        call("nb_batch_energy", self.handle, dptr(K), dptr(W), dptr(P), dptr(L))
+        rc = nat.lib.nb_batch_energy(self.handle, dptr(K), dptr(W), dptr(P), dptr(L))
+        if rc == nat.NB_ERR_NONFINITE:
+            # NaN / Inf values are returned as they are, with a warning
+            warnings.warn(nat.last_error(), RuntimeWarning, stacklevel=2)
+        else:
+            nat.check(rc, "nb_batch_energy")
         return {"K": K, "W": W, "P": P, "L": L}
 
     def sync(self):

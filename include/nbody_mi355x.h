@@ -296,7 +296,9 @@ NB_API int nb_batch_kdk_steps(nb_batch *b, int64_t nsteps, int32_t use_graph);
  * velocities synchronised to the step: v - a dt/2) after every
  * `every`-th step into energy[s * (nsteps / every) + k]. */
 NB_API int nb_batch_kdk_steps_tracked(nb_batch *b, int64_t nsteps, int32_t every, double *energy);
-/* Per-system K, W (= 1/2 sum m phi), P[3], L[3]; arrays of nsys / 3*nsys. */
+/* Per-system K, W (= 1/2 sum m phi), P[3], L[3]; arrays of nsys / 3*nsys.
+ * The outputs are always written; NB_ERR_NONFINITE names the first system
+ * with a NaN or Inf among the requested ones. */
 NB_API int nb_batch_energy(nb_batch *b, double *K, double *W, double *P, double *L);
 NB_API int nb_batch_sync(nb_batch *b);
 NB_API int nb_batch_set_tuning(nb_batch *b, const nb_batch_tuning *t);

@@ -193,6 +193,20 @@ def test_potential_alone_with_central_bodies(pkg, orc):
     assert np.median(ep) <= 1e-9 and ep.max() <= 1e-5
 
 
+def test_energy_reports_nonfinite_system(pkg, clusters):
+    """A NaN velocity in system 3: nb_batch_energy writes every system's values,
+    returns NB_ERR_NONFINITE naming system 3, and the Python host warns."""
+    vel = clusters["vel"].copy()
+    off = clusters["offsets"]
+    vel[off[3] + 5, 0] = np.nan
+    ens = dict(clusters, vel=vel)
+    with make(pkg, ens, "split", resident=0) as e:
+        with pytest.warns(RuntimeWarning, match="system 3"):
+            en = e.energy()
+    assert np.isnan(en["K"][3]) and np.isfinite(np.delete(en["K"], 3)).all()
+    assert np.isfinite(en["W"]).all()
+
+
 def test_more_systems_than_the_sort_key_once_held(pkg, orc):
     """9000 systems on the ragged (Morton-sorted) path: the sort key keeps
     the system index above the Morton code for any number of systems, so
