@@ -62,7 +62,8 @@ int default_segments(int64_t n, int tpt, int far_path) {
     int64_t nseg = (want + nblk - 1) / std::max<int64_t>(nblk, 1);
     // far path 4 keeps at least 16 segments: measured on the two-sphere
     // collision at 2^21 / 2^22 particles, 16 segments ran 6 % / 5 % faster
-    // than the 2 / 1 that the block count alone gives
+    // than the 2 / 1 that the block count alone gives.  32 gained only
+    // 0.4 % / 0.2 % more there and doubles the partial sums (4 GB at 2^22).
     if (far_path == FAR_MFMA_BF16) nseg = std::max<int64_t>(nseg, 16);
     nseg = std::min<int64_t>(std::max<int64_t>(nseg, 1), std::max<int64_t>(ntiles, 1));
     nseg = std::min<int64_t>(nseg, 64);
