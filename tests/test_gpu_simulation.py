@@ -103,7 +103,8 @@ def test_snapshots_analysers_and_stop_file(pkg, orc, tmp_path):
     sim2 = pkg.Simulation(pkg.StarData(pos, Vel=vel, Mass=mass), TimeStep=1e-3, TimeEnd=1.0,
                           ForceSofteningTable=soft, OutputDir=str(tmp_path),
                           poststep=[lambda s: open(os.path.join(str(tmp_path), "stop"), "w").close()])
-    pkg.run(sim2)
+    with pytest.warns(UserWarning, match="Interrupted by user"):   # src/run.jl:153-157
+        pkg.run(sim2)
     assert sim2.timeinfo.stepcount == 1
     sim.close()
     sim2.close()
