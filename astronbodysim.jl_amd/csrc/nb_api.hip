@@ -905,6 +905,13 @@ int nb_set_tuning(nb_ctx *c, const nb_tuning *t) {
         set_error("invalid tuning");
         return NB_ERR_INVALID;
     }
+    // the matrix-core far paths hold one target per lane (as nb_tuner_load requires)
+    if ((t->far_path == FAR_MFMA_SUM || t->far_path == FAR_MFMA_R2 ||
+         t->far_path == FAR_MFMA_BF16) && t->targets_per_thread != 1) {
+        set_error("far_path %d needs targets_per_thread 1 (got %d)", t->far_path,
+                  t->targets_per_thread);
+        return NB_ERR_INVALID;
+    }
     drop_graph(c);
     c->tun = *t;
     c->sort.interval = t->sort_interval;

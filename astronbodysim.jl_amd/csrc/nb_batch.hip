@@ -489,6 +489,15 @@ __global__ void k_bgather(const V *__restrict__ src, const int64_t *__restrict__
     dst[i] = src[idx[i]];
 }
 
+// dst[perm[i]] = src[i]: back to the caller's order.
+template <typename V>
+__global__ void k_bscatter(const V *__restrict__ src, const int64_t *__restrict__ perm, int64_t n,
+                           V *__restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dst[perm[i]] = src[i];
+}
+
 __global__ void k_bcompose(const int64_t *__restrict__ old_perm, const int64_t *__restrict__ idx,
                            int64_t n, int64_t *__restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -616,6 +625,34 @@ static int batch_sort(nb_batch *b) {
     b->sorted = true;
     b->calls_since_sort = 0;
     return NB_OK;
+}
+
+// Undo the Morton order: every per-particle array back in the caller's order.
+// The resident kernel needs it (it takes the central bodies and the origin by
+// position in the system), and an ensemble sorted while it ran the ragged
+// path (resident = 0, or before nb_batch_set_tuning switched resident on)
+// comes back here before its first resident launch.
+static int batch_unsort(nb_batch *b) {
+    if (!b->perm) {
+        b->sorted = false;
+        return NB_OK;
+    }
+    const int64_t n = b->n;
+    hipStream_t s = b->stream;
+    if (n > 0) {
+        if (!b->spare) NB_TRY(dalloc(&b->spare, sizeof(double) * (size_t)n));
+        double *arrs[12] = {b->x, b->y, b->z, b->vx, b->vy, b->vz, b->m, b->h, b->ax, b->ay, b->az, b->pot};
+        for (double *p : arrs) {
+            k_bscatter<double><<<blocks_for(n), 256, 0, s>>>(p, b->perm, n, (double *)b->spare);
+            NB_TRY(launch_ok("k_bscatter"));
+            NB_HIP(hipMemcpyAsync(p, b->spare, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
+        }
+        NB_HIP(hipStreamSynchronize(s));
+    }
+    dfree(b->perm);
+    b->sorted = false;
+    b->calls_since_sort = 0;
+    return rebuild_central(b);
 }
 
 // Systems small enough for the resident kernel stay in caller order (the
@@ -746,7 +783,8 @@ static int resident_steps(nb_batch *b, int64_t nsteps, int every, double *energy
 static int batch_steps(nb_batch *b, int64_t nsteps, bool use_graph) {
     if (nsteps <= 0 || b->n == 0) return NB_OK;
     if (resident_ok(b) && use_graph) {
-        // the resident kernel runs unsorted (caller order); restore it if needed
+        // the resident kernel runs in the caller's order
+        NB_TRY(batch_unsort(b));
         return resident_steps(b, nsteps, 0, nullptr, 0);
     }
     int64_t k = 0;
@@ -1113,6 +1151,7 @@ int nb_batch_kdk_steps_tracked(nb_batch *b, int64_t nsteps, int32_t every, doubl
     const int64_t slots = nsteps / every;
     const size_t ns = (size_t)b->nsys;
     if (slots == 0) return batch_steps(b, nsteps, true);
+    if (resident_ok(b)) NB_TRY(batch_unsort(b));
     double *dev = nullptr;
     NB_TRY(dalloc(&dev, sizeof(double) * ns * (size_t)slots));
     int rc = NB_OK;
@@ -1210,6 +1249,12 @@ int nb_batch_set_tuning(nb_batch *b, const nb_batch_tuning *t) {
         t->blocks < 0 || t->blocks > (1 << 20) || t->steps_per_graph < 2 ||
         t->steps_per_graph > 4096 || t->sort_interval < 0 || t->far_path < 0 || t->far_path > 4) {
         set_error("invalid batch tuning");
+        return NB_ERR_INVALID;
+    }
+    if ((t->far_path == FAR_MFMA_SUM || t->far_path == FAR_MFMA_R2 ||
+         t->far_path == FAR_MFMA_BF16) && t->targets_per_thread != 1) {
+        set_error("far_path %d needs targets_per_thread 1 (got %d)", t->far_path,
+                  t->targets_per_thread);
         return NB_ERR_INVALID;
     }
     bdrop_graph(b);

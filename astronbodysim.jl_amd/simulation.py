@@ -556,7 +556,7 @@ def _fast_forward_steps(sim: Simulation) -> int:
             return k
         t = t_next
         k += 1
-        if k >= 256:   # the stop file is still checked every 256 steps
+        if k >= 256:   # the host checks the stop file between chunks
             return k
 
 
@@ -625,7 +625,9 @@ def _main_loop(sim: Simulation, use_graph: bool, append: bool, run_loop: bool):
     try:
         while state:
             t_total = time.time_ns()
-            nfast = _fast_forward_steps(sim)
+            # a stop file already present ends the run after one ordinary step,
+            # as the per-step check of run.jl:153-157 would
+            nfast = 0 if os.path.isfile(stopfile) else _fast_forward_steps(sim)
             if nfast >= 2:
                 dt = cfg.time.step.dt
                 t0 = time.time_ns()
@@ -638,6 +640,10 @@ def _main_loop(sim: Simulation, use_graph: bool, append: bool, run_loop: bool):
                 ti.dt = dt
                 ti.stepcount += nfast
                 add_timer(sim, "TOTAL", t_total, time.time_ns())
+                if os.path.isfile(stopfile):                 # run.jl:153-157, once per chunk
+                    import warnings
+                    warnings.warn("Interrupted by user")
+                    state = False
                 continue
             step(sim)                                        # run.jl:112
             t0 = time.time_ns()

@@ -264,6 +264,31 @@ def test_resident_matches_oracle_run(pkg, orc, planets):
         assert np.max(np.abs(d["vel"][sl] - vo)) < 1e-8
 
 
+def test_resident_after_sorted_ragged_force(pkg, orc, planets):
+    """An ensemble Morton-sorted by the ragged path (resident = 0) and then
+    switched to the resident kernel goes back to the caller's order first:
+    the resident kernel takes the central body by its position in the system,
+    so the FP64 central terms (and the oracle agreement) survive the switch."""
+    steps, dt = 100, 1e-3
+    with make(pkg, planets, "split_central", dt=dt, resident=0, sort_interval=16) as e:
+        e.set_central(1)
+        e.compute_force()                    # sorts every system (ragged path)
+        e.set_tuning(resident=1, sort_interval=16)
+        e.half_kick(+1)
+        e.kdk_steps(steps, use_graph=True)   # resident kernel
+        e.half_kick(-1)
+        d = e.download()
+    off = planets["offsets"]
+    for s in range(len(off) - 1):
+        sl = slice(off[s], off[s + 1])
+        xo, vo, _, _ = orc.run_const_kdk(planets["pos"][sl], planets["vel"][sl],
+                                         planets["mass"][sl], [0.0] * 6, 1.0, t_begin=0.0,
+                                         t_end=steps * dt, dt=dt, between_snapshots=1.0,
+                                         skip_self=True)
+        assert np.max(np.abs(d["pos"][sl] - xo)) < 1e-9
+        assert np.max(np.abs(d["vel"][sl] - vo)) < 1e-8
+
+
 def test_resident_and_ragged_paths_agree(pkg, planets):
     outs = []
     for resident in (1, 0):

@@ -110,6 +110,45 @@ def test_snapshots_analysers_and_stop_file(pkg, orc, tmp_path):
     sim2.close()
 
 
+def test_stop_file_without_host_events(pkg, orc, tmp_path):
+    """A run with no analyser, poststep or restart and one snapshot interval
+    hands stretches of steps to nb_kdk_steps.  A stop file present when the
+    loop starts (here written by the output function at snapshot 0) ends it
+    after one step, as src/run.jl:153-157 does; one that appears during a
+    stretch ends it at the end of that stretch (at most 256 steps)."""
+    pos, vel, mass = pkg.ics.plummer(256, seed=5)
+    soft = [0.02] * 6
+    stop = os.path.join(str(tmp_path), "stop")
+    calls = []
+
+    def out(sim):
+        calls.append(sim.timeinfo.system_time_float)
+        open(stop, "w").close()
+
+    sim = pkg.Simulation(pkg.StarData(pos, Vel=vel, Mass=mass), TimeStep=1e-3, TimeEnd=10.0,
+                         TimeBetweenSnapshots=10.0, ForceSofteningTable=soft, output_func=out,
+                         OutputDir=str(tmp_path))
+    with pytest.warns(UserWarning, match="Interrupted by user"):
+        pkg.run(sim)
+    assert calls == [0.0] and sim.timeinfo.stepcount == 1
+    sim.close()
+
+    # the file appears while a stretch runs: a thread writes it after the
+    # loop has started; the run ends within one stretch of 256 steps
+    import threading
+    os.remove(stop)
+    sim = pkg.Simulation(pkg.StarData(pos, Vel=vel, Mass=mass), TimeStep=1e-3, TimeEnd=100.0,
+                         TimeBetweenSnapshots=100.0, ForceSofteningTable=soft,
+                         output_func=lambda s: threading.Timer(0.2, lambda: open(stop, "w").close()).start(),
+                         OutputDir=str(tmp_path))
+    with pytest.warns(UserWarning, match="Interrupted by user"):
+        pkg.run(sim)
+    n = sim.timeinfo.stepcount
+    assert 1 <= n < orc.count_steps(0.0, 100.0, 1e-3)
+    print(f"stop file during a stretch: run ended after {n} steps")
+    sim.close()
+
+
 def test_public_api_pieces(pkg, orc):
     pos, vel, mass = pkg.ics.plummer(800, seed=4)
     soft = [0.01] * 6
@@ -253,10 +292,12 @@ def test_gadget2_snapshots_match_oracle(pkg, orc, format2):
 
 
 @pytest.mark.parametrize("adaptive", [False, True])
-def test_restart_resumes_bit_identical(pkg, tmp_path, adaptive):
+def test_restart_resumes_bit_identical(pkg, orc, tmp_path, adaptive):
     """saverestart inside run (src/run.jl:138-141), loadrestart
     (src/restart.jl:127-136) and restart(sim) (src/run.jl:186-189): in FP64 the
-    resumed run ends in exactly the state of the uninterrupted one."""
+    resumed run ends in exactly the state of the uninterrupted one, and both
+    end where the oracle's uninterrupted run loop (constant or block steps)
+    ends on the same inputs."""
     import shutil
     pos, vel, mass = pkg.ics.plummer(700, seed=41)
     soft = [0.03] * 6
@@ -291,3 +332,17 @@ def test_restart_resumes_bit_identical(pkg, tmp_path, adaptive):
         assert np.array_equal(d.Ti_endstep, ref.Ti_endstep)
         assert np.array_equal(d.Ti_begstep, ref.Ti_begstep)
     sim2.close()
+    # the resumed run against the FP64 oracle, not only against itself
+    if adaptive:
+        xo, vo, _, info = orc.run_adaptive(pos, vel, mass, soft, 1.0, t_begin=0.0, t_end=0.04,
+                                           time_base=1 << 14, err_tol=1e-3, between_snapshots=0.01)
+        assert np.array_equal(d.Ti_endstep, info["ti_end"])
+        assert np.array_equal(d.Ti_begstep, info["ti_beg"])
+    else:
+        xo, vo, _, info = orc.run_const_kdk(pos, vel, mass, soft, 1.0, t_begin=0.0, t_end=0.04,
+                                            dt=2e-3, between_snapshots=0.01)
+    assert sim2.timeinfo.stepcount == info["steps"]
+    print(f"restart ({'adaptive' if adaptive else 'constant'}): max |dx| vs oracle "
+          f"{np.max(np.abs(d.Pos - xo)):.3e}, max |dv| {np.max(np.abs(d.Vel - vo)):.3e}")
+    assert np.max(np.abs(d.Pos - xo)) < 1e-11
+    assert np.max(np.abs(d.Vel - vo)) < 1e-9

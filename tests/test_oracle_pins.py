@@ -221,11 +221,19 @@ def test_plummer_virial_and_potential(pkg, orc):
 
 
 def test_self_potential_term(pkg, orc):
-    """public compute_potential(sim) adds G m_i / (h_i/2.8) (src/base/Potential.jl:5-24)."""
+    """public compute_potential(sim) adds G m_i / (h_i/2.8) (src/base/Potential.jl:5-24):
+    the constant is minus the spline's centre value, -W_p(0) = 14/5, taken here
+    from the exact-fraction kernel (Springel 2005) at r -> 0, not from the
+    oracle's literal."""
+    h, G = 0.05, 2.0
+    wp0 = float(spline_pot_exact(mp.mpf(h) * mp.mpf("1e-30"), 1, h) * h)   # W_p(0+)
+    assert wp0 == pytest.approx(-2.8, rel=1e-15)
     pos, _, mass = pkg.ics.plummer(300, seed=4)
-    p0 = orc.compute_potential(pos, mass, [0.05] * 6, 2.0)
-    p1 = orc.compute_potential(pos, mass, [0.05] * 6, 2.0, self_term=True)
-    assert np.allclose(p1 - p0, 2.0 * mass / (0.05 / 2.8), rtol=1e-12)
+    p0 = orc.compute_potential(pos, mass, [h] * 6, G)
+    p1 = orc.compute_potential(pos, mass, [h] * 6, G, self_term=True)
+    assert np.allclose(p1 - p0, G * mass * (-wp0) / h, rtol=1e-12)
+    # the other branch's constant, 16/5, would fail
+    assert not np.allclose(p1 - p0, G * mass * 3.2 / h, rtol=1e-3)
 
 
 def test_active_filter_and_ranges(pkg, orc):
@@ -402,6 +410,90 @@ def test_adaptive_run_binary_energy(pkg, orc):
     assert abs(E1 - E0) / abs(E0) < 1e-3
     assert info["steps"] > 50 and info["outputs"][-1] == pytest.approx(2.0)
     assert np.all(info["ti_end"] == 1 << 20)
+
+
+def test_adaptive_reduces_to_constant_kdk(pkg, orc):
+    """With MaxStep forcing every particle onto one power-of-two step (the
+    timestep criterion never binds), the block-step loop is the constant-dt
+    KDK loop: every particle is active at every block step, each kick spans
+    (tstart, tend) = one step and the last one half a step
+    (src/base/Timestep.jl:47-82,105-114 against :84-133).  On this
+    power-of-two timeline both loops drift by exactly dt, so positions agree
+    bit for bit; velocities differ only by the rounding of v + a dt - a dt/2
+    against v + a dt/2 at the end.  A kick interval off by half a step in
+    either loop would move the positions by about a dt^2."""
+    pos, vel, mass = pkg.ics.plummer(64, seed=31)
+    soft = [0.05] * 6
+    dt, t_end = 2.0 ** -10, 0.125
+    xc, vc, _, ic = orc.run_const_kdk(pos, vel, mass, soft, 1.0, t_begin=0.0, t_end=t_end, dt=dt)
+    xa, va, _, ia = orc.run_adaptive(pos, vel, mass, soft, 1.0, t_begin=0.0, t_end=t_end,
+                                     time_base=1 << 20, err_tol=1e6, max_step=dt)
+    assert ic["steps"] == ia["steps"] == 128
+    assert ia["nactive"] == [64] * 128
+    assert np.array_equal(xa, xc)
+    assert np.max(np.abs(va - vc)) <= 4e-16 * np.max(np.abs(vc))
+
+
+def compute_dt_pow2(acc, h, err_tol, T, time_base):
+    """The timestep criterion sqrt(2 eta h/|a|) of test/Local-Timestep.jl:9-19,
+    rounded down onto the power-of-two timeline T/2^k (k <= log2 time_base)
+    of the block scheme."""
+    dt = math.sqrt(2.0 * err_tol * h / math.sqrt(float(np.dot(acc, acc))))
+    k = 0
+    while T * 2.0 ** -k > dt and 2 ** k < time_base:
+        k += 1
+    return T * 2.0 ** -k
+
+
+def integrate_adapt_leapfrog(x1, v1, compute_acc, compute_dt, t_end):
+    """test/Local-Timestep.jl:125-164, restated: kick v by a_{i+1} (dt_i +
+    dt_{i+1})/2 between drifts of dt_i, with a forward half kick first and a
+    backward half kick last; dt_{i+1} is clipped to the time left, so the last
+    one is 0 (the run loop's final step, src/base/Timestep.jl:55-60)."""
+    x, v, t = x1.copy(), v1.copy(), 0.0
+    a = compute_acc(x)
+    dt = min(compute_dt(a), t_end)
+    v = v + 0.5 * a * dt
+    steps = []
+    while t < t_end:
+        x = x + v * dt
+        t = t + dt
+        a = compute_acc(x)
+        dt_new = min(compute_dt(a), t_end - t)
+        v = v + a * 0.5 * (dt + dt_new)
+        steps.append(dt)
+        dt = dt_new
+    v = v - 0.5 * a * dt
+    return x, v, steps
+
+
+def test_adaptive_matches_variable_step_hand_integrator(orc):
+    """One test particle on an eccentric Kepler orbit in a point-mass
+    background field: the block-step run loop against the reference's own
+    adaptive leapfrog hand integrator (test/Local-Timestep.jl:125-164,
+    compared with run(sim) at :618-680), both taking the acceleration from
+    the same closed form.  The step changes between 2^-11 and 2^-8 of T along
+    the orbit (pericentre near t = 1.2), so a wrong kick interval (tstart/tend of
+    src/base/Timestep.jl:64-72) cannot pass."""
+    GM, h, eta, T, tb = 1.0, 0.01, 0.01, 2.0, 1 << 20
+
+    def field(p):
+        p = np.atleast_2d(p)
+        r = np.linalg.norm(p, axis=1)
+        return -GM * p / r[:, None] ** 3
+
+    pos = np.array([[1.0, 0.0, 0.0]])
+    vel = np.array([[0.0, 0.4, 0.03]])
+    mass = np.array([1e-12])
+    xa, va, _, info = orc.run_adaptive(pos, vel, mass, [h] * 6, 1.0, t_begin=0.0, t_end=T,
+                                       time_base=tb, err_tol=eta,
+                                       post=lambda p, a: a + field(p))
+    xh, vh, steps = integrate_adapt_leapfrog(pos[0], vel[0], lambda x: field(x)[0],
+                                             lambda a: compute_dt_pow2(a, h, eta, T, tb), T)
+    assert info["steps"] == len(steps)
+    assert len(set(steps)) >= 3                   # the step does change along the orbit
+    assert np.max(np.abs(xa[0] - xh)) < 1e-12
+    assert np.max(np.abs(va[0] - vh)) < 1e-11
 
 
 def test_mond_nu_inverts_the_mu_functions(orc):
