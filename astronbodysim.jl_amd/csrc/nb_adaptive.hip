@@ -482,7 +482,12 @@ int nb_adaptive_compute_force(nb_ctx *c, int64_t t_int, int64_t *nactive) {
         a.tile_box = c->tile_box; a.use_origin = 0;
         a.nseg = nseg; a.seg_len = tiles_per_seg * BS; a.nblk_i = (int)nblk; a.G = c->G;
         a.ax = gax; a.ay = gay; a.az = gaz; a.pot = nullptr; a.part = c->part;
-        NB_TRY(launch_force_f32(a, 1, false, s));
+        const int fp = c->tun.far_path == FAR_MFMA_BF16 ? FAR_MFMA_BF16 : FAR_VALU;
+        if (fp == FAR_MFMA_BF16) {
+            NB_TRY(launch_source_weights(c->m, false, n, c->wsrc, s));
+            a.sw = c->wsrc;
+        }
+        NB_TRY(launch_force_f32(a, 1, false, s, true, fp));
         if (nseg > 1) NB_TRY(launch_finalize(c->part, nseg, na, c->G, gax, gay, gaz, nullptr, false, s));
     }
     // post-processing of the active targets, in the order of

@@ -51,8 +51,7 @@ namespace {
 int default_tpt(int64_t) { return 1; }
 
 int default_far_path(int mode, int64_t n) {
-    const bool split = mode == NB_MODE_SPLIT || mode == NB_MODE_SPLIT_CENTRAL;
-    return split && n >= (1 << 16) ? FAR_MFMA_BF16 : FAR_VALU;
+    return mode != NB_MODE_FP64 && n >= (1 << 16) ? FAR_MFMA_BF16 : FAR_VALU;
 }
 
 int default_segments(int64_t n, int tpt, int far_path) {
@@ -238,7 +237,12 @@ int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = tru
         a.part = c->part;
         a.skip_src = nullptr;
         a.accumulate = 0;
-        NB_TRY(launch_force_f32(a, tpt, with_pot, s, !(with_pot && pot_only)));
+        const bool acc_out = !(with_pot && pot_only);
+        if ((!with_pot || !acc_out) && c->tun.far_path == FAR_MFMA_BF16) {
+            NB_TRY(launch_source_weights(c->m, false, n, c->wsrc, s));
+            a.sw = c->wsrc;
+        }
+        NB_TRY(launch_force_f32(a, tpt, with_pot, s, acc_out, c->tun.far_path));
     }
     if (allow_timing) NB_TRY(record_stop(c, e0));
     if (g.nseg > 1)
@@ -903,6 +907,10 @@ int nb_set_tuning(nb_ctx *c, const nb_tuning *t) {
         t->j_segments < 1 || t->j_segments > 256 || t->sort_interval < 0 || t->far_path < 0 ||
         t->far_path > 4) {
         set_error("invalid tuning");
+        return NB_ERR_INVALID;
+    }
+    if (c->mode == NB_MODE_FP32 && t->far_path != FAR_VALU && t->far_path != FAR_MFMA_BF16) {
+        set_error("the FP32 mode takes far_path 0 or 4 (got %d)", t->far_path);
         return NB_ERR_INVALID;
     }
     // the matrix-core far paths hold one target per lane (as nb_tuner_load requires)

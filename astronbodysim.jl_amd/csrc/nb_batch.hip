@@ -300,116 +300,240 @@ struct ResArgs {
     int energy_slots;
 };
 
-template <bool ALL_F64>
-__global__ __launch_bounds__(256) void k_batch_resident(ResArgs a, int nsteps) {
-    __shared__ double sx[256], sy[256], sz[256], sm[256];
-    __shared__ float4 shi[256], slo[256];
-    __shared__ int sidx[256];
-    __shared__ int wave_cnt[4];
-    __shared__ double red[4];
-    const int s = blockIdx.x;
-    const int t = threadIdx.x;
-    const int64_t o0 = a.off[s], o1 = a.off[s + 1];
-    const int ns = (int)(o1 - o0);
-    const bool act = t < ns;
-    const int64_t i = o0 + t;
-    double X = 0, Y = 0, Z = 0, VX = 0, VY = 0, VZ = 0, AX = 0, AY = 0, AZ = 0, M = 0;
-    if (act) {
-        X = a.x[i]; Y = a.y[i]; Z = a.z[i];
-        VX = a.vx[i]; VY = a.vy[i]; VZ = a.vz[i];
-        AX = a.ax[i]; AY = a.ay[i]; AZ = a.az[i];
-        M = a.m[i];
+__device__ __forceinline__ float readlane_f32(float v, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// 1/sqrt(x) in FP64 for x > 0: v_rsq_f64 (relative error up to about 2^-23)
+// refined by two Newton steps y <- y + y (1 - x y^2) / 2, each of which
+// squares the relative error (times 3/2), so the result is within a few ulp.
+// Replaces the sqrt and divide of the reference's m / (r^2 r) for the
+// central-body terms.
+__device__ __forceinline__ double rsq_f64(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const double e = __builtin_fma(-x * y, y, 1.0);
+        y = __builtin_fma(0.5 * y, e, y);
     }
-    const double G = a.Gs[s], dt = a.dts[s], hsys = a.hs[s];
-    const double hinv = 1.0 / hsys, hinv3 = hinv * hinv * hinv;
+    return y;
+}
+
+// FP32 hi/lo pair of the resident kernel: source (sh, sl, sm) taken from
+// scalar registers, target k of the lane.  SOFT: the spline inside h;
+// otherwise Newtonian with the self pair (r = 0) contributing nothing.
+template <bool SOFT>
+__device__ __forceinline__ void res_pair_f32(float sxh, float syh, float szh, float sxl, float syl,
+                                             float szl, float smf, float xh, float yh, float zh,
+                                             float xl, float yl, float zl, float h2cf, float h2f,
+                                             float hinvf, float hinv3f, float &fx, float &fy,
+                                             float &fz, float &fq) {
+    if constexpr (SOFT) {
+        pair_f32_hilo<true, false>(sxh, syh, szh, smf, sxl, syl, szl, xh, yh, zh, xl, yl, zl, h2cf,
+                                   h2f, hinvf, hinv3f, fx, fy, fz, fq);
+    } else {
+        const float dx = (sxh - xh) + (sxl - xl);
+        const float dy = (syh - yh) + (syl - yl);
+        const float dz = (szh - zh) + (szl - zl);
+        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+        const float ri = __builtin_amdgcn_rsqf(r2);
+        const float fac = r2 > 0.0f ? smf * (ri * ri) * ri : 0.0f;
+        fx = __builtin_fmaf(fac, dx, fx);
+        fy = __builtin_fmaf(fac, dy, fy);
+        fz = __builtin_fmaf(fac, dz, fz);
+    }
+}
+
+// State of one resident system: lane l holds particles l, l + 64, ...
+template <int NSLOT>
+struct ResState {
+    double X[NSLOT], Y[NSLOT], Z[NSLOT], VX[NSLOT], VY[NSLOT], VZ[NSLOT], AX[NSLOT], AY[NSLOT],
+        AZ[NSLOT], M[NSLOT];
+    bool act[NSLOT];
+    uint64_t mmask[NSLOT];   // massive particles (sources)
+    uint64_t cmask[NSLOT];   // of which the central bodies (FP64 terms)
+};
+
+// nsteps KDK steps of one system; SOFT (h > 0) is wave-uniform and chosen
+// outside the step loop, so the source loops carry no branches.
+template <int NSLOT, bool SOFT>
+__device__ __forceinline__ void resident_run(const ResArgs &a, ResState<NSLOT> &r, int s, int lane,
+                                             int nsteps, double G, double dt, double hsys) {
+    const double hinv = SOFT ? 1.0 / hsys : 0.0, hinv3 = hinv * hinv * hinv;
     const float hf = (float)hsys, h2f = hf * hf, h2cf = fmaxf(h2f, FLT_MIN);
-    const float hinvf = 1.0f / hf, hinv3f = hinvf * hinvf * hinvf;
-    // compact the massive sources (deterministic: local index order)
-    const bool massive = act && M != 0.0;
-    const unsigned long long bal = __ballot(massive);
-    const int lane = t & 63, w = t >> 6;
-    const int before = __popcll(bal & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
-    if (lane == 0) wave_cnt[w] = __popcll(bal);
-    __syncthreads();
-    int wbase = 0;
-    for (int q = 0; q < w; ++q) wbase += wave_cnt[q];
-    const int nmass = wave_cnt[0] + wave_cnt[1] + wave_cnt[2] + wave_cnt[3];
-    const int slot = wbase + before;
-    if (massive) {
-        sidx[slot] = t;
-        sm[slot] = M;
-    }
+    const float hinvf = SOFT ? 1.0f / hf : 0.0f, hinv3f = hinvf * hinvf * hinvf;
     int e_slot = 0;
     for (int step = 0; step < nsteps; ++step) {
         // drift (FP64)
-        X += VX * dt; Y += VY * dt; Z += VZ * dt;
-        __syncthreads();   // previous step's readers of the source arrays are done
-        if (massive) { sx[slot] = X; sy[slot] = Y; sz[slot] = Z; }
-        __syncthreads();
-        // origin: the system's first particle (the central body of a planetary system)
-        const double ox = nmass > 0 ? sx[0] : 0.0;
-        const double oy = nmass > 0 ? sy[0] : 0.0, oz = nmass > 0 ? sz[0] : 0.0;
-        if (!ALL_F64 && massive) {
-            const double dx = X - ox, dy = Y - oy, dz = Z - oz;
-            const float hx = (float)dx, hy = (float)dy, hz = (float)dz;
-            shi[slot] = make_float4(hx, hy, hz, (float)M);
-            slo[slot] = make_float4((float)(dx - (double)hx), (float)(dy - (double)hy),
-                                    (float)(dz - (double)hz), 0.0f);
+#pragma unroll
+        for (int k = 0; k < NSLOT; ++k) {
+            r.X[k] += r.VX[k] * dt; r.Y[k] += r.VY[k] * dt; r.Z[k] += r.VZ[k] * dt;
         }
-        __syncthreads();
-        double Ax = 0.0, Ay = 0.0, Az = 0.0, Aq = 0.0;
-        float fx = 0.0f, fy = 0.0f, fz = 0.0f, fq = 0.0f;
-        if (act) {
-            const double dxi = X - ox, dyi = Y - oy, dzi = Z - oz;
-            const float xih = (float)dxi, yih = (float)dyi, zih = (float)dzi;
-            const float xil = (float)(dxi - (double)xih), yil = (float)(dyi - (double)yih),
-                        zil = (float)(dzi - (double)zih);
-            for (int k = 0; k < nmass; ++k) {
-                const int j = sidx[k];
-                if (j == t) continue;
-                if (ALL_F64 || j < a.ncentral) {
-                    pair_f64<false>(sx[k], sy[k], sz[k], sm[k], X, Y, Z, hsys, hinv, hinv3, Ax, Ay,
-                                    Az, Aq);
-                } else {
-                    const float4 v = shi[k], l = slo[k];
-                    pair_f32_hilo<true, false>(v.x, v.y, v.z, v.w, l.x, l.y, l.z, xih, yih, zih, xil,
-                                         yil, zil, h2cf, h2f, hinvf, hinv3f, fx, fy, fz, fq);
+        // origin: the system's first particle (the central body of a planetary system)
+        const double ox = readlane_f64(r.X[0], 0), oy = readlane_f64(r.Y[0], 0), oz = readlane_f64(r.Z[0], 0);
+        float xh[NSLOT], yh[NSLOT], zh[NSLOT], xl[NSLOT], yl[NSLOT], zl[NSLOT], mf[NSLOT];
+        float fx[NSLOT], fy[NSLOT], fz[NSLOT], fq[NSLOT];
+        double Ax[NSLOT], Ay[NSLOT], Az[NSLOT], Aq[NSLOT];
+#pragma unroll
+        for (int k = 0; k < NSLOT; ++k) {
+            const double dx = r.X[k] - ox, dy = r.Y[k] - oy, dz = r.Z[k] - oz;
+            xh[k] = (float)dx; yh[k] = (float)dy; zh[k] = (float)dz;
+            xl[k] = (float)(dx - (double)xh[k]);
+            yl[k] = (float)(dy - (double)yh[k]);
+            zl[k] = (float)(dz - (double)zh[k]);
+            mf[k] = (float)r.M[k];
+            fx[k] = fy[k] = fz[k] = fq[k] = 0.0f;
+            Ax[k] = Ay[k] = Az[k] = Aq[k] = 0.0;
+        }
+#pragma unroll
+        for (int ks = 0; ks < NSLOT; ++ks) {
+            // central bodies: FP64 terms
+            uint64_t mc = r.cmask[ks];
+            while (mc) {
+                const int j = (int)__builtin_ctzll(mc);
+                mc &= mc - 1;
+                const double sx = readlane_f64(r.X[ks], j), sy = readlane_f64(r.Y[ks], j);
+                const double sz = readlane_f64(r.Z[ks], j), sm = readlane_f64(r.M[ks], j);
+#pragma unroll
+                for (int kt = 0; kt < NSLOT; ++kt) {
+                    if constexpr (SOFT) {
+                        pair_f64<false>(sx, sy, sz, sm, r.X[kt], r.Y[kt], r.Z[kt], hsys, hinv, hinv3,
+                                        Ax[kt], Ay[kt], Az[kt], Aq[kt]);
+                    } else {
+                        const double dx = sx - r.X[kt], dy = sy - r.Y[kt], dz = sz - r.Z[kt];
+                        const double r2 = dx * dx + dy * dy + dz * dz;
+                        const double ri = rsq_f64(r2);
+                        const double fac = r2 > 0.0 ? sm * (ri * ri) * ri : 0.0;
+                        Ax[kt] += dx * fac; Ay[kt] += dy * fac; Az[kt] += dz * fac;
+                    }
+                }
+            }
+            // the other massive bodies: FP32 hi/lo terms, two sources per trip
+            uint64_t mm = r.mmask[ks] & ~r.cmask[ks];
+            while (mm) {
+                const int j0 = (int)__builtin_ctzll(mm);
+                mm &= mm - 1;
+                const int j1 = mm ? (int)__builtin_ctzll(mm) : j0;
+                const bool two = mm != 0;
+                mm &= mm - 1;
+                const float axh = readlane_f32(xh[ks], j0), ayh = readlane_f32(yh[ks], j0);
+                const float azh = readlane_f32(zh[ks], j0), axl = readlane_f32(xl[ks], j0);
+                const float ayl = readlane_f32(yl[ks], j0), azl = readlane_f32(zl[ks], j0);
+                const float am = readlane_f32(mf[ks], j0);
+                const float bxh = readlane_f32(xh[ks], j1), byh = readlane_f32(yh[ks], j1);
+                const float bzh = readlane_f32(zh[ks], j1), bxl = readlane_f32(xl[ks], j1);
+                const float byl = readlane_f32(yl[ks], j1), bzl = readlane_f32(zl[ks], j1);
+                const float bm = two ? readlane_f32(mf[ks], j1) : 0.0f;   // one left: massless copy
+#pragma unroll
+                for (int kt = 0; kt < NSLOT; ++kt) {
+                    res_pair_f32<SOFT>(axh, ayh, azh, axl, ayl, azl, am, xh[kt], yh[kt], zh[kt], xl[kt],
+                                       yl[kt], zl[kt], h2cf, h2f, hinvf, hinv3f, fx[kt], fy[kt], fz[kt],
+                                       fq[kt]);
+                    res_pair_f32<SOFT>(bxh, byh, bzh, bxl, byl, bzl, bm, xh[kt], yh[kt], zh[kt], xl[kt],
+                                       yl[kt], zl[kt], h2cf, h2f, hinvf, hinv3f, fx[kt], fy[kt], fz[kt],
+                                       fq[kt]);
                 }
             }
         }
-        AX = G * (Ax + (double)fx);
-        AY = G * (Ay + (double)fy);
-        AZ = G * (Az + (double)fz);
         // kick (FP64)
-        VX += AX * dt; VY += AY * dt; VZ += AZ * dt;
+#pragma unroll
+        for (int k = 0; k < NSLOT; ++k) {
+            r.AX[k] = G * (Ax[k] + (double)fx[k]);
+            r.AY[k] = G * (Ay[k] + (double)fy[k]);
+            r.AZ[k] = G * (Az[k] + (double)fz[k]);
+            r.VX[k] += r.AX[k] * dt; r.VY[k] += r.AY[k] * dt; r.VZ[k] += r.AZ[k] * dt;
+        }
         if (a.energy && (step + 1) % a.energy_every == 0) {
-            // total energy with velocities synchronised to the step
-            double e = 0.0;
-            if (act) {
-                const double ux = VX - 0.5 * AX * dt, uy = VY - 0.5 * AY * dt, uz = VZ - 0.5 * AZ * dt;
-                e = 0.5 * M * (ux * ux + uy * uy + uz * uz);
-                if (M != 0.0) {
-                    double bx = 0, by = 0, bz = 0, q = 0.0;
-                    for (int k = 0; k < nmass; ++k) {
-                        if (sidx[k] == t) continue;
-                        pair_f64<true>(sx[k], sy[k], sz[k], sm[k], X, Y, Z, hsys, hinv, hinv3, bx,
-                                       by, bz, q);
+            // total energy with velocities synchronised to the step; the
+            // potential in FP64 over the massive sources (the reference's
+            // pair potential, src/directsum/potential.jl:2-27)
+            double q[NSLOT];
+#pragma unroll
+            for (int k = 0; k < NSLOT; ++k) q[k] = 0.0;
+#pragma unroll
+            for (int ks = 0; ks < NSLOT; ++ks) {
+                uint64_t mm = r.mmask[ks];
+                while (mm) {
+                    const int j = (int)__builtin_ctzll(mm);
+                    mm &= mm - 1;
+                    const double sx = readlane_f64(r.X[ks], j), sy = readlane_f64(r.Y[ks], j);
+                    const double sz = readlane_f64(r.Z[ks], j), sm = readlane_f64(r.M[ks], j);
+#pragma unroll
+                    for (int kt = 0; kt < NSLOT; ++kt) {
+                        double bx = 0, by = 0, bz = 0;
+                        if (!(ks == kt && lane == j))
+                            pair_f64<true>(sx, sy, sz, sm, r.X[kt], r.Y[kt], r.Z[kt], hsys, SOFT ? hinv : 0.0,
+                                           SOFT ? hinv3 : 0.0, bx, by, bz, q[kt]);
                     }
-                    e -= 0.5 * G * M * q;
                 }
+            }
+            double e = 0.0;
+#pragma unroll
+            for (int k = 0; k < NSLOT; ++k) {
+                if (!r.act[k]) continue;
+                const double ux = r.VX[k] - 0.5 * r.AX[k] * dt, uy = r.VY[k] - 0.5 * r.AY[k] * dt,
+                             uz = r.VZ[k] - 0.5 * r.AZ[k] * dt;
+                e += 0.5 * r.M[k] * (ux * ux + uy * uy + uz * uz) - 0.5 * G * r.M[k] * q[k];
             }
             e = wave_sum(e);
-            if (lane == 0) red[w] = e;
-            __syncthreads();
-            if (t == 0 && e_slot < a.energy_slots)
-                a.energy[(int64_t)s * a.energy_slots + e_slot] = red[0] + red[1] + red[2] + red[3];
+            if (lane == 0 && e_slot < a.energy_slots)
+                a.energy[(int64_t)s * a.energy_slots + e_slot] = e;
             ++e_slot;
         }
     }
-    if (act) {
-        a.x[i] = X; a.y[i] = Y; a.z[i] = Z;
-        a.vx[i] = VX; a.vy[i] = VY; a.vz[i] = VZ;
-        a.ax[i] = AX; a.ay[i] = AY; a.az[i] = AZ;
+}
+
+// One wavefront per system: lane l holds particles l, l + 64, ... (NSLOT
+// slots), in the caller's order, in registers for the whole launch.  Each
+// massive particle is a source: its coordinates are broadcast from the lane
+// that holds it with v_readlane (no LDS, no barriers).  Pair terms of the
+// central bodies (ALL_F64: of every source) are FP64; the others are FP32
+// hi/lo terms relative to the system's first particle.  Drift and kick are
+// FP64.  Every `energy_every` steps the system's total energy (velocities
+// synchronised to the step, FP64 potential over the massive sources) goes to
+// energy[s * energy_slots + k].
+template <int NSLOT, bool ALL_F64>
+__global__ __launch_bounds__(64) void k_batch_resident(ResArgs a, int nsteps) {
+    const int s = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int64_t o0 = a.off[s];
+    const int ns = (int)(a.off[s + 1] - o0);
+    if (ns <= 0) {   // an empty system has zero energy
+        if (a.energy)
+            for (int k = lane; k < a.energy_slots; k += 64) a.energy[(int64_t)s * a.energy_slots + k] = 0.0;
+        return;
+    }
+    ResState<NSLOT> r;
+    const int ncentral = ALL_F64 ? 1 << 30 : a.ncentral;
+#pragma unroll
+    for (int k = 0; k < NSLOT; ++k) {
+        const int li = lane + 64 * k;
+        const bool act = li < ns;
+        r.act[k] = act;
+        const int64_t i = o0 + li;
+        r.X[k] = act ? a.x[i] : 0.0; r.Y[k] = act ? a.y[i] : 0.0; r.Z[k] = act ? a.z[i] : 0.0;
+        r.VX[k] = act ? a.vx[i] : 0.0; r.VY[k] = act ? a.vy[i] : 0.0; r.VZ[k] = act ? a.vz[i] : 0.0;
+        r.AX[k] = act ? a.ax[i] : 0.0; r.AY[k] = act ? a.ay[i] : 0.0; r.AZ[k] = act ? a.az[i] : 0.0;
+        r.M[k] = act ? a.m[i] : 0.0;
+        r.mmask[k] = __builtin_amdgcn_ballot_w64(act && r.M[k] != 0.0);
+        r.cmask[k] = __builtin_amdgcn_ballot_w64(act && r.M[k] != 0.0 && li < ncentral);
+    }
+    const double G = a.Gs[s], dt = a.dts[s], hsys = a.hs[s];
+    if (hsys > 0.0) resident_run<NSLOT, true>(a, r, s, lane, nsteps, G, dt, hsys);
+    else resident_run<NSLOT, false>(a, r, s, lane, nsteps, G, dt, hsys);
+#pragma unroll
+    for (int k = 0; k < NSLOT; ++k) {
+        if (!r.act[k]) continue;
+        const int64_t i = o0 + lane + 64 * k;
+        a.x[i] = r.X[k]; a.y[i] = r.Y[k]; a.z[i] = r.Z[k];
+        a.vx[i] = r.VX[k]; a.vy[i] = r.VY[k]; a.vz[i] = r.VZ[k];
+        a.ax[i] = r.AX[k]; a.ay[i] = r.AY[k]; a.az[i] = r.AZ[k];
     }
 }
 
@@ -773,10 +897,19 @@ static int resident_steps(nb_batch *b, int64_t nsteps, int every, double *energy
         set_error("too many steps for one resident launch");
         return NB_ERR_INVALID;
     }
-    if (b->mode == NB_MODE_FP64)
-        k_batch_resident<true><<<b->nsys, 256, 0, b->stream>>>(a, (int)nsteps);
-    else
-        k_batch_resident<false><<<b->nsys, 256, 0, b->stream>>>(a, (int)nsteps);
+    if (b->nsys <= 0) return NB_OK;
+    const int nslot = (int)std::max<int64_t>(1, (b->max_ns + 63) / 64);   // resident_ok: <= 4
+    const bool f64 = b->mode == NB_MODE_FP64;
+#define NB_RES(NS_)                                                                     \
+    do {                                                                                \
+        if (f64) k_batch_resident<NS_, true><<<b->nsys, 64, 0, b->stream>>>(a, (int)nsteps); \
+        else k_batch_resident<NS_, false><<<b->nsys, 64, 0, b->stream>>>(a, (int)nsteps);    \
+    } while (0)
+    if (nslot == 1) NB_RES(1);
+    else if (nslot == 2) NB_RES(2);
+    else if (nslot == 3) NB_RES(3);
+    else NB_RES(4);
+#undef NB_RES
     return launch_ok("k_batch_resident");
 }
 

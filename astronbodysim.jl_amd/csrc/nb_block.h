@@ -208,9 +208,13 @@ typedef unsigned nb_u4v __attribute__((ext_vector_type(4)));
 constexpr float FARW_MASSLESS = 1.329228e36f;   // 2^120
 
 // Shared memory of one all-pairs block.
+// HILO: positions relative to the block's FP64 origin, near tiles on hi/lo
+// coordinates.  FP64 state always; FP32 state with the bf16 matrix-core far
+// tiles, whose squared-distance expansion needs origin-relative coordinates
+// (there the hi/lo pair holds the FP32 position minus the FP64 origin exactly).
 template <typename T, int MF = 0>
 struct BlockShared {
-    static constexpr bool HILO = sizeof(T) == 8;
+    static constexpr bool HILO = sizeof(T) == 8 || MF == FAR_MFMA_BF16;
     float4 sj[2][BS];
     float4 sjl[HILO ? 2 : 1][HILO ? BS : 1];
     static constexpr bool COLS = MF == 1 || MF == 2;   // the MFMA far paths

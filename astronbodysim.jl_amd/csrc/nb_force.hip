@@ -20,9 +20,10 @@
 namespace nb {
 
 // FP32-pair kernel.  T = double (SPLIT: FP64 state, per-block origin, hi/lo
-// coordinates on near tiles) or float (FP32 state, absolute coordinates).
-// MF selects the matrix-core far tiles (SPLIT, TPT 1, forces or potentials
-// alone).  ACC = false writes the potential only.
+// coordinates on near tiles) or float (FP32 state: absolute coordinates, or,
+// with the bf16 far tiles, the SPLIT layout of the FP32 positions).  MF
+// selects the matrix-core far tiles (TPT 1, forces or potentials alone).
+// ACC = false writes the potential only.
 template <typename T, int TPT, bool POT, int MF = 0, bool ACC = true>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MF == 4 ? 4 : 3))) void k_force_f32(ForceArgs<T> a) {
     __shared__ BlockShared<T, MF> sh;
@@ -219,8 +220,25 @@ static int launch_f32_impl(const ForceArgs<T> &a, int tpt, bool pot, hipStream_t
     return check_launch("k_force_f32");
 }
 
-int launch_force_f32(const ForceArgs<float> &a, int tpt, bool pot, hipStream_t s, bool acc) {
-    return launch_f32_impl<float, FAR_VALU>(a, tpt, pot, s, acc || !pot);
+int launch_force_f32(const ForceArgs<float> &a, int tpt, bool pot, hipStream_t s, bool acc,
+                     int far_path) {
+    if (!acc && !pot) acc = true;
+    if (far_path == FAR_MFMA_BF16 && tpt == 1 && (!pot || !acc)) {
+        // FP32 state through the per-block FP64 origin: hi/lo near tiles, bf16
+        // matrix-core squared distances on the far tiles (forces or potential alone)
+        const int64_t nb = (int64_t)a.nblk_i * a.nseg;
+        if (nb <= 0) return NB_OK;
+        if (nb > INT32_MAX) {
+            set_error("force grid too large (%lld blocks)", (long long)nb);
+            return NB_ERR_INVALID;
+        }
+        ForceArgs<float> b = a;
+        b.use_origin = 1;
+        if (pot) k_force_f32<float, 1, true, FAR_MFMA_BF16, false><<<(unsigned)nb, BS, 0, s>>>(b);
+        else k_force_f32<float, 1, false, FAR_MFMA_BF16><<<(unsigned)nb, BS, 0, s>>>(b);
+        return check_launch("k_force_f32<float, bf16 mfma>");
+    }
+    return launch_f32_impl<float, FAR_VALU>(a, tpt, pot, s, acc);
 }
 int launch_force_split(const ForceArgs<double> &a, int tpt, bool pot, hipStream_t s, int far_path,
                        bool acc) {

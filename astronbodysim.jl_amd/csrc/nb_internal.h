@@ -65,7 +65,10 @@ constexpr int FAR_MFMA_BF16 = 4;  // bf16-split r^2 on the matrix cores, sum on 
 int launch_tile_boxes(const void *x, const void *y, const void *z, int is_double, int64_t n,
                       double *box, hipStream_t s);
 // acc = false (with pot) writes the potential alone.
-int launch_force_f32(const ForceArgs<float> &a, int tpt, bool pot, hipStream_t s, bool acc = true);
+// far_path FAR_MFMA_BF16 (tpt 1, forces or potential alone) runs the FP32
+// state relative to a per-block FP64 origin; any other value is far path 0.
+int launch_force_f32(const ForceArgs<float> &a, int tpt, bool pot, hipStream_t s, bool acc = true,
+                     int far_path = FAR_VALU);
 int launch_force_split(const ForceArgs<double> &a, int tpt, bool pot, hipStream_t s,
                        int far_path = 0, bool acc = true);
 int launch_force_f64(const ForceArgs<double> &a, bool pot, hipStream_t s);

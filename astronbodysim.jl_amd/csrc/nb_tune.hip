@@ -92,11 +92,13 @@ int autotune_ctx(nb_ctx *c) {
     nb_tuning best = keep;
     float best_ms = 1e30f;
     // A single evaluation is timed per candidate when it is long.
-    // candidates (targets per thread, far path); the matrix-core and weighted
-    // far paths exist in the SPLIT modes only
-    const int cand[9][2] = {{1, 0}, {2, 0}, {4, 0}, {1, 3}, {2, 3}, {4, 3}, {1, 4}, {1, 1}, {1, 2}};
+    // candidates (targets per thread, far path); the weighted and FP32
+    // matrix-core far paths exist in the SPLIT modes only, the bf16 one in
+    // the SPLIT and FP32 modes
+    const int cand[9][2] = {{1, 0}, {2, 0}, {4, 0}, {1, 4}, {1, 3}, {2, 3}, {4, 3}, {1, 1}, {1, 2}};
     const bool split = c->mode == NB_MODE_SPLIT || c->mode == NB_MODE_SPLIT_CENTRAL;
-    for (int ti = 0; ti < (split ? 9 : 3); ++ti) {
+    const int ncand = split ? 9 : (c->mode == NB_MODE_FP32 ? 4 : 3);
+    for (int ti = 0; ti < ncand; ++ti) {
         const int tpt = cand[ti][0];
         const int mf = cand[ti][1];
         if (c->mode == NB_MODE_FP64 && tpt != 1) continue;

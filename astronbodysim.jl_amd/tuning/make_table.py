@@ -2,11 +2,13 @@
 (tuning/mi355x.json): autotune one seeded system per size and precision mode
 and save what the autotuner measured.
 
-    python -m astronbodysim.jl_amd.tuning.make_table [out.json]
+    python -m astronbodysim.jl_amd.tuning.make_table [out.json] [--modes=split,fp32,fp64]
 
 (or run this file directly).  Sizes 2^12 .. 2^20 in the split mode on
 two-Plummer collisions (the benchmark's geometry), and 2^12 .. 2^16 in the
-fp32 and fp64 modes on Plummer spheres.  Needs the GPU; a few minutes.
+fp32 and fp64 modes on Plummer spheres.  With --modes= only those modes are
+measured again and the table's other entries are kept.  Needs the GPU; a
+few minutes.
 """
 from __future__ import annotations
 
@@ -23,13 +25,16 @@ def _package():
     return ge.load_package()
 
 
-def main(out=None):
+def main(out=None, modes=None):
     pkg = _package()
     nat = pkg._native
     out = out or str(nat.DEFAULT_TUNING)
     nat.tuner_clear()
+    if modes:    # keep the entries of the modes not measured again
+        nat.tuner_load(nat.DEFAULT_TUNING)
     plan = [("split", k) for k in range(12, 21)] + [("fp32", k) for k in range(12, 17)] + \
            [("fp64", k) for k in range(12, 16)]
+    plan = [(m, k) for m, k in plan if not modes or m in modes]
     for mode, k in plan:
         n = 1 << k
         if mode == "split":
@@ -50,4 +55,6 @@ def main(out=None):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else None)
+    args = [a for a in sys.argv[1:] if not a.startswith("--modes")]
+    sel = [a.split("=", 1)[1] for a in sys.argv[1:] if a.startswith("--modes=")]
+    main(args[0] if args else None, sel[0].split(",") if sel else None)

@@ -28,6 +28,7 @@ sys.path.insert(0, str(ROOT))
 METRIC = "pair interactions/s at force rel err ≤1e-6 vs FP64; batched vs plain speedup"
 FLOPS_PER_PAIR = 20
 PEAK_FP32_TFLOPS = 157.3          # MI355X FP32 vector, published (packed)
+PEAK_FP64_TFLOPS = 78.6           # MI355X FP64 vector, published
 
 
 def parse():
@@ -69,6 +70,122 @@ def progress(msg):
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
+def _force_timing(ctx, reps=5):
+    """Median HIP-event duration (ms) of `reps` separately timed force
+    evaluations on the context's stream."""
+    ctx.set_timing(True)
+    ctx.compute_force()
+    ctx.sync()
+    out = []
+    for _ in range(reps):
+        c0, m0 = ctx.get_timing()
+        ctx.compute_force()
+        ctx.sync()
+        c1, m1 = ctx.get_timing()
+        out.append((m1 - m0) / max(c1 - c0, 1))
+    ctx.set_timing(False)
+    return float(np.median(out))
+
+
+def bench_c1(pkg, nat, args):
+    """Plummer N = 4096, eps = 0.01, FP64, 100 KDK steps at dt = 1e-3: the
+    reference's own CPU-runnable case.  Both the FP64 mode and split precision
+    run the 100 steps (graph-captured), and the oracle's run loop
+    (src/run.jl:46-184 restated) runs them on the host for the comparison."""
+    from oracle import oracle as orc
+    n, h, dt, steps = 4096, 0.01, 1e-3, 100
+    pos, vel, mass = pkg.ics.plummer(n, seed=1)
+    t0 = time.perf_counter()
+    xo, vo, _, info = orc.run_const_kdk(pos, vel, mass, [h] * 6, 1.0, t_begin=0.0,
+                                        t_end=steps * dt, dt=dt)
+    t_cpu = time.perf_counter() - t0
+    E0o = sum(orc.energy(pos, vel, mass, orc.compute_potential(pos, mass, [h] * 6))[:2])
+    E1o = sum(orc.energy(xo, vo, mass, orc.compute_potential(xo, mass, [h] * 6))[:2])
+    out = {"workload": "Plummer N=4096, eps=0.01, 100 KDK steps at dt=1e-3",
+           "steps": info["steps"], "oracle": {"cpu_s": t_cpu, "cores": orc.num_threads(),
+                                              "pairs_per_s": float(n) * n * steps / t_cpu,
+                                              "rel_energy_drift": abs(E1o - E0o) / abs(E0o)}}
+    for mode in ("fp64", "split"):
+        ctx = nat.Context(n, nat.MODES[mode])
+        ctx.set_params(1.0, [h] * 6)
+        ctx.upload(pos, vel, mass)
+        e0 = ctx.energy()
+        force_ms = _force_timing(ctx)
+        # the 100 steps from the initial state (checked against the oracle) ...
+        ctx.compute_force()
+        ctx.half_kick(+1, dt)
+        ctx.kdk_steps(steps, dt, 0.0, use_graph=True)
+        ctx.half_kick(-1, dt)
+        e1 = ctx.energy()
+        d = ctx.download(pos=True, vel=True, acc=False)
+        # ... then 100 more, timed, replaying the graphs the first run built
+        ctx.half_kick(+1, dt)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.kdk_steps(steps, dt, steps * dt, use_graph=True)
+        ctx.sync()
+        t_gpu = time.perf_counter() - t0
+        ctx.close()
+        E0, E1 = e0["K"] + e0["W"], e1["K"] + e1["W"]
+        out[mode] = {"s": t_gpu, "ms_per_step": t_gpu / steps * 1e3,
+                     "pairs_per_s": float(n) * n * steps / t_gpu,
+                     "force_ms": force_ms,
+                     "force_frac_of_peak": FLOPS_PER_PAIR * float(n) * n / (force_ms * 1e-3) / 1e12
+                                           / (PEAK_FP64_TFLOPS if mode == "fp64" else PEAK_FP32_TFLOPS),
+                     "peak_tflops": PEAK_FP64_TFLOPS if mode == "fp64" else PEAK_FP32_TFLOPS,
+                     "max_abs_dx_vs_oracle": float(np.max(np.abs(d["pos"] - xo))),
+                     "rel_energy_drift": abs(E1 - E0) / abs(E0),
+                     "drift_minus_oracle": abs((E1 - E0) - (E1o - E0o)) / abs(E0o)}
+    return out
+
+
+def bench_c2(pkg, nat, args):
+    """Plummer N = 65536, eps = 0.005, FP32 state (the reference's floattype =
+    Float32 GPU mode): one force evaluation (HIP events, median of 5) and 50
+    KDK steps at dt = 5e-4; forces of a seeded target sample against the FP64
+    oracle on the FP32-rounded inputs; energy drift (nb_energy)."""
+    n, h, dt, steps = 65536, 0.005, 5e-4, 50
+    pos, vel, mass = pkg.ics.plummer(n, seed=2)
+    ctx = nat.Context(n, nat.NB_MODE_FP32)
+    ctx.set_params(1.0, [h] * 6)
+    ctx.upload(pos, vel, mass)
+    tun = ctx.get_tuning()
+    e0 = ctx.energy()
+    force_ms = _force_timing(ctx)
+    ctx.compute_force()
+    acc = ctx.download(pos=False, vel=False)["acc"]
+    ctx.half_kick(+1, dt)
+    ctx.kdk_steps(steps, dt, 0.0, use_graph=True)
+    ctx.half_kick(-1, dt)
+    e1 = ctx.energy()
+    # 50 more steps, timed, replaying the graphs the first 50 built
+    ctx.half_kick(+1, dt)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.kdk_steps(steps, dt, steps * dt, use_graph=True)
+    ctx.sync()
+    t_gpu = time.perf_counter() - t0
+    ctx.close()
+    pairs = float(n) * n
+    out = {"workload": "Plummer N=65536, eps=0.005, FP32 state, 1 force + 50 KDK steps at dt=5e-4",
+           "tuning": tun, "force_ms": force_ms, "force_pairs_per_s": pairs / (force_ms * 1e-3),
+           "force_frac_of_peak": FLOPS_PER_PAIR * pairs / (force_ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS,
+           "steps": steps, "s": t_gpu, "ms_per_step": t_gpu / steps * 1e3,
+           "pairs_per_s": pairs * steps / t_gpu,
+           "rel_energy_drift": abs((e1["K"] + e1["W"]) - (e0["K"] + e0["W"])) / abs(e0["K"] + e0["W"])}
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        p32 = pos.astype(np.float32).astype(np.float64)
+        m32 = mass.astype(np.float32).astype(np.float64)
+        rng = np.random.default_rng(22)
+        sample = np.sort(rng.choice(n, 2048, replace=False))
+        ref = orc.force_at_points(p32[sample], h, p32, m32, variant="simd")
+        e = orc.rel_err(acc[sample], ref)
+        out["force_rel_err"] = {"median": float(np.median(e)), "p99": float(np.percentile(e, 99)),
+                                "max": float(e.max()), "targets": int(sample.size)}
+    return out
+
+
 def bench_clusters(pkg, nat, args):
     """512 Plummer clusters, N log-uniform in [256, 16384], a ~ U[0.5, 2],
     h = 0.01 a, 200 KDK steps at dt = 1e-3: one ragged persistent launch per
@@ -107,11 +224,10 @@ def bench_clusters(pkg, nat, args):
         err = {"median": float(np.median(es)), "max": float(es.max()), "clusters": 6}
     except Exception as exc:  # pragma: no cover - reported, not fatal
         err = {"error": str(exc)}
-    # sequential plain launches (one context per cluster), measured on a
-    # seeded subset and scaled by the subset's share of the pairs
-    rng = np.random.default_rng(11)
-    subset = np.sort(rng.choice(512, 64, replace=False))
-    sub_steps = 20
+    # sequential plain launches: one context per cluster, every cluster, the
+    # same 200 graph-captured steps
+    subset = np.arange(512)
+    sub_steps = steps
     ctxs = []
     for s in subset:
         sl = slice(off[s], off[s + 1])
@@ -130,15 +246,15 @@ def bench_clusters(pkg, nat, args):
     t_sub = time.perf_counter() - t0
     for c in ctxs:
         c.close()
-    sub_pairs = float((ns[subset].astype(np.float64) ** 2).sum())
-    t_plain = t_sub / sub_steps * steps * (pairs / sub_pairs)
+    t_plain = t_sub
     return {"workload": "512 Plummer clusters, N logU[256,16384], 200 KDK steps",
             "n_total": int(off[-1]), "pairs_per_step": pairs, "steps": steps,
             "batched_s": t_batch, "batched_pairs_per_s": pairs * steps / t_batch,
-            "plain_s_estimate": t_plain, "plain_pairs_per_s": pairs * steps / t_plain,
+            "batched_frac_of_peak": FLOPS_PER_PAIR * pairs * steps / t_batch / 1e12 / PEAK_FP32_TFLOPS,
+            "plain_s": t_plain, "plain_pairs_per_s": pairs * steps / t_plain,
             "speedup": t_plain / t_batch, "force_rel_err": err, "tuning": tun,
-            "plain_sample": f"{len(subset)} seeded clusters x {sub_steps} steps, "
-                            "scaled by their share of the pairs"}
+            "plain": "all 512 clusters, one context each, 200 graph-captured steps each, "
+                     "one after the other"}
 
 
 def bench_planetary(pkg, nat, args):
@@ -334,6 +450,10 @@ def main():
 
     batch = None
     if not args.no_batch:
+        progress("C1: Plummer N=4096, 100 steps, FP64 and split precision, with the oracle run")
+        c1 = bench_c1(pkg, nat, args) if not args.no_cpu else None
+        progress("C2: Plummer N=65536, FP32 state")
+        c2 = bench_c2(pkg, nat, args)
         batch = {"clusters": bench_clusters(pkg, nat, args),
                  "planetary": bench_planetary(pkg, nat, args)}
 
@@ -358,11 +478,11 @@ def main():
         # one force evaluation including the host-buffer boundary (PCIe): not `value`
         "host_inclusive": {"pairs_per_s": float(n) * float(n) / pcie_s, "s": pcie_s,
                            "what": "nb_upload (pos, vel, mass) + nb_compute_force + nb_download (acc)"},
-        # compute-bound: priced against MI355X's FP32 peak (157.3 TFLOP/s, the
-        # figure shared by FP32 MFMA and packed FP32 vector issue); the pair
-        # loop issues unpacked FP32 VALU + v_rsq_f32, and with far path 4 the
-        # far tiles' squared distances come from bf16 MFMAs beside it
-        "roofline": {"bound": "mfma",
+        # compute-bound on the vector ALUs: priced against MI355X's FP32 peak
+        # (157.3 TFLOP/s); the pair loop issues unpacked FP32 VALU + v_rsq_f32,
+        # and with far path 4 the far tiles' squared distances come from bf16
+        # MFMAs beside it (a quarter MFMA per 16 pairs), so VALU issue sets the rate
+        "roofline": {"bound": "valu",
                      "compute_unit": "FP32 VALU (unpacked) + v_rsq_f32; bf16 MFMA r^2 on far tiles"
                                      if tuning["far_path"] == 4 else "FP32 VALU (unpacked) + v_rsq_f32",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
@@ -374,6 +494,7 @@ def main():
                      "avg_kernel_ms": avg_force_s * 1e3, "launches": nforce,
                      "median_single_force_ms": float(np.median(singles)), "single_forces": len(singles)},
         "cpu_baseline": cpu,
+        "configs": None if args.no_batch else {"plummer_4096_fp64": c1, "plummer_65536_fp32": c2},
         "batched": batch,
         "device": {"name": dev["name"], "arch": dev["arch"], "cu": dev["compute_units"]},
     }

@@ -268,8 +268,8 @@ def test_segment_longer_than_one_class_group(pkg, orc, far):
     assert np.median(e) <= 2e-7 and np.percentile(e, 99) <= 2e-6
 
 
-@pytest.mark.parametrize("far", [0, 4])
-def test_active_subset_force_far_paths(pkg, orc, far):
+@pytest.mark.parametrize("mode,far", [("split", 0), ("split", 4), ("fp32", 0), ("fp32", 4)])
+def test_active_subset_force_far_paths(pkg, orc, mode, far):
     """Adaptive steps: the active-subset force (every third particle active)
     takes the context's far path; inactive particles keep their values
     (src/directsum/gravity.jl:94-98)."""
@@ -278,9 +278,13 @@ def test_active_subset_force_far_paths(pkg, orc, far):
     pos, vel, mass = pkg.ics.collision(10000, seeds=(7, 8))
     n = len(mass)
     soft = [0.002] * 6
-    ref = orc.compute_force(pos, mass, soft)
+    if mode == "fp32":   # the FP32 state rounds the inputs: compare on the rounded inputs
+        ref = orc.compute_force(pos.astype(np.float32).astype(np.float64),
+                                mass.astype(np.float32).astype(np.float64), soft)
+    else:
+        ref = orc.compute_force(pos, mass, soft)
     active = np.arange(n) % 3 == 0
-    with make_ctx(pkg, pos, vel, mass, soft, "split", tuning=(1, 4, 0, far)) as ctx:
+    with make_ctx(pkg, pos, vel, mass, soft, mode, tuning=(1, 4, 0, far)) as ctx:
         ctx.compute_force()
         call("nb_adaptive_init", ctx.handle, 1e-3, 1 << 10, 1e-2, 0.0, 1.0)
         ti_end = np.where(active, 64, 128).astype(np.int64)
@@ -294,7 +298,8 @@ def test_active_subset_force_far_paths(pkg, orc, far):
     assert na.value == int(active.sum())
     assert np.all(acc[~active] == 7.0)
     e = orc.rel_err(acc[active], ref[active])
-    print(f"far_path={far}: median {np.median(e):.3e} p99 {np.percentile(e, 99):.3e} max {e.max():.3e}")
+    print(f"{mode} far_path={far}: median {np.median(e):.3e} p99 {np.percentile(e, 99):.3e} "
+          f"max {e.max():.3e}")
     assert np.median(e) <= 2e-7 and np.percentile(e, 99) <= 2e-6
 
 
@@ -318,9 +323,47 @@ def test_far_outlier_particle(pkg, orc, far):
     assert np.median(e) <= 2e-7 and np.percentile(e, 99) <= 2e-6 and e[-1] <= 1e-6
 
 
+@pytest.mark.parametrize("geometry,far,nseg,sort", [
+    ("plummer", 0, 1, 0), ("plummer", 4, 1, 0), ("plummer", 4, 5, 1), ("collision", 0, 4, 16),
+    ("collision", 4, 4, 16), ("translated", 4, 4, 16)])
+def test_fp32_state_far_paths(pkg, orc, geometry, far, nseg, sort):
+    """The FP32 drop-in mode (the reference's floattype = Float32 GPU runs)
+    with far path 0 and with the bf16 matrix-core far tiles, which take the
+    FP32 positions relative to a per-block FP64 origin (hi/lo near tiles):
+    against the oracle on the FP32-rounded inputs."""
+    if geometry == "plummer":
+        pos, vel, mass = pkg.ics.plummer(4096, seed=1)
+        soft = [0.01] * 6
+    else:
+        pos, vel, mass = pkg.ics.collision(8192, seeds=(3, 4))
+        soft = [0.002] * 6
+        if geometry == "translated":
+            pos = pos + np.array([300.0, -125.0, 40.0])
+    p32 = pos.astype(np.float32).astype(np.float64)
+    m32 = mass.astype(np.float32).astype(np.float64)
+    ref = orc.compute_force(p32, m32, soft)
+    with make_ctx(pkg, pos, vel, mass, soft, "fp32", tuning=(1, nseg, sort, far)) as ctx:
+        ctx.compute_force()
+        acc = ctx.download(pos=False, vel=False)["acc"]
+    e = orc.rel_err(acc, ref)
+    print(f"fp32 {geometry} far_path={far}: median {np.median(e):.3e} p99 "
+          f"{np.percentile(e, 99):.3e} max {e.max():.3e}")
+    assert np.median(e) <= 2e-7 and np.percentile(e, 99) <= 2e-6 and e.max() <= 1e-4
+
+
+def test_fp32_state_rejects_split_only_far_paths(pkg):
+    pos, vel, mass = pkg.ics.plummer(300, seed=2)
+    with make_ctx(pkg, pos, vel, mass, [0.01] * 6, "fp32") as ctx:
+        for far in (1, 2, 3):
+            with pytest.raises(pkg._native.NBError):
+                ctx.set_tuning(1, 1, 0, far)
+        with pytest.raises(pkg._native.NBError):
+            ctx.set_tuning(2, 1, 0, 4)
+
+
 @pytest.mark.parametrize("mode,tpt,far", [
     ("split", 1, 0), ("split", 1, 3), ("split", 1, 4), ("split", 2, 0), ("split", 4, 3),
-    ("fp32", 1, 0), ("fp32", 2, 0), ("fp32", 4, 0)])
+    ("fp32", 1, 0), ("fp32", 2, 0), ("fp32", 4, 0), ("fp32", 1, 4)])
 def test_potential_alone_far_paths(pkg, orc, mode, tpt, far):
     """compute_potential runs potential-only kernels (with far path 4 on the
     matrix cores: m/r = rsq(w r^2) m^(2/3)); the accelerations are left as
@@ -559,3 +602,60 @@ def test_errors_surface(pkg):
             ctx.upload(np.zeros((10, 3)), None, np.ones(10), np.full(10, 9, np.uint8))
         with pytest.raises(nat.NBError):
             ctx.set_params(1.0, [-1.0] * 6)
+
+
+def _far_tile_worst_case(Re, m_src, shift):
+    """512 particles: a block of 256 massless targets around the origin (lanes
+    l and l^32 of each wave mirror each other, so the block centroid is
+    exactly the shift), with target 0 at +Re and target 32 at -Re on the x
+    axis (every other target within 0.3 Re); then one tile of 256 sources
+    beyond the wave's box on the +x ray, the first of mass m_src at
+    x' = 1.5 Re (1 + 1e-5) -- just past the 0.5 Re reach of the weighted far
+    tiles -- and 255 massless ones up to 0.1 Re further out."""
+    rng = np.random.default_rng(41)
+    pos = np.zeros((512, 3))
+    for w in range(4):
+        half = rng.uniform(-0.3 * Re / math.sqrt(3), 0.3 * Re / math.sqrt(3), (32, 3))
+        pos[64 * w:64 * w + 32] = half
+        pos[64 * w + 32:64 * w + 64] = -half
+    pos[0] = [Re, 0.0, 0.0]
+    pos[32] = [-Re, 0.0, 0.0]
+    xs = 1.5 * Re * (1 + 1e-5)
+    pos[256] = [xs, 0.0, 0.0]
+    pos[257:] = np.stack([xs + np.linspace(0.001, 0.1, 255) * Re, rng.uniform(-1e-3, 1e-3, 255) * Re,
+                          rng.uniform(-1e-3, 1e-3, 255) * Re], axis=1)
+    mass = np.zeros(512)
+    mass[256] = m_src
+    return pos + shift, mass
+
+
+@pytest.mark.parametrize("far", [0, 3, 4])
+def test_far_tile_worst_case_per_pair_error(pkg, orc, far):
+    """The worst geometry of the far tiles' r^2 expansion: the target at the
+    wave's Euclidean reach Re from the block origin and the source on the same
+    ray at exactly the 0.5 Re reach beyond it, so (|x_i'| + |x_j'|) / r = 5
+    and the expansion's bound is 3 eps 5^2 = 75 eps in w r^2 (eps = 2^-23);
+    the mirrored target (-Re) sees the same source anti-aligned (ratio 1).
+    Source masses span ten decades (w = m^(-2/3) over about 6.7 decades) and
+    the block sits at the origin or 1e6 away.  Each target's force is that
+    single pair's, so its relative error is the per-pair error: it must stay
+    within 1.5 times 1.5 x 75 eps (f = (w r^2)^(-3/2)) plus 10 eps for the
+    rsq, the cube and the S-form subtraction."""
+    eps = 2.0 ** -23
+    bound = 1.5 * 1.5 * 75 * eps + 10 * eps
+    worst = {0: 0.0, 32: 0.0}
+    for Re in (1.0, 37.0):
+        for shift in (np.zeros(3), np.array([1e6, -2e6, 5e5])):
+            for m_src in 10.0 ** np.arange(-10, 1):
+                pos, mass = _far_tile_worst_case(Re, m_src, shift)
+                with make_ctx(pkg, pos, None, mass, [1e-6] * 6, "split",
+                              tuning=(1, 1, 0, far)) as ctx:
+                    ctx.compute_force()
+                    acc = ctx.download(pos=False, vel=False)["acc"]
+                for t in (0, 32):
+                    ref = orc.pair_force(pos[t], pos[256], m_src, 1.0, 1e-6)
+                    e = np.linalg.norm(acc[t] - ref) / np.linalg.norm(ref)
+                    worst[t] = max(worst[t], e)
+    print(f"far_path={far}: worst per-pair error aligned {worst[0] / eps:.1f} eps, "
+          f"anti-aligned {worst[32] / eps:.1f} eps (bound {bound / eps:.0f} eps)")
+    assert worst[0] <= bound and worst[32] <= bound
