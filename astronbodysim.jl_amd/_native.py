@@ -110,6 +110,9 @@ _SIGS = {
     "nb_tuner_save": (C.c_int, [C.c_char_p]),
     "nb_tuner_load": (C.c_int, [C.c_char_p]),
     "nb_tuner_clear": (C.c_int, []),
+    # device pointers (and the stream) travel as plain addresses
+    "nb_compute_force_noalloc": (C.c_int, [C.c_int32, C.c_int64, _P, _P, _P, _P, _P, C.c_double,
+                                           C.c_double, _P, _P, _P, _P, _P]),
 }
 
 _OPTIONAL = {}
@@ -388,6 +391,16 @@ class Context:
         ms = C.c_double()
         call("nb_get_timing", self.handle, C.byref(cnt), C.byref(ms))
         return cnt.value, ms.value
+
+
+def compute_force_noalloc(mode: int, n: int, x: int, y: int, z: int, m: int, ax: int, ay: int,
+                          az: int, pot: int = 0, h: int = 0, h_const: float = 0.0, G: float = 1.0,
+                          stream: int = 0):
+    """nb_compute_force_noalloc on caller-owned device arrays, given as device
+    addresses (0 = NULL for pot, h and stream).  Enqueued on `stream`; the
+    caller synchronises."""
+    call("nb_compute_force_noalloc", int(mode), int(n), x, y, z, m, h or None, float(h_const),
+         float(G), ax, ay, az, pot or None, stream or None)
 
 
 def tuner_save(path):

@@ -148,6 +148,29 @@ NB_API int nb_compute_potential(nb_ctx *ctx, int32_t add_self_term);
 NB_API int nb_force_at_points(nb_ctx *ctx, int64_t npts, const double *pts, double h, double *acc);
 NB_API int nb_potential_at_points(nb_ctx *ctx, int64_t npts, const double *pts, double h, double *pot);
 
+/* Allocation-free force evaluation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory -- code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc.  It computes what
+ * nb_compute_force computes (the same kernels; FP64 state with FP32 pair
+ * arithmetic in NB_MODE_SPLIT, all FP64 in NB_MODE_FP64) on n particles
+ * that the caller keeps in DEVICE memory as SoA arrays x, y, z, m, with the
+ * per-target softening h[i] (h == NULL: h_const for every particle), and
+ * writes ax, ay, az (and pot unless NULL: the unbiased potential) there.
+ * Particles are taken in the order given (no Morton sort).  The launches are
+ * enqueued on `stream` (a hipStream_t; NULL = the default stream) and the
+ * call returns without waiting.  It calls no hipMalloc / hipFree: the tile
+ * boxes and per-segment partial sums use one fixed device array of
+ * NB_NOALLOC_SCRATCH_BYTES, which every GPU the library runs on holds until
+ * the process ends; n above NB_NOALLOC_MAX_N is rejected (NB_ERR_INVALID).
+ * All calls on one GPU share that array, so calls that overlap on one GPU
+ * (other streams, other host threads) race: serialise them. */
+#define NB_NOALLOC_SCRATCH_BYTES (64ll << 20)
+#define NB_NOALLOC_MAX_N (1ll << 28)
+NB_API int nb_compute_force_noalloc(int32_t precision_mode, int64_t n, const double *x,
+                                    const double *y, const double *z, const double *m,
+                                    const double *h, double h_const, double G, double *ax,
+                                    double *ay, double *az, double *pot, void *stream);
+
 /* ---- constant-timestep leapfrog (KDK) --------------------------------- */
 
 /* Vel += Acc * forward * 0.5 * dt; replaces leapfrog_half_kick (ConstantTimestep)

@@ -122,7 +122,7 @@ def _c_arg(decl):
     stars = d.count("*") + (1 if arr else 0)
     words = d.replace("*", " ").split()
     base = words[0]
-    if base in ("nb_ctx", "nb_batch"):
+    if base in ("nb_ctx", "nb_batch") or (base == "void" and stars == 1):   # opaque pointers
         return ("handle_out" if stars == 2 else "handle", "")
     assert base in _C_BASES, decl
     return ("ptr" if stars else "val", base)
@@ -243,3 +243,16 @@ def test_julia_stub_structs_match_header():
             mm = re.fullmatch(r"(\w+)\s+(\w+)(?:\[(\d+)\])?", " ".join(line.split()))
             cf.append((mm.group(2), (mm.group(1), int(mm.group(3) or 1))))
         assert jf == cf, (jl, jf, cf)
+
+
+def test_noalloc_rejects_sizes_beyond_its_scratch(pkg):
+    """nb_compute_force_noalloc refuses n above NB_NOALLOC_MAX_N (and a bad
+    mode) before it touches a device."""
+    nat = pkg._native
+    p = C.c_void_p(16)
+    args = [p, p, p, p, None, 0.01, 1.0, p, p, p, None, None]
+    assert nat.lib.nb_compute_force_noalloc(2, (1 << 28) + 1, *args) == nat.NB_ERR_INVALID
+    assert "scratch" in nat.last_error()
+    assert nat.lib.nb_compute_force_noalloc(1, 100, *args) == nat.NB_ERR_INVALID
+    assert nat.lib.nb_compute_force_noalloc(2, -1, *args) == nat.NB_ERR_INVALID
+    assert nat.lib.nb_compute_force_noalloc(2, 0, *args) == nat.NB_OK
