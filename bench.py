@@ -70,21 +70,20 @@ def progress(msg):
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
-def _force_timing(ctx, reps=5):
-    """Median HIP-event duration (ms) of `reps` separately timed force
-    evaluations on the context's stream."""
-    ctx.set_timing(True)
+def _force_timing(ctx, reps=10):
+    """Average HIP-event duration (ms) of the force kernel over `reps` force
+    evaluations enqueued back to back (a host wait between them would leave
+    the GPU idle when the start event is taken, which inflates a sub-ms
+    kernel's time by the launch gap)."""
     ctx.compute_force()
     ctx.sync()
-    out = []
+    ctx.set_timing(True)
     for _ in range(reps):
-        c0, m0 = ctx.get_timing()
         ctx.compute_force()
-        ctx.sync()
-        c1, m1 = ctx.get_timing()
-        out.append((m1 - m0) / max(c1 - c0, 1))
+    ctx.sync()
+    count, ms = ctx.get_timing()
     ctx.set_timing(False)
-    return float(np.median(out))
+    return ms / max(count, 1)
 
 
 def bench_c1(pkg, nat, args):
@@ -141,7 +140,7 @@ def bench_c1(pkg, nat, args):
 
 def bench_c2(pkg, nat, args):
     """Plummer N = 65536, eps = 0.005, FP32 state (the reference's floattype =
-    Float32 GPU mode): one force evaluation (HIP events, median of 5) and 50
+    Float32 GPU mode): one force evaluation (HIP events, mean of 10) and 50
     KDK steps at dt = 5e-4; forces of a seeded target sample against the FP64
     oracle on the FP32-rounded inputs; energy drift (nb_energy)."""
     n, h, dt, steps = 65536, 0.005, 5e-4, 50
@@ -311,8 +310,13 @@ def bench_planetary(pkg, nat, args):
     out = {"workload": "2048 planetary systems x 73 bodies, eps=0, 1e4 KDK steps, "
                        "energy every 100 steps",
            "steps": steps, "batched_s": t_batch,
+           "us_per_step": t_batch / steps * 1e6,
            "batched_pairs_per_s_allpairs": allpairs * steps / t_batch,
            "batched_pairs_per_s_evaluated": evaluated * steps / t_batch,
+           # 20 flops per pair against the FP32 peak; the evaluated pairs (every
+           # body against the 9 massive ones) are the work the kernel does
+           "evaluated_frac_of_peak": FLOPS_PER_PAIR * evaluated * steps / t_batch / 1e12 / PEAK_FP32_TFLOPS,
+           "allpairs_frac_of_peak": FLOPS_PER_PAIR * allpairs * steps / t_batch / 1e12 / PEAK_FP32_TFLOPS,
            "plain_s_estimate": t_plain, "speedup": t_plain / t_batch,
            "rel_energy_err": {"median": float(np.median(rel)),
                               "p99": float(np.percentile(rel.max(1), 99)),
