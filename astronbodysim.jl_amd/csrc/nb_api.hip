@@ -16,9 +16,10 @@
 #include <utility>
 #include <vector>
 
+#include "nb_ctx.h"
 #include "nb_integrate.h"
 #include "nb_internal.h"
-#include "nb_ctx.h"
+#include "nb_pair.h"
 
 #define NB_VERSION_MAJOR 1
 #define NB_VERSION_MINOR 0
@@ -165,6 +166,32 @@ __global__ void k_gather_central(const double *__restrict__ x, const double *__r
     cm[t] = m[i];
 }
 
+// The potential of each central body summed over every source in FP64 (the
+// FP32 kernel gave it the non-central sources' terms in FP32): one block per
+// central body, a fixed-order block reduction, the result overwrites pot.
+__global__ __launch_bounds__(256) void k_central_target_pot(const double *__restrict__ x,
+                                                            const double *__restrict__ y,
+                                                            const double *__restrict__ z,
+                                                            const double *__restrict__ m,
+                                                            const double *__restrict__ h, int64_t n,
+                                                            const int64_t *__restrict__ cidx,
+                                                            double G, double *__restrict__ pot) {
+    __shared__ double red[4];
+    const int64_t i = cidx[blockIdx.x];
+    const double xi = x[i], yi = y[i], zi = z[i], hi = h[i];
+    const double hinv = 1.0 / hi, hinv3 = hinv * hinv * hinv;
+    double bx = 0.0, by = 0.0, bz = 0.0, q = 0.0;
+    for (int64_t j = threadIdx.x; j < n; j += 256) {
+        if (j == i) continue;
+        pair_f64<true>(x[j], y[j], z[j], m[j], xi, yi, zi, hi, hinv, hinv3, bx, by, bz, q);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) q += __shfl_xor(q, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = q;
+    __syncthreads();
+    if (threadIdx.x == 0) pot[i] = -G * (((red[0] + red[1]) + red[2]) + red[3]);
+}
+
 int gather_central(nb_ctx *c) {
     const int64_t k = c->ncentral;
     if (k <= 0) return NB_OK;
@@ -263,6 +290,12 @@ int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = tru
         a.ax = c->ax; a.ay = c->ay; a.az = c->az; a.pot = c->pot;
         a.accumulate = 1;
         NB_TRY(launch_force_f64(a, with_pot, s));
+        if (with_pot) {
+            k_central_target_pot<<<(unsigned)k, 256, 0, s>>>((const double *)c->x, (const double *)c->y,
+                                                            (const double *)c->z, (const double *)c->m,
+                                                            c->h, n, c->cidx, c->G, c->pot);
+            NB_HIP(hipGetLastError());
+        }
     }
     for (const BgField &f : c->bg)
         NB_TRY(background(c->x, c->y, c->z, n, c->is_double, f, c->ax, c->ay, c->az,

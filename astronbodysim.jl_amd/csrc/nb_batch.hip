@@ -190,6 +190,9 @@ __global__ __launch_bounds__(BS) void k_batch_force_f64(BatchArgs a) {
 }
 
 // FP64 terms of the central bodies of each system, added to acc (and pot).
+// With POT, a central body's own potential is summed over every source of its
+// system in FP64 (not only over the central ones), so that energies carry the
+// central bodies' terms in FP64 whichever side of the pair they are on.
 template <bool POT, bool ACC = true>
 __global__ void k_batch_central(const int32_t *__restrict__ sys_of, const int64_t *__restrict__ cidx,
                                 int ncentral, const double *__restrict__ Gs,
@@ -197,13 +200,24 @@ __global__ void k_batch_central(const int32_t *__restrict__ sys_of, const int64_
                                 const double *__restrict__ z, const double *__restrict__ m,
                                 const double *__restrict__ h, int64_t n, double *__restrict__ ax,
                                 double *__restrict__ ay, double *__restrict__ az,
-                                double *__restrict__ pot) {
+                                double *__restrict__ pot, const int64_t *__restrict__ off,
+                                const uint8_t *__restrict__ central) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int s = sys_of[i];
     const double xi = x[i], yi = y[i], zi = z[i], hi = h[i];
     const double hinv = 1.0 / hi, hinv3 = hinv * hinv * hinv;
     double Ax = 0.0, Ay = 0.0, Az = 0.0, Aq = 0.0;
+    if (POT && central[i]) {
+        double bx = 0.0, by = 0.0, bz = 0.0;
+        for (int64_t j = off[s]; j < off[s + 1]; ++j) {
+            if (j == i) continue;
+            pair_f64<true>(x[j], y[j], z[j], m[j], xi, yi, zi, hi, hinv, hinv3, bx, by, bz, Aq);
+        }
+        pot[i] = -Gs[s] * Aq;
+        Aq = 0.0;
+        if (!ACC) return;
+    }
     for (int c = 0; c < ncentral; ++c) {
         const int64_t j = cidx[(int64_t)s * ncentral + c];
         if (j == i) continue;
@@ -215,7 +229,7 @@ __global__ void k_batch_central(const int32_t *__restrict__ sys_of, const int64_
         ay[i] += G * Ay;
         az[i] += G * Az;
     }
-    if (POT) pot[i] -= G * Aq;
+    if (POT && !central[i]) pot[i] -= G * Aq;
 }
 
 // v += a * coef * dt_s ; optionally then x += v * dt_s (the next step's drift)
@@ -853,15 +867,15 @@ static int bforce(nb_batch *b, bool with_pot, bool allow_timing, bool pot_only =
         if (with_pot && pot_only && !f64)
             k_batch_central<true, false><<<blocks_for(b->n), 256, 0, s>>>(
                 b->sys_of, b->cidx, b->ncentral, b->Gs, b->x, b->y, b->z, b->m, b->h, b->n, b->ax,
-                b->ay, b->az, b->pot);
+                b->ay, b->az, b->pot, b->off, b->central);
         else if (with_pot)
             k_batch_central<true><<<blocks_for(b->n), 256, 0, s>>>(
                 b->sys_of, b->cidx, b->ncentral, b->Gs, b->x, b->y, b->z, b->m, b->h, b->n, b->ax,
-                b->ay, b->az, b->pot);
+                b->ay, b->az, b->pot, b->off, b->central);
         else
             k_batch_central<false><<<blocks_for(b->n), 256, 0, s>>>(
                 b->sys_of, b->cidx, b->ncentral, b->Gs, b->x, b->y, b->z, b->m, b->h, b->n, b->ax,
-                b->ay, b->az, b->pot);
+                b->ay, b->az, b->pot, b->off, b->central);
         NB_TRY(launch_ok("k_batch_central"));
     }
     return NB_OK;

@@ -182,6 +182,19 @@ static bool json_int(const std::string &obj, const char *key, long *out) {
     return true;
 }
 
+static bool json_double(const std::string &obj, const char *key, double *out) {
+    std::string k = std::string("\"") + key + "\"";
+    size_t p = obj.find(k);
+    if (p == std::string::npos) return false;
+    p = obj.find(':', p + k.size());
+    if (p == std::string::npos) return false;
+    char *end = nullptr;
+    double v = strtod(obj.c_str() + p + 1, &end);
+    if (end == obj.c_str() + p + 1) return false;
+    *out = v;
+    return true;
+}
+
 extern "C" int nb_tuner_load(const char *path) {
     nb::clear_error();
     if (!path) {
@@ -219,12 +232,15 @@ extern "C" int nb_tuner_load(const char *path) {
         json_int(obj, "sort_interval", &si);
         json_int(obj, "far_path", &mf);
         if ((tpt != 1 && tpt != 2 && tpt != 4) || nseg < 1 || nseg > 256 || mode < 0 || mode > 3 ||
-            si < 0 || mf < 0 || mf > 4 || ((mf == 1 || mf == 2 || mf == 4) && tpt != 1)) {
+            si < 0 || mf < 0 || mf > 4 || ((mf == 1 || mf == 2 || mf == 4) && tpt != 1) ||
+            (mode == NB_MODE_FP32 && mf != 0 && mf != 4)) {
             nb::set_error("%s: entry out of range", path);
             return NB_ERR_INVALID;
         }
+        double ms = 0.0;   // the measured time, kept so that a saved table still carries it
+        json_double(obj, "ms", &ms);
         nb::TuneEntry e{(int)mode, (int)log2n, {(int32_t)tpt, (int32_t)nseg, (int32_t)si, (int32_t)mf},
-                        0.0};
+                        ms};
         entries.push_back(e);
         pos = b + 1;
     }

@@ -5,8 +5,8 @@ and save what the autotuner measured.
     python -m astronbodysim.jl_amd.tuning.make_table [out.json] [--modes=split,fp32,fp64]
 
 (or run this file directly).  Sizes 2^12 .. 2^20 in the split mode on
-two-Plummer collisions (the benchmark's geometry), and 2^12 .. 2^16 in the
-fp32 and fp64 modes on Plummer spheres.  With --modes= only those modes are
+two-Plummer collisions (the benchmark's geometry), 2^12 .. 2^18 in the fp32
+mode and 2^12 .. 2^15 in the fp64 mode on Plummer spheres.  With --modes= only those modes are
 measured again and the table's other entries are kept.  Needs the GPU; a
 few minutes.
 """
@@ -32,7 +32,7 @@ def main(out=None, modes=None):
     nat.tuner_clear()
     if modes:    # keep the entries of the modes not measured again
         nat.tuner_load(nat.DEFAULT_TUNING)
-    plan = [("split", k) for k in range(12, 21)] + [("fp32", k) for k in range(12, 17)] + \
+    plan = [("split", k) for k in range(12, 21)] + [("fp32", k) for k in range(12, 19)] + \
            [("fp64", k) for k in range(12, 16)]
     plan = [(m, k) for m, k in plan if not modes or m in modes]
     for mode, k in plan:

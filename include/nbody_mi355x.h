@@ -76,12 +76,13 @@ typedef struct nb_tuning {
     int32_t targets_per_thread; /* 1, 2 or 4                                   */
     int32_t j_segments;         /* source-range split (>=1; partials reduced in fixed order) */
     int32_t sort_interval;      /* re-sort into Morton order every k force calls (0 = never) */
-    int32_t far_path;           /* far-tile arithmetic (SPLIT modes; forces only, potentials take
-                                   0): 0 FP32 VALU, 12 ops + rsq per pair; 3 weighted VALU,
-                                   10 ops + rsq on the farthest tiles; 1 sum on the matrix
-                                   cores, 2 also r^2 there; 4 r^2 from bf16-split products
-                                   on the matrix cores beside a VALU sum (1, 2 and 4:
-                                   targets_per_thread 1) */
+    int32_t far_path;           /* far-tile arithmetic: 0 FP32 VALU, 12 ops + rsq per pair;
+                                   3 weighted VALU, 10 ops + rsq on the farthest tiles; 1 sum
+                                   on the matrix cores, 2 also r^2 there; 4 r^2 from
+                                   bf16-split products on the matrix cores beside a VALU sum
+                                   (also for potentials alone).  1, 2 and 4 need
+                                   targets_per_thread 1; the FP32 mode takes 0 or 4, the
+                                   SPLIT modes any; forces with potentials take 0 */
 } nb_tuning;
 
 /* ---- library / device ------------------------------------------------- */

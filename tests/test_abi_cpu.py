@@ -94,8 +94,14 @@ def test_tuner_table_roundtrip(pkg, tmp_path):
     nat.tuner_save(out)
     text = out.read_text()
     assert '"log2n": 16' in text and '"j_segments": 8' in text and '"targets_per_thread": 2' in text
+    assert '"ms": 1.5' in text          # the measured time survives a load and a save
     bad = tmp_path / "bad.json"
     bad.write_text('{"entries": [{"mode": 2, "log2n": 16, "targets_per_thread": 3, "j_segments": 1}]}')
+    with pytest.raises(nat.NBError):
+        nat.tuner_load(bad)
+    # the FP32 mode has far paths 0 and 4 only
+    bad.write_text('{"entries": [{"mode": 1, "log2n": 16, "targets_per_thread": 1, "j_segments": 4, '
+                   '"far_path": 3}]}')
     with pytest.raises(nat.NBError):
         nat.tuner_load(bad)
     nat.tuner_clear()

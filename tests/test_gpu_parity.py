@@ -505,6 +505,27 @@ def test_central_body_fp64_terms(pkg, orc):
     assert np.median(e) < 1e-9
 
 
+def test_central_body_potential_and_energy_fp64(pkg, orc):
+    """split_central takes the central body's own potential over every source
+    in FP64 too (its terms on either side of a pair), so the total energy of
+    a planetary system agrees with the FP64 oracle to about 1e-10 (the FP32
+    planet-planet terms), not 1e-7 (an FP32 star potential)."""
+    sysd = pkg.ics.planetary_ensemble(nsys=1, seed=7)
+    pos, vel, mass = sysd["pos"], sysd["vel"], sysd["mass"]
+    pref = orc.compute_potential(pos, mass, [0.0] * 6, skip_self=True)
+    K, W, _, _ = orc.energy(pos, vel, mass, pref)
+    with make_ctx(pkg, pos, vel, mass, [0.0] * 6, "split_central") as ctx:
+        ctx.set_central(1)
+        ctx.compute_potential(self_term=False)
+        pot = ctx.download(pos=False, vel=False, acc=False, pot=True)["pot"]
+        en = ctx.energy()
+    ep = np.abs(pot - pref) / np.abs(pref)
+    eE = abs((en["K"] + en["W"]) - (K + W)) / abs(K + W)
+    print(f"central potential error {ep[0]:.3e}, planets max {ep[1:9].max():.3e}; energy {eE:.3e}")
+    assert ep[0] <= 1e-13 and ep.max() <= 1e-6
+    assert eE <= 1e-9
+
+
 @pytest.mark.parametrize("far", [0, 3, 4])
 def test_far_paths_with_central_and_massless_sources(pkg, orc, far):
     """A star, massive bodies and massless test particles in split_central
