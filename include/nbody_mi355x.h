@@ -20,8 +20,11 @@
  *   NB_MODE_SPLIT : FP64 state; FP32 pair arithmetic on positions taken
  *                   relative to a per-block FP64 origin; per-tile FP32 sums
  *                   folded into FP64 accumulators; FP64 kick/drift/energy.
- *   NB_MODE_SPLIT_CENTRAL : NB_MODE_SPLIT plus the pair terms of the
- *                   particles marked central (nb_set_central) in FP64.
+ *   NB_MODE_SPLIT_CENTRAL : NB_MODE_SPLIT plus, in FP64, the pair terms whose
+ *                   source is a particle marked central (nb_set_central), for
+ *                   every target, and the potential of each central particle
+ *                   over every source (so energies carry every term of a
+ *                   central particle in FP64).
  *
  * One context owns one HIP stream on the device that was current when it was
  * created.  Calls on one context must not overlap; separate contexts are
