@@ -262,3 +262,16 @@ def test_noalloc_rejects_sizes_beyond_its_scratch(pkg):
     assert nat.lib.nb_compute_force_noalloc(1, 100, *args) == nat.NB_ERR_INVALID
     assert nat.lib.nb_compute_force_noalloc(2, -1, *args) == nat.NB_ERR_INVALID
     assert nat.lib.nb_compute_force_noalloc(2, 0, *args) == nat.NB_OK
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    """Without the HIP library the package does not import: there is no CPU
+    fallback to run instead."""
+    import subprocess
+    import sys
+    code = ("import __graft_entry__ as g\n"
+            "try:\n    g.load_package()\nexcept ImportError as e:\n    print('IMPORTERROR', e)\n")
+    env = dict(__import__("os").environ, NBODY_MI355X_LIB=str(tmp_path / "missing.so"))
+    out = subprocess.run([sys.executable, "-c", code], cwd=str(ROOT), env=env, capture_output=True,
+                         text=True, timeout=120)
+    assert "IMPORTERROR" in out.stdout and "no CPU fallback" in out.stdout, out.stdout + out.stderr
