@@ -210,11 +210,16 @@ constexpr float FARW_MASSLESS = 1.329228e36f;   // 2^120
 // Shared memory of one all-pairs block.
 // HILO: positions relative to the block's FP64 origin, near tiles on hi/lo
 // coordinates.  FP64 state always; FP32 state with the bf16 matrix-core far
-// tiles, whose squared-distance expansion needs origin-relative coordinates
-// (there the hi/lo pair holds the FP32 position minus the FP64 origin exactly).
+// tiles, whose squared-distance expansion needs origin-relative coordinates.
+// ABSNEAR (FP32 state with those far tiles): the near tiles take the FP32
+// positions themselves instead of hi/lo, as the FP32-state loops do: two
+// FP32 positions differ by an FP32 subtraction with a relative error of at
+// most 2^-24, as small as the hi/lo difference's, for 12 VALU per pair
+// instead of 18.  sjl then holds (x, y, z, m) of each source.
 template <typename T, int MF = 0>
 struct BlockShared {
     static constexpr bool HILO = sizeof(T) == 8 || MF == FAR_MFMA_BF16;
+    static constexpr bool ABSNEAR = sizeof(T) == 4 && MF == FAR_MFMA_BF16;
     float4 sj[2][BS];
     float4 sjl[HILO ? 2 : 1][HILO ? BS : 1];
     static constexpr bool COLS = MF == 1 || MF == 2;   // the MFMA far paths
@@ -659,6 +664,7 @@ __device__ __forceinline__ void force_block(
     const uint8_t *__restrict__ skip_src, bool use_origin, double (&Ax)[TPT], double (&Ay)[TPT],
     double (&Az)[TPT], double (&Aq)[TPT], bool (&valid)[TPT]) {
     constexpr bool HILO = BlockShared<T, MF>::HILO;
+    constexpr bool ABSNEAR = BlockShared<T, MF>::ABSNEAR;
     static_assert(!(MF == FAR_MFMA_SUM || MF == FAR_MFMA_R2) || (HILO && TPT == 1 && !POT),
                   "MFMA far tiles: SPLIT, 1 target/lane, acc only");
     static_assert(MF != FAR_WEIGHTED || !POT, "weighted far tiles: acc only");
@@ -685,6 +691,7 @@ __device__ __forceinline__ void force_block(
     const double ox = uniform_f64(st.ox), oy = uniform_f64(st.oy), oz = uniform_f64(st.oz);
 
     float xi[TPT], yi[TPT], zi[TPT], xil[TPT], yil[TPT], zil[TPT];
+    float xa[TPT], ya[TPT], za[TPT];   // ABSNEAR: the FP32 positions themselves
     float h2c[TPT], h2[TPT], hinv[TPT], hinv3[TPT];
     float ax[TPT], ay[TPT], az[TPT], aq[TPT];
 #pragma unroll
@@ -696,6 +703,9 @@ __device__ __forceinline__ void force_block(
         xil[k] = (float)(dx - (double)xi[k]);
         yil[k] = (float)(dy - (double)yi[k]);
         zil[k] = (float)(dz - (double)zi[k]);
+        xa[k] = (float)px[k];
+        ya[k] = (float)py[k];
+        za[k] = (float)pz[k];
         const float hf = (float)ph[k];
         h2[k] = hf * hf;
         h2c[k] = fmaxf(h2[k], FLT_MIN);
@@ -705,7 +715,8 @@ __device__ __forceinline__ void force_block(
         Ax[k] = Ay[k] = Az[k] = Aq[k] = 0.0;
     }
     // Tiles are classified per wave (the branch is wave-uniform).
-    const WaveGeom wg = wave_geometry<TPT>(px, py, pz, ph, valid, ox, oy, oz, HILO);
+    // (ABSNEAR has no close class: every near tile takes the FP32 positions)
+    const WaveGeom wg = wave_geometry<TPT>(px, py, pz, ph, valid, ox, oy, oz, HILO && !ABSNEAR);
     const double soft_d2 = wg.soft_d2, close_d2 = wg.close_d2;
     const double close_d = sqrt(close_d2);
     // FAR_MFMA_BF16: lane l holds, for target group g, the K slice (l >> 4) of
@@ -777,12 +788,16 @@ __device__ __forceinline__ void force_block(
         if (rv) {
             const double dx = (double)rx - ox, dy = (double)ry - oy, dz = (double)rz - oz;
             const float hx = (float)dx, hy = (float)dy, hz = (float)dz;
-            sh.sj[buf][tid] = make_float4(hx, hy, hz, (float)rm);
-            if (HILO)
+            // m^(2/3) of the potential-only far tiles rides in the w slot of
+            // the lo record (ABSNEAR: of the hi record)
+            const float cw = (ROWS && !ACC && rw > 0.0f) ? __builtin_amdgcn_rcpf(rw) : 0.0f;
+            sh.sj[buf][tid] = make_float4(hx, hy, hz, (ABSNEAR && !ACC) ? cw : (float)rm);
+            if (ABSNEAR)
+                sh.sjl[HILO ? buf : 0][HILO ? tid : 0] = make_float4((float)rx, (float)ry, (float)rz, (float)rm);
+            else if (HILO)
                 sh.sjl[HILO ? buf : 0][HILO ? tid : 0] =
                     make_float4((float)(dx - (double)hx), (float)(dy - (double)hy),
-                                (float)(dz - (double)hz),
-                                (ROWS && !ACC && rw > 0.0f) ? __builtin_amdgcn_rcpf(rw) : 0.0f);
+                                (float)(dz - (double)hz), cw);
             if constexpr (MF == 1 || MF == 2) {
                 const double md = (double)rm;
                 sh.sB[buf][0][tid] = (float)(md * dx);
@@ -817,7 +832,10 @@ __device__ __forceinline__ void force_block(
             }
         } else {
             sh.sj[buf][tid] = make_float4(PAD_COORD, PAD_COORD, PAD_COORD, 0.0f);
-            if (HILO) sh.sjl[HILO ? buf : 0][HILO ? tid : 0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (ABSNEAR)
+                sh.sjl[HILO ? buf : 0][HILO ? tid : 0] = make_float4(PAD_COORD, PAD_COORD, PAD_COORD, 0.0f);
+            else if (HILO)
+                sh.sjl[HILO ? buf : 0][HILO ? tid : 0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if constexpr (MF == 1 || MF == 2) {
                 sh.sB[buf][0][tid] = 0.0f;
                 sh.sB[buf][1][tid] = 0.0f;
@@ -858,19 +876,22 @@ __device__ __forceinline__ void force_block(
         }
         const uint64_t tbit = 1ull << (it & 63);
         if (m_soft & tbit) {
-            if (HILO)
+            if (ABSNEAR)
+                tile_f32<TPT, true, POT>(sh.sjl[HILO ? buf : 0], xa, ya, za, h2c, h2, hinv, hinv3, ax, ay, az,
+                                         aq);
+            else if (HILO)
                 tile_f32_hilo<TPT, true, POT>(sh.sj[buf], sh.sjl[HILO ? buf : 0], xi, yi, zi, xil,
                                               yil, zil, h2c, h2, hinv, hinv3, ax, ay, az, aq);
             else
                 tile_f32<TPT, true, POT>(sh.sj[buf], xi, yi, zi, h2c, h2, hinv, hinv3, ax, ay, az,
                                          aq);
-        } else if (HILO && (m_close & tbit)) {
+        } else if (HILO && !ABSNEAR && (m_close & tbit)) {
             tile_f32_hilo<TPT, false, POT>(sh.sj[buf], sh.sjl[HILO ? buf : 0], xi, yi, zi, xil,
                                            yil, zil, h2c, h2, hinv, hinv3, ax, ay, az, aq);
         } else if (ROWS && (m_far & tbit)) {
             if constexpr (!ACC) {
                 float Sq[4];
-                tile_mf3_pot(sh.sjl[HILO ? buf : 0], sh.sA[0], tid & 63, Bg, Sq);
+                tile_mf3_pot(ABSNEAR ? sh.sj[buf] : sh.sjl[HILO ? buf : 0], sh.sA[0], tid & 63, Bg, Sq);
                 Aq[0] += (double)fold_mf3_one(Sq);
             } else {
                 float S3[4][4];
@@ -895,6 +916,8 @@ __device__ __forceinline__ void force_block(
                            b2y, b2z, ci, S);
             else
                 tile_mfma(sh.sj[buf], &sh.sB[buf][0][0], tid & 3, xi[0], yi[0], zi[0], S);
+        } else if (ABSNEAR) {
+            tile_f32<TPT, false, POT>(sh.sjl[HILO ? buf : 0], xa, ya, za, h2c, h2, hinv, hinv3, ax, ay, az, aq);
         } else {
             tile_f32<TPT, false, POT>(sh.sj[buf], xi, yi, zi, h2c, h2, hinv, hinv3, ax, ay, az, aq);
         }
