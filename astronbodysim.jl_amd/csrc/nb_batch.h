@@ -16,6 +16,16 @@ struct nb_batch {
     int64_t n = 0;
     int64_t max_ns = 0;
     std::vector<int64_t> h_off;
+    std::vector<int32_t> h_nm;    // massive particles of each system (set by upload)
+    std::vector<uint8_t> h_massive;   // per particle in the caller's order: mass != 0
+    std::vector<double> h_mmax;   // largest |mass| of each system
+    // packed resident layout (k_batch_resident_packed), rebuilt when stale
+    int32_t *pk_lane = nullptr;   // [blocks][threads] particle, -1 for an idle lane
+    int32_t *pk_meta = nullptr;   // [blocks][threads] (system in block << 16) | first flag | record slot
+    int32_t *pk_blk = nullptr;    // [blocks][stride] block table
+    double *pk_tiny = nullptr;    // [blocks][2] r^2 offsets (FP32, FP64)
+    int pk_nblk = 0, pk_threads = 0, pk_smax = 0, pk_recs = 0;
+    bool pk_stale = true;
     std::vector<double> hG, hh, hdt;
     int64_t *off = nullptr;       // [nsys+1]
     int32_t *sys_of = nullptr;    // system of each particle

@@ -10,10 +10,12 @@
 //    is the same block routine as the one-system kernel (force_block), with
 //    the system's own source range, per-system Morton order, FP64 origin and
 //    hi/lo near tiles.  KDK steps are captured in HIP graphs.
-//  * resident path (every system <= 256 particles): one block per system
-//    keeps its system in registers/LDS and runs many KDK steps in a single
-//    launch; central bodies use FP64 pair terms, the other massive bodies FP32
-//    hi/lo terms relative to the system's first particle.
+//  * resident path (systems whose massive particles plus a share of their
+//    massless ones fit 256 lanes of one to four wavefronts): one block per
+//    system keeps its particles in registers and runs many KDK steps in a
+//    single launch, the sources broadcast through LDS; central bodies use FP64
+//    pair terms, the other massive bodies FP32 hi/lo terms relative to the
+//    system's first massive particle.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -301,7 +303,7 @@ __global__ __launch_bounds__(256) void k_batch_energy(
     }
 }
 
-// ---- resident multi-step kernel (systems of <= 256 particles) ------------
+// ---- resident multi-step kernel (small systems, many steps per launch) ----
 struct ResArgs {
     const int64_t *off;
     const double *Gs, *hs, *dts;
@@ -339,14 +341,16 @@ __device__ __forceinline__ double rsq_f64(double x) {
     return y;
 }
 
-// FP32 hi/lo pair of the resident kernel: source (sh, sl, sm) taken from
-// scalar registers, target k of the lane.  SOFT: the spline inside h;
-// otherwise Newtonian with the self pair (r = 0) contributing nothing.
+// FP32 hi/lo pair of the resident kernel: source (sh, sl, sm), target k of
+// the lane.  SOFT: the spline inside h.  Otherwise Newtonian, with r^2 + tiny
+// in place of r^2 (tiny from resident_tiny): the self pair (r = 0) then has a
+// finite factor times dx = 0, so it contributes nothing without a select, and
+// every pair with r^2 above about 2^24 tiny keeps its FP32 r^2 unchanged.
 template <bool SOFT>
 __device__ __forceinline__ void res_pair_f32(float sxh, float syh, float szh, float sxl, float syl,
                                              float szl, float smf, float xh, float yh, float zh,
                                              float xl, float yl, float zl, float h2cf, float h2f,
-                                             float hinvf, float hinv3f, float &fx, float &fy,
+                                             float hinvf, float hinv3f, float tiny, float &fx, float &fy,
                                              float &fz, float &fq) {
     if constexpr (SOFT) {
         pair_f32_hilo<true, false>(sxh, syh, szh, smf, sxl, syl, szl, xh, yh, zh, xl, yl, zl, h2cf,
@@ -355,33 +359,77 @@ __device__ __forceinline__ void res_pair_f32(float sxh, float syh, float szh, fl
         const float dx = (sxh - xh) + (sxl - xl);
         const float dy = (syh - yh) + (syl - yl);
         const float dz = (szh - zh) + (szl - zl);
-        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, tiny)));
         const float ri = __builtin_amdgcn_rsqf(r2);
-        const float fac = r2 > 0.0f ? smf * (ri * ri) * ri : 0.0f;
+        const float fac = smf * (ri * ri) * ri;
         fx = __builtin_fmaf(fac, dx, fx);
         fy = __builtin_fmaf(fac, dy, fy);
         fz = __builtin_fmaf(fac, dz, fz);
     }
 }
 
-// State of one resident system: lane l holds particles l, l + 64, ...
+// The r^2 offsets of the unsoftened resident pairs: tiny = (4 max(m_max, 1) /
+// MAX)^(2/3) makes m r^-3 at r^2 = tiny at most MAX / 4 for every source mass
+// m <= m_max (MAX = FLT_MAX or DBL_MAX), so a zero separation gives a finite
+// factor times zero.  In FP32 it is at least 5.2e-26, in FP64 at least 1e-205.
+// (The cube root comes first: (4 m / DBL_MAX)^2 itself underflows.)
+__device__ __forceinline__ void resident_tiny(double m_max, float &tiny32, double &tiny64) {
+    const double m = fmax(m_max, 1.0);
+    const double c32 = cbrt(4.0 * m / (double)FLT_MAX), c64 = cbrt(4.0 * m / DBL_MAX);
+    tiny32 = (float)(c32 * c32);
+    tiny64 = c64 * c64;
+}
+
+// State of one resident wave: position p = lane + 64 k holds one particle
+// (see k_batch_resident), the massive ones at positions 0 .. nm - 1.
 template <int NSLOT>
 struct ResState {
     double X[NSLOT], Y[NSLOT], Z[NSLOT], VX[NSLOT], VY[NSLOT], VZ[NSLOT], AX[NSLOT], AY[NSLOT],
         AZ[NSLOT], M[NSLOT];
     bool act[NSLOT];
     uint64_t mmask[NSLOT];   // massive particles (sources)
-    uint64_t cmask[NSLOT];   // of which the central bodies (FP64 terms)
+    int nm, nc;              // massive particles, of which the first nc are central (FP64 terms)
+    float tiny32;            // r^2 offsets of the unsoftened pairs (resident_tiny)
+    double tiny64;
 };
 
-// nsteps KDK steps of one system; SOFT (h > 0) is wave-uniform and chosen
-// outside the step loop, so the source loops carry no branches.
+// One source of a resident wave in LDS, 32 B: a central body as FP64
+// (x, y, z, m), any other as FP32 (x_hi, y_hi, z_hi, m) and (x_lo, y_lo, z_lo, 0)
+// relative to the wave's origin.
+struct ResSrc {
+    float4 a, b;
+};
+
+__device__ __forceinline__ float4 f64x2_bits(double u, double v) {
+    const unsigned long long bu = __builtin_bit_cast(unsigned long long, u);
+    const unsigned long long bv = __builtin_bit_cast(unsigned long long, v);
+    return make_float4(__builtin_bit_cast(float, (unsigned)bu), __builtin_bit_cast(float, (unsigned)(bu >> 32)),
+                       __builtin_bit_cast(float, (unsigned)bv), __builtin_bit_cast(float, (unsigned)(bv >> 32)));
+}
+__device__ __forceinline__ void f64x2_of(float4 f, double &u, double &v) {
+    u = __builtin_bit_cast(double, ((unsigned long long)__builtin_bit_cast(unsigned, f.y) << 32) |
+                                       __builtin_bit_cast(unsigned, f.x));
+    v = __builtin_bit_cast(double, ((unsigned long long)__builtin_bit_cast(unsigned, f.w) << 32) |
+                                       __builtin_bit_cast(unsigned, f.z));
+}
+
+// nsteps KDK steps of one wave's particles; SOFT (h > 0) is wave-uniform and
+// chosen outside the step loop, so the source loops carry no branches.  Each
+// step the massive lanes publish their source records to the wave's own LDS
+// array src (positions 0 .. nm - 1), which every lane then reads as
+// broadcasts.  A wavefront's LDS instructions complete in issue order, so the
+// wavefront-scope fences (code-motion barriers here) are all the ordering
+// the write-then-read and read-then-next-write sequences need.
 template <int NSLOT, bool SOFT>
-__device__ __forceinline__ void resident_run(const ResArgs &a, ResState<NSLOT> &r, int s, int lane,
-                                             int nsteps, double G, double dt, double hsys) {
+__device__ __forceinline__ void resident_run(const ResArgs &a, ResState<NSLOT> &r, ResSrc *src, int s,
+                                             int lane, bool energy_wave, int nsteps, double G, double dt,
+                                             double hsys) {
     const double hinv = SOFT ? 1.0 / hsys : 0.0, hinv3 = hinv * hinv * hinv;
     const float hf = (float)hsys, h2f = hf * hf, h2cf = fmaxf(h2f, FLT_MIN);
     const float hinvf = SOFT ? 1.0f / hf : 0.0f, hinv3f = hinvf * hinvf * hinvf;
+    const int nm = r.nm, nc = r.nc;
+    const float tiny32 = r.tiny32;
+    const double tiny64 = r.tiny64;
     int e_slot = 0;
     for (int step = 0; step < nsteps; ++step) {
         // drift (FP64)
@@ -389,9 +437,10 @@ __device__ __forceinline__ void resident_run(const ResArgs &a, ResState<NSLOT> &
         for (int k = 0; k < NSLOT; ++k) {
             r.X[k] += r.VX[k] * dt; r.Y[k] += r.VY[k] * dt; r.Z[k] += r.VZ[k] * dt;
         }
-        // origin: the system's first particle (the central body of a planetary system)
+        // origin: the system's first massive particle (the central body of a
+        // planetary system)
         const double ox = readlane_f64(r.X[0], 0), oy = readlane_f64(r.Y[0], 0), oz = readlane_f64(r.Z[0], 0);
-        float xh[NSLOT], yh[NSLOT], zh[NSLOT], xl[NSLOT], yl[NSLOT], zl[NSLOT], mf[NSLOT];
+        float xh[NSLOT], yh[NSLOT], zh[NSLOT], xl[NSLOT], yl[NSLOT], zl[NSLOT];
         float fx[NSLOT], fy[NSLOT], fz[NSLOT], fq[NSLOT];
         double Ax[NSLOT], Ay[NSLOT], Az[NSLOT], Aq[NSLOT];
 #pragma unroll
@@ -401,60 +450,52 @@ __device__ __forceinline__ void resident_run(const ResArgs &a, ResState<NSLOT> &
             xl[k] = (float)(dx - (double)xh[k]);
             yl[k] = (float)(dy - (double)yh[k]);
             zl[k] = (float)(dz - (double)zh[k]);
-            mf[k] = (float)r.M[k];
             fx[k] = fy[k] = fz[k] = fq[k] = 0.0f;
             Ax[k] = Ay[k] = Az[k] = Aq[k] = 0.0;
-        }
-#pragma unroll
-        for (int ks = 0; ks < NSLOT; ++ks) {
-            // central bodies: FP64 terms
-            uint64_t mc = r.cmask[ks];
-            while (mc) {
-                const int j = (int)__builtin_ctzll(mc);
-                mc &= mc - 1;
-                const double sx = readlane_f64(r.X[ks], j), sy = readlane_f64(r.Y[ks], j);
-                const double sz = readlane_f64(r.Z[ks], j), sm = readlane_f64(r.M[ks], j);
-#pragma unroll
-                for (int kt = 0; kt < NSLOT; ++kt) {
-                    if constexpr (SOFT) {
-                        pair_f64<false>(sx, sy, sz, sm, r.X[kt], r.Y[kt], r.Z[kt], hsys, hinv, hinv3,
-                                        Ax[kt], Ay[kt], Az[kt], Aq[kt]);
-                    } else {
-                        const double dx = sx - r.X[kt], dy = sy - r.Y[kt], dz = sz - r.Z[kt];
-                        const double r2 = dx * dx + dy * dy + dz * dz;
-                        const double ri = rsq_f64(r2);
-                        const double fac = r2 > 0.0 ? sm * (ri * ri) * ri : 0.0;
-                        Ax[kt] += dx * fac; Ay[kt] += dy * fac; Az[kt] += dz * fac;
-                    }
-                }
+            const int p = lane + 64 * k;
+            if (p < nc) {
+                src[p].a = f64x2_bits(r.X[k], r.Y[k]);
+                src[p].b = f64x2_bits(r.Z[k], r.M[k]);
+            } else if (p < nm) {
+                src[p].a = make_float4(xh[k], yh[k], zh[k], (float)r.M[k]);
+                src[p].b = make_float4(xl[k], yl[k], zl[k], 0.0f);
             }
-            // the other massive bodies: FP32 hi/lo terms, two sources per trip
-            uint64_t mm = r.mmask[ks] & ~r.cmask[ks];
-            while (mm) {
-                const int j0 = (int)__builtin_ctzll(mm);
-                mm &= mm - 1;
-                const int j1 = mm ? (int)__builtin_ctzll(mm) : j0;
-                const bool two = mm != 0;
-                mm &= mm - 1;
-                const float axh = readlane_f32(xh[ks], j0), ayh = readlane_f32(yh[ks], j0);
-                const float azh = readlane_f32(zh[ks], j0), axl = readlane_f32(xl[ks], j0);
-                const float ayl = readlane_f32(yl[ks], j0), azl = readlane_f32(zl[ks], j0);
-                const float am = readlane_f32(mf[ks], j0);
-                const float bxh = readlane_f32(xh[ks], j1), byh = readlane_f32(yh[ks], j1);
-                const float bzh = readlane_f32(zh[ks], j1), bxl = readlane_f32(xl[ks], j1);
-                const float byl = readlane_f32(yl[ks], j1), bzl = readlane_f32(zl[ks], j1);
-                const float bm = two ? readlane_f32(mf[ks], j1) : 0.0f;   // one left: massless copy
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        // central bodies: FP64 terms
+        for (int j = 0; j < nc; ++j) {
+            double sx, sy, sz, sm;
+            f64x2_of(src[j].a, sx, sy);
+            f64x2_of(src[j].b, sz, sm);
 #pragma unroll
-                for (int kt = 0; kt < NSLOT; ++kt) {
-                    res_pair_f32<SOFT>(axh, ayh, azh, axl, ayl, azl, am, xh[kt], yh[kt], zh[kt], xl[kt],
-                                       yl[kt], zl[kt], h2cf, h2f, hinvf, hinv3f, fx[kt], fy[kt], fz[kt],
-                                       fq[kt]);
-                    res_pair_f32<SOFT>(bxh, byh, bzh, bxl, byl, bzl, bm, xh[kt], yh[kt], zh[kt], xl[kt],
-                                       yl[kt], zl[kt], h2cf, h2f, hinvf, hinv3f, fx[kt], fy[kt], fz[kt],
-                                       fq[kt]);
+            for (int kt = 0; kt < NSLOT; ++kt) {
+                if constexpr (SOFT) {
+                    pair_f64<false>(sx, sy, sz, sm, r.X[kt], r.Y[kt], r.Z[kt], hsys, hinv, hinv3, Ax[kt],
+                                    Ay[kt], Az[kt], Aq[kt]);
+                } else {
+                    const double dx = sx - r.X[kt], dy = sy - r.Y[kt], dz = sz - r.Z[kt];
+                    const double r2 = dx * dx + dy * dy + (dz * dz + tiny64);
+                    const double ri = rsq_f64(r2);
+                    const double fac = sm * (ri * ri) * ri;
+                    Ax[kt] += dx * fac; Ay[kt] += dy * fac; Az[kt] += dz * fac;
                 }
             }
         }
+        // the other massive bodies: FP32 hi/lo terms, two sources per trip
+        // (position nm holds a massless record for an odd count)
+        for (int j = nc; j < nm; j += 2) {
+            const float4 ah = src[j].a, al = src[j].b, bh = src[j + 1].a, bl = src[j + 1].b;
+#pragma unroll
+            for (int kt = 0; kt < NSLOT; ++kt) {
+                res_pair_f32<SOFT>(ah.x, ah.y, ah.z, al.x, al.y, al.z, ah.w, xh[kt], yh[kt], zh[kt], xl[kt],
+                                   yl[kt], zl[kt], h2cf, h2f, hinvf, hinv3f, tiny32, fx[kt], fy[kt], fz[kt],
+                                   fq[kt]);
+                res_pair_f32<SOFT>(bh.x, bh.y, bh.z, bl.x, bl.y, bl.z, bh.w, xh[kt], yh[kt], zh[kt], xl[kt],
+                                   yl[kt], zl[kt], h2cf, h2f, hinvf, hinv3f, tiny32, fx[kt], fy[kt], fz[kt],
+                                   fq[kt]);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         // kick (FP64)
 #pragma unroll
         for (int k = 0; k < NSLOT; ++k) {
@@ -463,7 +504,7 @@ __device__ __forceinline__ void resident_run(const ResArgs &a, ResState<NSLOT> &
             r.AZ[k] = G * (Az[k] + (double)fz[k]);
             r.VX[k] += r.AX[k] * dt; r.VY[k] += r.AY[k] * dt; r.VZ[k] += r.AZ[k] * dt;
         }
-        if (a.energy && (step + 1) % a.energy_every == 0) {
+        if (a.energy && energy_wave && (step + 1) % a.energy_every == 0) {
             // total energy with velocities synchronised to the step; the
             // potential in FP64 over the massive sources (the reference's
             // pair potential, src/directsum/potential.jl:2-27)
@@ -503,52 +544,279 @@ __device__ __forceinline__ void resident_run(const ResArgs &a, ResState<NSLOT> &
     }
 }
 
-// One wavefront per system: lane l holds particles l, l + 64, ... (NSLOT
-// slots), in the caller's order, in registers for the whole launch.  Each
-// massive particle is a source: its coordinates are broadcast from the lane
-// that holds it with v_readlane (no LDS, no barriers).  Pair terms of the
-// central bodies (ALL_F64: of every source) are FP64; the others are FP32
-// hi/lo terms relative to the system's first particle.  Drift and kick are
-// FP64.  Every `energy_every` steps the system's total energy (velocities
-// synchronised to the step, FP64 potential over the massive sources) goes to
-// energy[s * energy_slots + k].
+// W wavefronts per system (block of 64 W threads, W <= 4), each holding the
+// system's massive particles and a 1/W share of its massless ones in
+// registers for the whole launch: position p = lane + 64 k (k < NSLOT) of
+// wave w holds the p-th massive particle (p < nM, the same in every wave) or
+// massless particle w * share + p - nM (share = ceil(nT / W)), particles
+// counted in the caller's order.  Massless particles are no sources, so each
+// wave integrates its own copy of the massive ones (identical instructions on
+// identical data, hence identical values) and the waves never exchange data.
+// Each massive particle is a source: its coordinates are broadcast from the
+// lane that holds it with v_readlane.  Pair terms of the central bodies (the
+// first ncentral particles; ALL_F64: of every source) are FP64; the others are
+// FP32 hi/lo terms relative to the first massive particle.  Drift and kick
+// are FP64.  Every `energy_every` steps wave 0, which holds every massive
+// particle, writes the system's total energy (velocities synchronised to the
+// step, FP64 potential over the massive sources; massless particles add
+// nothing) to energy[s * energy_slots + k].
+// The host picks NSLOT and W with nM + share <= 64 NSLOT for every system.
 template <int NSLOT, bool ALL_F64>
-__global__ __launch_bounds__(64) void k_batch_resident(ResArgs a, int nsteps) {
+__global__ __launch_bounds__(256) void k_batch_resident(ResArgs a, int nsteps) {
     const int s = blockIdx.x;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, W = blockDim.x >> 6;
     const int64_t o0 = a.off[s];
     const int ns = (int)(a.off[s + 1] - o0);
     if (ns <= 0) {   // an empty system has zero energy
-        if (a.energy)
+        if (a.energy && wave == 0)
             for (int k = lane; k < a.energy_slots; k += 64) a.energy[(int64_t)s * a.energy_slots + k] = 0.0;
         return;
     }
+    // particle of each (wave, position), -1 for none (systems of up to 1024
+    // particles)
+    __shared__ int16_t held[4][64 * NSLOT];
+    if (wave == 0) {
+#pragma unroll
+        for (int w = 0; w < 4; ++w)
+#pragma unroll
+            for (int k = 0; k < NSLOT; ++k) held[w][lane + 64 * k] = -1;
+        int nm = 0;
+        for (int base = 0; base < ns; base += 64)
+            nm += __builtin_popcountll(__builtin_amdgcn_ballot_w64(base + lane < ns && a.m[o0 + base + lane] != 0.0));
+        const int share = (ns - nm + W - 1) / W;
+        const uint64_t below = (1ull << lane) - 1ull;
+        int before = 0;   // massive particles before this chunk of 64
+        for (int base = 0; base < ns; base += 64) {
+            const int p = base + lane;
+            const bool massive = p < ns && a.m[o0 + p] != 0.0;
+            const uint64_t mk = __builtin_amdgcn_ballot_w64(massive);
+            const int rm = before + __builtin_popcountll(mk & below);
+            before += __builtin_popcountll(mk);
+            if (p >= ns) continue;
+            if (massive) {
+                if (rm < 64 * NSLOT)
+                    for (int w = 0; w < W; ++w) held[w][rm] = (int16_t)p;
+            } else {
+                const int rt = p - rm, w = rt / share, pos = nm + rt % share;
+                if (pos < 64 * NSLOT) held[w][pos] = (int16_t)p;
+            }
+        }
+    }
+    __syncthreads();
     ResState<NSLOT> r;
+    int pid[NSLOT];
     const int ncentral = ALL_F64 ? 1 << 30 : a.ncentral;
+    r.nm = r.nc = 0;
 #pragma unroll
     for (int k = 0; k < NSLOT; ++k) {
-        const int li = lane + 64 * k;
-        const bool act = li < ns;
+        const int p = held[wave][lane + 64 * k];
+        pid[k] = p;
+        const bool act = p >= 0;
         r.act[k] = act;
-        const int64_t i = o0 + li;
+        const int64_t i = o0 + (act ? p : 0);
         r.X[k] = act ? a.x[i] : 0.0; r.Y[k] = act ? a.y[i] : 0.0; r.Z[k] = act ? a.z[i] : 0.0;
         r.VX[k] = act ? a.vx[i] : 0.0; r.VY[k] = act ? a.vy[i] : 0.0; r.VZ[k] = act ? a.vz[i] : 0.0;
         r.AX[k] = act ? a.ax[i] : 0.0; r.AY[k] = act ? a.ay[i] : 0.0; r.AZ[k] = act ? a.az[i] : 0.0;
         r.M[k] = act ? a.m[i] : 0.0;
         r.mmask[k] = __builtin_amdgcn_ballot_w64(act && r.M[k] != 0.0);
-        r.cmask[k] = __builtin_amdgcn_ballot_w64(act && r.M[k] != 0.0 && li < ncentral);
+        r.nm += __builtin_popcountll(r.mmask[k]);
+        // the central bodies are the first massive positions
+        r.nc += __builtin_popcountll(__builtin_amdgcn_ballot_w64(act && r.M[k] != 0.0 && p < ncentral));
+    }
+    double m_max = 0.0;
+#pragma unroll
+    for (int k = 0; k < NSLOT; ++k) m_max = fmax(m_max, r.act[k] ? fabs(r.M[k]) : 0.0);
+    resident_tiny(wave_max(m_max), r.tiny32, r.tiny64);
+    // 64 NSLOT + 1 records per wave: position nm is a massless record at the
+    // origin (it pads an odd count of FP32 sources)
+    extern __shared__ ResSrc res_src[];
+    ResSrc *src = res_src + wave * (64 * NSLOT + 1);
+    if (lane == 0) {
+        src[r.nm].a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        src[r.nm].b = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     const double G = a.Gs[s], dt = a.dts[s], hsys = a.hs[s];
-    if (hsys > 0.0) resident_run<NSLOT, true>(a, r, s, lane, nsteps, G, dt, hsys);
-    else resident_run<NSLOT, false>(a, r, s, lane, nsteps, G, dt, hsys);
+    const bool energy_wave = wave == 0;
+    if (hsys > 0.0) resident_run<NSLOT, true>(a, r, src, s, lane, energy_wave, nsteps, G, dt, hsys);
+    else resident_run<NSLOT, false>(a, r, src, s, lane, energy_wave, nsteps, G, dt, hsys);
 #pragma unroll
     for (int k = 0; k < NSLOT; ++k) {
-        if (!r.act[k]) continue;
-        const int64_t i = o0 + lane + 64 * k;
+        // the massive particles are written by wave 0 only
+        if (!r.act[k] || (wave > 0 && r.M[k] != 0.0)) continue;
+        const int64_t i = o0 + pid[k];
         a.x[i] = r.X[k]; a.y[i] = r.Y[k]; a.z[i] = r.Z[k];
         a.vx[i] = r.VX[k]; a.vy[i] = r.VY[k]; a.vz[i] = r.VZ[k];
         a.ax[i] = r.AX[k]; a.ay[i] = r.AY[k]; a.az[i] = r.AZ[k];
     }
+}
+
+// ---- packed resident kernel -----------------------------------------------
+// Several systems per workgroup, split by role: the systems' massive
+// particles fill the first wavefronts (each system's within one wave), their
+// massless particles the waves after, so a planetary system's 64 test
+// particles take one full wave and the massive waves serve several systems.
+// Each step every massive lane publishes its 32-byte source record (ResSrc:
+// FP64 for a central body, FP32 hi/lo relative to its system's origin
+// otherwise) into one of two LDS buffers; one workgroup barrier later every
+// lane reads its own system's records.  The buffer alternates with the step,
+// so a step's writes never meet the previous step's reads (every lane has
+// passed this step's barrier only after those reads) and one barrier per step
+// suffices.  A system's records: slots 0 .. NCB - 1 for its central bodies,
+// NCB .. NMB - 1 for the others, massless zero records where it has fewer
+// (NCB, NMB: the block's largest counts), and one more zero record that pads
+// an odd FP32 count.  The origin of a system is the position its first
+// massive particle had when the launch began: the hi/lo split represents
+// x - origin exactly to about 2^-48 |x - origin| for any nearby origin, so it
+// need not follow the particle.
+struct PackArgs {
+    const int32_t *lane, *meta, *blk;
+    const double *tiny;
+    int smax;        // systems per block at most (block table stride 5 + 3 smax)
+    int recs;        // records per system
+    const double *Gs, *hs, *dts;
+    double *x, *y, *z, *vx, *vy, *vz;
+    const double *m;
+    double *ax, *ay, *az;
+    double *energy;      // [nsys][slots] or nullptr (zeroed by the host)
+    int energy_every;
+    int energy_slots;
+};
+
+// Block table: [0] NCB, [1] NMB, [2] systems in the block, [3] softened,
+// [4] unused, then smax system ids, smax origin particles (-1: none) and smax
+// massive counts.  meta: bits 16+ system in block, bit 15 first massive lane
+// of its system, bits 0-14 record slot (0x7fff: massless).
+constexpr int PK_NONE = 0x7fff;
+
+template <bool SOFT>
+__device__ __forceinline__ void packed_run(const PackArgs &a, const int32_t *bi, int p, int meta, int nsteps,
+                                           ResSrc *rec, ResSrc *erec, double *es) {
+    const int t = threadIdx.x;
+    const int NCB = bi[0], NMB = bi[1], nsb = bi[2];
+    const int sl = meta >> 16, slot = meta & PK_NONE;
+    const bool first = (meta >> 15) & 1;
+    const bool act = p >= 0, massive = act && slot != PK_NONE;
+    const int s = bi[5 + sl];
+    const int orig = bi[5 + a.smax + sl];
+    const int nm_s = bi[5 + 2 * a.smax + sl];
+    const double G = a.Gs[s], dt = a.dts[s], hsys = a.hs[s];
+    const double hinv = SOFT ? 1.0 / hsys : 0.0, hinv3 = hinv * hinv * hinv;
+    const float hf = (float)hsys, h2f = hf * hf, h2cf = fmaxf(h2f, FLT_MIN);
+    const float hinvf = SOFT ? 1.0f / hf : 0.0f, hinv3f = hinvf * hinvf * hinvf;
+    const float tiny32 = (float)a.tiny[2 * blockIdx.x];
+    const double tiny64 = a.tiny[2 * blockIdx.x + 1];
+    const double ox = orig >= 0 ? a.x[orig] : 0.0, oy = orig >= 0 ? a.y[orig] : 0.0,
+                 oz = orig >= 0 ? a.z[orig] : 0.0;
+    double X = act ? a.x[p] : 0.0, Y = act ? a.y[p] : 0.0, Z = act ? a.z[p] : 0.0;
+    double VX = act ? a.vx[p] : 0.0, VY = act ? a.vy[p] : 0.0, VZ = act ? a.vz[p] : 0.0;
+    double AX = act ? a.ax[p] : 0.0, AY = act ? a.ay[p] : 0.0, AZ = act ? a.az[p] : 0.0;
+    const double M = act ? a.m[p] : 0.0;
+    const int rb = sl * a.recs;
+    const int nrec = nsb * a.recs;
+    int e_slot = 0;
+    for (int step = 0; step < nsteps; ++step) {
+        ResSrc *src = rec + (step & 1) * nrec;
+        X += VX * dt; Y += VY * dt; Z += VZ * dt;
+        const double dx0 = X - ox, dy0 = Y - oy, dz0 = Z - oz;
+        const float xh = (float)dx0, yh = (float)dy0, zh = (float)dz0;
+        const float xl = (float)(dx0 - (double)xh), yl = (float)(dy0 - (double)yh), zl = (float)(dz0 - (double)zh);
+        if (massive) {
+            if (slot < NCB) {
+                src[rb + slot].a = f64x2_bits(X, Y);
+                src[rb + slot].b = f64x2_bits(Z, M);
+            } else {
+                src[rb + slot].a = make_float4(xh, yh, zh, (float)M);
+                src[rb + slot].b = make_float4(xl, yl, zl, 0.0f);
+            }
+        }
+        __syncthreads();
+        double Ax = 0.0, Ay = 0.0, Az = 0.0, Aq = 0.0;
+        for (int j = 0; j < NCB; ++j) {
+            double sx, sy, sz, sm;
+            f64x2_of(src[rb + j].a, sx, sy);
+            f64x2_of(src[rb + j].b, sz, sm);
+            if constexpr (SOFT) {
+                pair_f64<false>(sx, sy, sz, sm, X, Y, Z, hsys, hinv, hinv3, Ax, Ay, Az, Aq);
+            } else {
+                const double dx = sx - X, dy = sy - Y, dz = sz - Z;
+                const double r2 = dx * dx + dy * dy + (dz * dz + tiny64);
+                const double ri = rsq_f64(r2);
+                const double fac = sm * (ri * ri) * ri;
+                Ax += dx * fac; Ay += dy * fac; Az += dz * fac;
+            }
+        }
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f, fq = 0.0f;
+        for (int j = NCB; j < NMB; j += 2) {
+            const float4 ah = src[rb + j].a, al = src[rb + j].b, bh = src[rb + j + 1].a, bl = src[rb + j + 1].b;
+            res_pair_f32<SOFT>(ah.x, ah.y, ah.z, al.x, al.y, al.z, ah.w, xh, yh, zh, xl, yl, zl, h2cf, h2f,
+                               hinvf, hinv3f, tiny32, fx, fy, fz, fq);
+            res_pair_f32<SOFT>(bh.x, bh.y, bh.z, bl.x, bl.y, bl.z, bh.w, xh, yh, zh, xl, yl, zl, h2cf, h2f,
+                               hinvf, hinv3f, tiny32, fx, fy, fz, fq);
+        }
+        AX = G * (Ax + (double)fx);
+        AY = G * (Ay + (double)fy);
+        AZ = G * (Az + (double)fz);
+        VX += AX * dt; VY += AY * dt; VZ += AZ * dt;
+        if (a.energy && (step + 1) % a.energy_every == 0) {
+            // total energy with velocities synchronised to the step: FP64
+            // records of every massive particle, the FP64 potential over them
+            // (src/directsum/potential.jl:2-27), a sum over the system's
+            // massive lanes in lane order; massless particles add nothing
+            if (massive) {
+                erec[rb + slot].a = f64x2_bits(X, Y);
+                erec[rb + slot].b = f64x2_bits(Z, M);
+            }
+            __syncthreads();
+            double e = 0.0;
+            if (massive) {
+                double q = 0.0;
+                for (int j = 0; j < NMB; ++j) {
+                    double sx, sy, sz, sm;
+                    f64x2_of(erec[rb + j].a, sx, sy);
+                    f64x2_of(erec[rb + j].b, sz, sm);
+                    double bx = 0.0, by = 0.0, bz = 0.0;
+                    if (j != slot && sm != 0.0)
+                        pair_f64<true>(sx, sy, sz, sm, X, Y, Z, hsys, SOFT ? hinv : 0.0, SOFT ? hinv3 : 0.0, bx,
+                                       by, bz, q);
+                }
+                const double ux = VX - 0.5 * AX * dt, uy = VY - 0.5 * AY * dt, uz = VZ - 0.5 * AZ * dt;
+                e = 0.5 * M * (ux * ux + uy * uy + uz * uz) - 0.5 * G * M * q;
+            }
+            es[t] = e;
+            __syncthreads();
+            if (first && e_slot < a.energy_slots) {
+                double sum = 0.0;
+                for (int k = 0; k < nm_s; ++k) sum += es[t + k];
+                a.energy[(int64_t)s * a.energy_slots + e_slot] = sum;
+            }
+            ++e_slot;
+        }
+    }
+    if (act) {
+        a.x[p] = X; a.y[p] = Y; a.z[p] = Z;
+        a.vx[p] = VX; a.vy[p] = VY; a.vz[p] = VZ;
+        a.ax[p] = AX; a.ay[p] = AY; a.az[p] = AZ;
+    }
+}
+
+// One workgroup per block of the packed layout (k_batch_resident_packed's
+// host side, build_packed, makes the tables); the zero records are written
+// first.  Every lane, idle ones included, takes every barrier.
+__global__ __launch_bounds__(1024) void k_batch_resident_packed(PackArgs a, int nsteps) {
+    extern __shared__ ResSrc pk_src[];
+    const int32_t *bi = a.blk + (int64_t)blockIdx.x * (5 + 3 * a.smax);
+    const int nrec = bi[2] * a.recs;
+    ResSrc *rec = pk_src, *erec = pk_src + 2 * nrec;
+    double *es = reinterpret_cast<double *>(pk_src + 3 * nrec);
+    for (int i = threadIdx.x; i < 3 * nrec; i += blockDim.x) {
+        pk_src[i].a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        pk_src[i].b = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    __syncthreads();
+    const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = a.lane[li], meta = a.meta[li];
+    if (bi[3]) packed_run<true>(a, bi, p, meta, nsteps, rec, erec, es);
+    else packed_run<false>(a, bi, p, meta, nsteps, rec, erec, es);
 }
 
 // ---- Morton order within systems -----------------------------------------
@@ -683,8 +951,21 @@ static void bdrop_graph(nb_batch *b) {
     b->g_steps = 0;
 }
 
+static int64_t resident_lanes(const nb_batch *b, int W);
+
+// The resident kernel runs when some wave count W <= 4 per system fits every
+// system's massive particles plus a 1/W share of its massless ones into 4
+// lane slots (256 lanes) per wave; before the masses are known, when every
+// system has at most 256 particles.
+static bool packed_ok(const nb_batch *b);
+
 static bool resident_ok(const nb_batch *b) {
-    return b->tun.resident && b->max_ns <= 256 && b->nsys > 0;
+    if (!b->tun.resident || b->nsys <= 0) return false;
+    if (b->h_nm.size() != (size_t)b->nsys) return b->max_ns <= 256;
+    if (packed_ok(b)) return true;
+    for (int W = 1; W <= 4; ++W)
+        if (resident_lanes(b, W) <= 256) return true;
+    return false;
 }
 
 // (system, target tile) items, the most expensive (largest system) first.
@@ -900,6 +1181,192 @@ static int enqueue_steps(nb_batch *b, int64_t S, bool timing) {
     return NB_OK;
 }
 
+// Lanes a resident wave needs when each system runs on W waves: its massive
+// particles plus a 1/W share of the massless ones.
+static int64_t resident_lanes(const nb_batch *b, int W) {
+    int64_t need = 0;
+    for (int32_t q = 0; q < b->nsys; ++q) {
+        const int64_t ns = b->h_off[q + 1] - b->h_off[q], nm = b->h_nm[(size_t)q];
+        need = std::max(need, nm + (ns - nm + W - 1) / W);
+    }
+    return need;
+}
+
+// Waves per system and lane slots per wave of the resident kernel: the
+// fewest wave slots (waves x slots) per system, ties to fewer waves (the
+// kernel is bound by VALU issue, and every extra wave repeats the massive
+// particles' arithmetic and the per-step overhead: on 2048 planetary
+// systems of 73 bodies one wave of two slots took 1.74 us per step, two
+// waves of one slot 1.85 us); tun.resident_waves forces the wave count where
+// its lanes fit 4 slots.
+static void resident_shape(const nb_batch *b, int &waves, int &nslot) {
+    int best = 1 << 30;
+    for (int W = 1; W <= 4; ++W) {
+        const int ns = (int)std::max<int64_t>(1, (resident_lanes(b, W) + 63) / 64);
+        if (ns > 4) continue;
+        if (b->tun.resident_waves == W) {
+            waves = W;
+            nslot = ns;
+            return;
+        }
+        if (W * ns < best) {
+            best = W * ns;
+            waves = W;
+            nslot = ns;
+        }
+    }
+}
+
+// The packed layout applies when no wave count is forced and every system
+// has at most 64 massive particles and fits one workgroup (one massive wave
+// plus its massless particles in at most 15 more).
+static bool packed_ok(const nb_batch *b) {
+    if (b->tun.resident_waves != 0 || b->h_nm.size() != (size_t)b->nsys) return false;
+    for (int32_t q = 0; q < b->nsys; ++q) {
+        const int64_t ns = b->h_off[q + 1] - b->h_off[q], nm = b->h_nm[(size_t)q];
+        if (nm > 64 || 1 + (ns - nm + 63) / 64 > 16) return false;
+    }
+    return true;
+}
+
+// Builds the packed tables.  Systems are ordered by (softened, central count,
+// massive count) and taken up to S at a time, S = nsys / (2 CUs) clamped to
+// 1..16 (two blocks per CU on large ensembles), a block closing early when
+// the next system is softened differently or would take it past 16 waves.
+// Massive particles are packed into the block's first waves without splitting
+// a system; massless ones follow, consecutively.
+static int build_packed(nb_batch *b) {
+    const int32_t ns_sys = b->nsys;
+    const bool f64 = b->mode == NB_MODE_FP64;
+    const int ncentral = b->mode == NB_MODE_SPLIT_CENTRAL ? b->ncentral : 0;
+    std::vector<int32_t> nc(ns_sys), order(ns_sys);
+    for (int32_t q = 0; q < ns_sys; ++q) {
+        int c = 0;
+        for (int64_t i = b->h_off[q]; i < b->h_off[q + 1]; ++i)
+            c += b->h_massive[(size_t)i] && (f64 || i - b->h_off[q] < ncentral);
+        nc[q] = c;
+        order[q] = q;
+    }
+    auto soft = [&](int32_t q) { return b->hh[(size_t)q] > 0.0; };
+    std::stable_sort(order.begin(), order.end(), [&](int32_t p, int32_t q) {
+        if (soft(p) != soft(q)) return soft(p) < soft(q);
+        if (nc[p] != nc[q]) return nc[p] < nc[q];
+        return b->h_nm[(size_t)p] < b->h_nm[(size_t)q];
+    });
+    // records per system are at most 65 (32 B each, three buffers): S also
+    // keeps the workgroup's LDS within 64 KiB
+    int rmax = 1;
+    for (int32_t q = 0; q < ns_sys; ++q) rmax = std::max(rmax, b->h_nm[(size_t)q] + 1);
+    const int s_lds = std::max(1, (65536 - 1024 * (int)sizeof(double)) / (3 * rmax * (int)sizeof(ResSrc)));
+    const int S = (int)std::min<int64_t>(std::min(16, s_lds),
+                                         std::max<int64_t>(1, ns_sys / (2 * (int64_t)b->cu_count)));
+    struct Blk {
+        std::vector<int32_t> sys;
+        int mw = 0, lanes_in_mw = 64, nt = 0;   // massive waves, lanes used in the last one, massless
+    };
+    std::vector<Blk> blks;
+    auto waves = [](const Blk &k) { return k.mw + (k.nt + 63) / 64; };
+    for (int32_t q : order) {
+        const int nm = b->h_nm[(size_t)q];
+        const int nt = (int)(b->h_off[q + 1] - b->h_off[q]) - nm;
+        bool open = !blks.empty() && (int)blks.back().sys.size() < S && soft(blks.back().sys[0]) == soft(q);
+        if (open) {
+            Blk t = blks.back();
+            if (nm > 0 && t.lanes_in_mw + nm > 64) { ++t.mw; t.lanes_in_mw = 0; }
+            t.lanes_in_mw += nm;
+            t.nt += nt;
+            open = waves(t) <= 16;
+        }
+        if (!open) blks.push_back(Blk{});
+        Blk &k = blks.back();
+        if (nm > 0 && k.lanes_in_mw + nm > 64) { ++k.mw; k.lanes_in_mw = 0; }
+        k.lanes_in_mw += nm;
+        k.nt += nt;
+        k.sys.push_back(q);
+    }
+    int maxw = 1, smax = 1, ncb_max = 0, nfb_max = 0;
+    for (const Blk &k : blks) {
+        maxw = std::max(maxw, waves(k));
+        smax = std::max(smax, (int)k.sys.size());
+        for (int32_t q : k.sys) {
+            ncb_max = std::max(ncb_max, nc[q]);
+            nfb_max = std::max(nfb_max, b->h_nm[(size_t)q] - nc[q]);
+        }
+    }
+    const int threads = 64 * maxw, stride = 5 + 3 * smax, recs = ncb_max + nfb_max + 1;
+    const size_t nb = blks.size();
+    std::vector<int32_t> lane(nb * threads, -1), meta(nb * threads, (int32_t)PK_NONE), blk(nb * stride, -1);
+    std::vector<double> tiny(2 * nb);
+    for (size_t kb = 0; kb < nb; ++kb) {
+        const Blk &k = blks[kb];
+        int ncb = 0, nfb = 0;
+        double mmax = 0.0;
+        for (int32_t q : k.sys) {
+            ncb = std::max(ncb, nc[q]);
+            nfb = std::max(nfb, b->h_nm[(size_t)q] - nc[q]);
+            mmax = std::max(mmax, b->h_mmax[(size_t)q]);
+        }
+        int32_t *bi = &blk[kb * stride];
+        bi[0] = ncb; bi[1] = ncb + nfb; bi[2] = (int32_t)k.sys.size(); bi[3] = soft(k.sys[0]); bi[4] = 0;
+        const double m = std::max(mmax, 1.0);
+        const double c32 = std::cbrt(4.0 * m / (double)FLT_MAX), c64 = std::cbrt(4.0 * m / DBL_MAX);
+        tiny[2 * kb] = (double)(float)(c32 * c32);
+        tiny[2 * kb + 1] = c64 * c64;
+        // massive waves first (each system's massive particles in one wave),
+        // then the massless particles
+        int mw_used = 0, cur_w = 0, in_w = 0;
+        for (size_t j = 0; j < k.sys.size(); ++j) {
+            const int32_t q = k.sys[j];
+            const int nm = b->h_nm[(size_t)q];
+            bi[5 + j] = q;
+            bi[5 + smax + j] = -1;
+            bi[5 + 2 * smax + j] = nm;
+            if (nm == 0) continue;
+            if (in_w + nm > 64) { ++cur_w; in_w = 0; }
+            int mlane = 64 * cur_w + in_w;
+            in_w += nm;
+            mw_used = cur_w + 1;
+            int ci = 0, fi = 0;
+            bool firstm = true;
+            for (int64_t i = b->h_off[q]; i < b->h_off[q + 1]; ++i) {
+                if (!b->h_massive[(size_t)i]) continue;
+                if (bi[5 + smax + j] < 0) bi[5 + smax + j] = (int32_t)i;
+                const bool central = f64 || i - b->h_off[q] < ncentral;
+                const int slot = central ? ci++ : ncb + fi++;
+                lane[kb * threads + mlane] = (int32_t)i;
+                meta[kb * threads + mlane] = ((int32_t)j << 16) | (firstm ? 0x8000 : 0) | slot;
+                firstm = false;
+                ++mlane;
+            }
+        }
+        int tlane = 64 * mw_used;
+        for (size_t j = 0; j < k.sys.size(); ++j) {
+            const int32_t q = k.sys[j];
+            for (int64_t i = b->h_off[q]; i < b->h_off[q + 1]; ++i) {
+                if (b->h_massive[(size_t)i]) continue;
+                lane[kb * threads + tlane] = (int32_t)i;
+                meta[kb * threads + tlane] = ((int32_t)j << 16) | PK_NONE;
+                ++tlane;
+            }
+        }
+    }
+    dfree(b->pk_lane); dfree(b->pk_meta); dfree(b->pk_blk); dfree(b->pk_tiny);
+    NB_TRY(dalloc(&b->pk_lane, sizeof(int32_t) * lane.size()));
+    NB_TRY(dalloc(&b->pk_meta, sizeof(int32_t) * meta.size()));
+    NB_TRY(dalloc(&b->pk_blk, sizeof(int32_t) * blk.size()));
+    NB_TRY(dalloc(&b->pk_tiny, sizeof(double) * tiny.size()));
+    NB_HIP(hipMemcpy(b->pk_lane, lane.data(), sizeof(int32_t) * lane.size(), hipMemcpyHostToDevice));
+    NB_HIP(hipMemcpy(b->pk_meta, meta.data(), sizeof(int32_t) * meta.size(), hipMemcpyHostToDevice));
+    NB_HIP(hipMemcpy(b->pk_blk, blk.data(), sizeof(int32_t) * blk.size(), hipMemcpyHostToDevice));
+    NB_HIP(hipMemcpy(b->pk_tiny, tiny.data(), sizeof(double) * tiny.size(), hipMemcpyHostToDevice));
+    b->pk_nblk = (int)nb;
+    b->pk_threads = threads;
+    b->pk_smax = smax;
+    b->pk_recs = recs;
+    b->pk_stale = false;
+    return NB_OK;
+}
+
 static int resident_steps(nb_batch *b, int64_t nsteps, int every, double *energy_dev, int slots) {
     ResArgs a{};
     a.off = b->off; a.Gs = b->Gs; a.hs = b->hs; a.dts = b->dts;
@@ -912,12 +1379,31 @@ static int resident_steps(nb_batch *b, int64_t nsteps, int every, double *energy
         return NB_ERR_INVALID;
     }
     if (b->nsys <= 0) return NB_OK;
-    const int nslot = (int)std::max<int64_t>(1, (b->max_ns + 63) / 64);   // resident_ok: <= 4
+    if (packed_ok(b)) {
+        if (b->pk_stale) NB_TRY(build_packed(b));
+        PackArgs pa{};
+        pa.lane = b->pk_lane; pa.meta = b->pk_meta; pa.blk = b->pk_blk; pa.tiny = b->pk_tiny;
+        pa.smax = b->pk_smax; pa.recs = b->pk_recs;
+        pa.Gs = b->Gs; pa.hs = b->hs; pa.dts = b->dts;
+        pa.x = b->x; pa.y = b->y; pa.z = b->z; pa.vx = b->vx; pa.vy = b->vy; pa.vz = b->vz;
+        pa.m = b->m; pa.ax = b->ax; pa.ay = b->ay; pa.az = b->az;
+        pa.energy = energy_dev; pa.energy_every = a.energy_every; pa.energy_slots = slots;
+        // systems without massive particles have zero energy and no lane that writes it
+        if (energy_dev)
+            NB_HIP(hipMemsetAsync(energy_dev, 0, sizeof(double) * (size_t)b->nsys * (size_t)slots, b->stream));
+        const size_t lds = 3 * (size_t)b->pk_smax * b->pk_recs * sizeof(ResSrc) + b->pk_threads * sizeof(double);
+        k_batch_resident_packed<<<b->pk_nblk, b->pk_threads, lds, b->stream>>>(pa, (int)nsteps);
+        return launch_ok("k_batch_resident_packed");
+    }
+    int waves = 1, nslot = 1;
+    resident_shape(b, waves, nslot);
     const bool f64 = b->mode == NB_MODE_FP64;
-#define NB_RES(NS_)                                                                     \
-    do {                                                                                \
-        if (f64) k_batch_resident<NS_, true><<<b->nsys, 64, 0, b->stream>>>(a, (int)nsteps); \
-        else k_batch_resident<NS_, false><<<b->nsys, 64, 0, b->stream>>>(a, (int)nsteps);    \
+    const dim3 block(64 * waves);
+    const size_t lds = sizeof(ResSrc) * (64 * (size_t)nslot + 1) * (size_t)waves;
+#define NB_RES(NS_)                                                                                \
+    do {                                                                                           \
+        if (f64) k_batch_resident<NS_, true><<<b->nsys, block, lds, b->stream>>>(a, (int)nsteps);  \
+        else k_batch_resident<NS_, false><<<b->nsys, block, lds, b->stream>>>(a, (int)nsteps);     \
     } while (0)
     if (nslot == 1) NB_RES(1);
     else if (nslot == 2) NB_RES(2);
@@ -1132,6 +1618,7 @@ int nb_batch_destroy(nb_batch *b) {
     dfree(b->tile_base); dfree(b->tile_sys); dfree(b->tile_box);
     dfree(b->items); dfree(b->counter); dfree(b->stage); dfree(b->red); dfree(b->wsrc);
     dfree(b->sort_tmp); dfree(b->keys); dfree(b->idx); dfree(b->sysbox); dfree(b->spare);
+    dfree(b->pk_lane); dfree(b->pk_meta); dfree(b->pk_blk); dfree(b->pk_tiny);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return NB_OK;
@@ -1149,6 +1636,7 @@ int nb_batch_set_params(nb_batch *b, const double *G, const double *soft, const 
             return NB_ERR_INVALID;
         }
     }
+    b->pk_stale = true;
     if (G != b->hG.data()) b->hG.assign(G, G + b->nsys);
     if (soft != b->hh.data()) b->hh.assign(soft, soft + b->nsys);
     if (dt != b->hdt.data()) b->hdt.assign(dt, dt + b->nsys);
@@ -1196,6 +1684,16 @@ int nb_batch_upload(nb_batch *b, const double *pos, const double *vel, const dou
         NB_HIP(hipMemcpyAsync(b->m, mass, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, s));
         NB_HIP(hipStreamSynchronize(s));
     }
+    b->h_nm.assign((size_t)b->nsys, 0);
+    b->h_mmax.assign((size_t)b->nsys, 0.0);
+    b->h_massive.assign((size_t)n, 0);
+    for (int32_t q = 0; q < b->nsys; ++q)
+        for (int64_t i = b->h_off[q]; i < b->h_off[q + 1]; ++i) {
+            b->h_massive[(size_t)i] = mass[i] != 0.0;
+            b->h_nm[(size_t)q] += mass[i] != 0.0;
+            b->h_mmax[(size_t)q] = std::max(b->h_mmax[(size_t)q], std::fabs(mass[i]));
+        }
+    b->pk_stale = true;
     b->uploaded = true;
     NB_TRY(nb_batch_set_params(b, b->hG.data(), b->hh.data(), b->hdt.data()));
     NB_TRY(rebuild_central(b));
@@ -1247,6 +1745,7 @@ int nb_batch_set_central(nb_batch *b, int32_t count) {
     bdrop_graph(b);
     dfree(b->cidx);
     b->ncentral = count;
+    b->pk_stale = true;
     NB_HIP(hipMemset(b->central, 0, (size_t)std::max<int64_t>(b->n, 1)));
     if (count > 0) {
         NB_TRY(dalloc(&b->cidx, sizeof(int64_t) * (size_t)b->nsys * (size_t)count));
@@ -1394,7 +1893,8 @@ int nb_batch_set_tuning(nb_batch *b, const nb_batch_tuning *t) {
     if (!t || (t->targets_per_thread != 1 && t->targets_per_thread != 2 &&
                t->targets_per_thread != 4) ||
         t->blocks < 0 || t->blocks > (1 << 20) || t->steps_per_graph < 2 ||
-        t->steps_per_graph > 4096 || t->sort_interval < 0 || t->far_path < 0 || t->far_path > 4) {
+        t->steps_per_graph > 4096 || t->sort_interval < 0 || t->far_path < 0 || t->far_path > 4 ||
+        t->resident_waves < 0 || t->resident_waves > 4) {
         set_error("invalid batch tuning");
         return NB_ERR_INVALID;
     }

@@ -25,7 +25,8 @@ class BatchTuning(C.Structure):
         ("sort_interval", C.c_int32),
         ("resident", C.c_int32),
         ("far_path", C.c_int32),
-        ("reserved", C.c_int32 * 2),
+        ("resident_waves", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -151,9 +152,9 @@ class Ensemble:
         call("nb_batch_sync", self.handle)
 
     def set_tuning(self, targets_per_thread=1, blocks=0, steps_per_graph=16, sort_interval=16,
-                   resident=1, far_path=0):
+                   resident=1, far_path=0, resident_waves=0):
         t = BatchTuning(int(targets_per_thread), int(blocks), int(steps_per_graph),
-                        int(sort_interval), int(resident), int(far_path))
+                        int(sort_interval), int(resident), int(far_path), int(resident_waves))
         call("nb_batch_set_tuning", self.handle, C.byref(t))
 
     def get_tuning(self) -> dict:
@@ -161,7 +162,8 @@ class Ensemble:
         call("nb_batch_get_tuning", self.handle, C.byref(t))
         return {"targets_per_thread": t.targets_per_thread, "blocks": t.blocks,
                 "steps_per_graph": t.steps_per_graph, "sort_interval": t.sort_interval,
-                "resident": t.resident, "far_path": t.far_path}
+                "resident": t.resident, "far_path": t.far_path,
+                "resident_waves": t.resident_waves}
 
     def autotune(self):
         call("nb_batch_autotune", self.handle)

@@ -4,9 +4,10 @@
 [0.4, 30], e < 0.1, i <= 2 deg, m/M log-uniform in [1e-7, 1e-3]) and 64
 massless test particles on circular orbits (r in [1, 40]), eps = 0, G = 1:
 the star's pair terms in FP64, the planets' in FP32 split precision, KDK
-steps at dt = 1e-3 in the resident kernel (one wavefront per system, the
-whole run in one launch), with every system's total energy tracked.  The
-reference runs one Simulation per system.
+steps at dt = 1e-3 in the resident kernel (the whole run in one launch;
+four systems per workgroup, their 36 massive bodies in one wavefront and
+each system's 64 test particles in another), with every system's total
+energy tracked.  The reference runs one Simulation per system.
 
     python astronbodysim.jl_amd/examples/planetary_ensemble.py [--nsys 2048] [--steps 10000]
         [--every 100] [--dt 1e-3]

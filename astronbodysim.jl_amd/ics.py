@@ -106,7 +106,7 @@ def _kepler_E(M, e):
     for _ in range(50):
         dE = (E - e * np.sin(E) - M) / (1.0 - e * np.cos(E))
         E -= dE
-        if np.max(np.abs(dE)) < 1e-15:
+        if dE.size == 0 or np.max(np.abs(dE)) < 1e-15:
             break
     return E
 

@@ -291,9 +291,15 @@ typedef struct nb_batch_tuning {
     int32_t blocks;             /* persistent blocks of the ragged kernel (0 = 8 per CU) */
     int32_t steps_per_graph;    /* KDK steps per captured HIP graph (even, >= 2)     */
     int32_t sort_interval;      /* re-sort every system into Morton order every k force calls (0 = never) */
-    int32_t resident;           /* 1: multi-step resident kernel when every system has <= 256 particles */
+    int32_t resident;           /* 1: multi-step resident kernel when every system fits it: its
+                                   massive particles plus a 1/W share of its massless ones in
+                                   <= 256 lanes for some W <= 4 (at most 256 particles, or
+                                   more with massless test particles) */
     int32_t far_path;           /* as nb_tuning.far_path */
-    int32_t reserved[2];
+    int32_t resident_waves;     /* wavefronts per system of the resident kernel, 1-4; each holds
+                                   every massive particle and a share of the massless ones
+                                   (0 = fewest lane slots, ties to fewer waves) */
+    int32_t reserved;
 } nb_batch_tuning;
 
 /* nsys systems; system s owns particles [offsets[s], offsets[s+1]) of the

@@ -228,8 +228,6 @@ _JL_FIELD = {"Int32": "int32_t", "Int64": "int64_t"}
 
 def test_julia_stub_structs_match_header():
     src = julia_stub()
-    hdr = re.sub(r"/\*.*?\*/", " ", "\n".join(h.read_text() for h in (ROOT / "include").glob("*.h")),
-                 flags=re.S)
     for jl, c in (("Tuning", "nb_tuning"), ("DevInfo", "nb_devinfo"), ("BatchTuning", "nb_batch_tuning")):
         m = re.search(r"struct %s\b[^\n]*\n(.*?)\nend" % jl, src, re.S)
         assert m, jl
@@ -239,16 +237,41 @@ def test_julia_stub_structs_match_header():
             t = re.fullmatch(r"NTuple\{(\d+), (\w+)\}", typ)
             jf.append((name, ({"UInt8": "char"}.get(t.group(2), _JL_FIELD.get(t.group(2))), int(t.group(1)))
                              if t else (_JL_FIELD[typ], 1)))
-        m = re.search(r"typedef struct %s \{(.*?)\} %s;" % (c, c), hdr, re.S)
-        assert m, c
-        cf = []
-        for line in m.group(1).split(";"):
-            line = re.sub(r"//.*", "", line).strip()
-            if not line:
-                continue
-            mm = re.fullmatch(r"(\w+)\s+(\w+)(?:\[(\d+)\])?", " ".join(line.split()))
-            cf.append((mm.group(2), (mm.group(1), int(mm.group(3) or 1))))
-        assert jf == cf, (jl, jf, cf)
+        assert jf == header_struct(c), (jl, jf, header_struct(c))
+
+
+def header_struct(c):
+    """[(field, (C type, count))] of `typedef struct c {...} c;` in the headers."""
+    hdr = re.sub(r"/\*.*?\*/", " ", "\n".join(h.read_text() for h in (ROOT / "include").glob("*.h")),
+                 flags=re.S)
+    m = re.search(r"typedef struct %s \{(.*?)\} %s;" % (c, c), hdr, re.S)
+    assert m, c
+    cf = []
+    for line in m.group(1).split(";"):
+        line = re.sub(r"//.*", "", line).strip()
+        if not line:
+            continue
+        mm = re.fullmatch(r"(\w+)\s+(\w+)(?:\[(\d+)\])?", " ".join(line.split()))
+        cf.append((mm.group(2), (mm.group(1), int(mm.group(3) or 1))))
+    return cf
+
+
+def test_ctypes_structs_match_header(pkg):
+    """The package's ctypes mirrors of the public structs have the header's
+    field names, types and array lengths, in order."""
+    import importlib
+    native = importlib.import_module(pkg.__name__ + "._native")
+    ensemble = importlib.import_module(pkg.__name__ + ".ensemble")
+    ctype = {C.c_int32: "int32_t", C.c_int64: "int64_t", C.c_char: "char", C.c_double: "double"}
+    for cls, c in ((native.Tuning, "nb_tuning"), (native.DevInfo, "nb_devinfo"),
+                   (ensemble.BatchTuning, "nb_batch_tuning")):
+        pf = []
+        for name, t in cls._fields_:
+            if hasattr(t, "_length_"):
+                pf.append((name, (ctype[t._type_], t._length_)))
+            else:
+                pf.append((name, (ctype[t], 1)))
+        assert pf == header_struct(c), (c, pf, header_struct(c))
 
 
 def test_noalloc_rejects_sizes_beyond_its_scratch(pkg):

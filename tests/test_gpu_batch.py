@@ -301,6 +301,165 @@ def test_resident_and_ragged_paths_agree(pkg, planets):
     assert np.max(np.abs(outs[0]["pos"] - outs[1]["pos"])) < 1e-10
 
 
+def _mixed_planetary(pkg):
+    """Planetary systems of different shapes in one ensemble: the C5 system
+    (star, 8 planets, 64 test particles), one of 201 bodies whose massive and
+    massless bodies are interleaved (indices 1..200 shuffled), one of 609
+    bodies (more than 256: its test particles go to three or four waves), a
+    star with three planets and no test particles, a star with test particles
+    only, a system of massless particles only, and a lone star."""
+    shapes = [(8, 64, False), (20, 180, True), (8, 600, True), (3, 0, False), (0, 40, False),
+              (0, 10, False), (0, 0, False)]
+    rng = np.random.default_rng(31)
+    parts = []
+    for k, (npl, nt, shuffle) in enumerate(shapes):
+        e = pkg.ics.planetary_ensemble(nsys=1, seed=40 + k, nplanets=npl, ntest=nt)
+        pos, vel, mass = e["pos"], e["vel"], e["mass"]
+        if shuffle:
+            idx = np.concatenate([[0], 1 + rng.permutation(len(mass) - 1)])
+            pos, vel, mass = pos[idx], vel[idx], mass[idx]
+        if (npl, nt) == (0, 10):
+            mass = np.zeros_like(mass)
+        parts.append((pos, vel, mass))
+    off = np.concatenate([[0], np.cumsum([len(p[2]) for p in parts])]).astype(np.int64)
+    return {"offsets": off, "pos": np.concatenate([p[0] for p in parts]),
+            "vel": np.concatenate([p[1] for p in parts]), "mass": np.concatenate([p[2] for p in parts])}
+
+
+@pytest.mark.parametrize("waves", [0, 1, 2, 3, 4])
+def test_resident_wave_layouts(pkg, orc, waves):
+    """The resident kernel with every wave count per system (0 = its own
+    choice; each wave holds the massive bodies and a share of the massless
+    ones; a count whose lanes do not fit falls back to the choice): 200 steps
+    against the oracle's KDK run loop, system by system, and bit-identical to
+    one wave per system, since each particle's arithmetic does not depend on
+    the layout."""
+    ens = _mixed_planetary(pkg)
+    steps, dt = 200, 1e-3
+
+    def run(w):
+        with make(pkg, ens, "split_central", dt=dt, resident=1, resident_waves=w) as e:
+            e.set_central(1)
+            e.compute_force()
+            e.half_kick(+1)
+            e.kdk_steps(steps, use_graph=True)
+            e.half_kick(-1)
+            return e.download()
+
+    d = run(waves)
+    off = ens["offsets"]
+    for s in range(len(off) - 1):
+        sl = slice(off[s], off[s + 1])
+        xo, vo, _, _ = orc.run_const_kdk(ens["pos"][sl], ens["vel"][sl], ens["mass"][sl], [0.0] * 6,
+                                         1.0, t_begin=0.0, t_end=steps * dt, dt=dt,
+                                         between_snapshots=1.0, skip_self=True)
+        assert np.max(np.abs(d["pos"][sl] - xo)) < 1e-9, s
+        assert np.max(np.abs(d["vel"][sl] - vo)) < 1e-8, s
+    if waves >= 2:   # (0: the packed layout, whose hi/lo origin differs)
+        d1 = run(1)
+        for k in ("pos", "vel", "acc"):
+            assert np.array_equal(d[k], d1[k]), k
+
+
+@pytest.mark.parametrize("mode", ["split", "split_central"])
+def test_resident_heavy_binary(pkg, orc, mode):
+    """The StaticTest binary (masses 1e10, test/StaticTest.jl:18-123) in the
+    resident kernel without softening, beside a planetary system: the r^2
+    offset that keeps the self pair finite scales with the largest mass, and
+    both systems follow the oracle's run loop for 100 steps."""
+    bp, bv, bm = pkg.ics.binary()
+    pl = pkg.ics.planetary_ensemble(nsys=1, seed=9)
+    ens = {"offsets": np.array([0, 2, 2 + len(pl["mass"])], dtype=np.int64),
+           "pos": np.concatenate([bp, pl["pos"]]), "vel": np.concatenate([bv, pl["vel"]]),
+           "mass": np.concatenate([bm, pl["mass"]])}
+    steps, dt = 100, 2.0 ** -20   # exact in binary: the oracle's t += dt loop runs 100 steps
+    with make(pkg, ens, mode, dt=dt, resident=1) as e:
+        if mode == "split_central":
+            e.set_central(1)
+        e.compute_force()
+        e.half_kick(+1)
+        e.kdk_steps(steps, use_graph=True)
+        e.half_kick(-1)
+        d = e.download()
+    assert all(np.isfinite(v).all() for v in d.values())
+    off = ens["offsets"]
+    # the binary's pair is FP32 (hi/lo) at least one way: a relative force
+    # error of a few 1e-8 over 0.8 orbits; the planetary system as before
+    for s, tol in ((0, 1e-5), (1, 1e-9)):
+        sl = slice(off[s], off[s + 1])
+        xo, vo, _, info = orc.run_const_kdk(ens["pos"][sl], ens["vel"][sl], ens["mass"][sl], [0.0] * 6,
+                                            1.0, t_begin=0.0, t_end=steps * dt, dt=dt,
+                                            between_snapshots=1.0, skip_self=True)
+        assert info["steps"] == steps
+        assert np.max(np.abs(d["pos"][sl] - xo)) <= tol * np.max(np.abs(xo)), s
+        assert np.max(np.abs(d["vel"][sl] - vo)) <= tol * np.max(np.abs(vo)), s
+
+
+def test_tracked_energy_wave_layouts(pkg):
+    """Tracked energies of the mixed planetary shapes agree between the
+    packed layout (0) and one or two waves per system; the system of
+    massless particles has zero energy in every layout."""
+    ens = _mixed_planetary(pkg)
+    steps, every, dt = 300, 100, 1e-3
+    Es = []
+    for w in (0, 1, 2):
+        with make(pkg, ens, "split_central", dt=dt, resident=1, resident_waves=w) as e:
+            e.set_central(1)
+            e.compute_force()
+            e.half_kick(+1)
+            Es.append(e.kdk_steps_tracked(steps, every))
+    nsys = len(ens["offsets"]) - 1
+    zero = [s for s in range(nsys) if not ens["mass"][ens["offsets"][s]:ens["offsets"][s + 1]].any()]
+    assert zero and Es[0].shape == (nsys, 3)
+    for E in Es:
+        assert np.all(E[zero] == 0.0)
+    # The layouts split positions into hi/lo about different origins (the
+    # packed one keeps the launch's first position), so their FP32
+    # planet-planet terms round differently: 4e-12 apart after 300 steps.
+    # (A lone star at rest has exactly zero energy too: zero must stay zero.)
+    for E in Es[1:]:
+        assert np.all(np.abs(E - Es[0]) <= 1e-10 * np.abs(Es[0]))
+
+
+def test_packed_resident_mixed_shapes(pkg, orc):
+    """1201 planetary systems of four shapes (3 or 8 planets, 0 to 100 test
+    particles) in the packed layout: two systems per workgroup, sorted by
+    massive count, so one workgroup pads a 4-body system's records to a
+    9-body one's.  100 steps of 20 sampled systems (both systems of that
+    workgroup among them) against the oracle's run loop, and tracked
+    energies against one wave per system."""
+    shapes = [(3, 0), (8, 64), (3, 100), (8, 20)]
+    parts = []
+    for k in range(1201):
+        npl, nt = shapes[k % 4]
+        e = pkg.ics.planetary_ensemble(nsys=1, seed=1000 + k, nplanets=npl, ntest=nt)
+        parts.append((e["pos"], e["vel"], e["mass"]))
+    off = np.concatenate([[0], np.cumsum([len(p[2]) for p in parts])]).astype(np.int64)
+    ens = {"offsets": off, "pos": np.concatenate([p[0] for p in parts]),
+           "vel": np.concatenate([p[1] for p in parts]), "mass": np.concatenate([p[2] for p in parts])}
+    steps, every, dt = 100, 50, 1e-3
+    out = {}
+    for w in (0, 1):
+        with make(pkg, ens, "split_central", dt=dt, resident=1, resident_waves=w) as e:
+            e.set_central(1)
+            e.compute_force()
+            e.half_kick(+1)
+            E = e.kdk_steps_tracked(steps, every)
+            e.half_kick(-1)
+            out[w] = (e.download(), E)
+    d, E0 = out[0]
+    assert np.max(np.abs(out[1][1] - E0) / np.abs(E0)) < 1e-10   # (different hi/lo origins)
+    rng = np.random.default_rng(3)
+    sample = sorted(set(rng.choice(1201, 18, replace=False).tolist()) | {1200, 1})
+    for s in sample:
+        sl = slice(off[s], off[s + 1])
+        xo, vo, _, _ = orc.run_const_kdk(ens["pos"][sl], ens["vel"][sl], ens["mass"][sl], [0.0] * 6,
+                                         1.0, t_begin=0.0, t_end=steps * dt, dt=dt,
+                                         between_snapshots=1.0, skip_self=True)
+        assert np.max(np.abs(d["pos"][sl] - xo)) < 1e-9, s
+        assert np.max(np.abs(d["vel"][sl] - vo)) < 1e-8, s
+
+
 def test_tracked_energy(pkg, orc, planets):
     steps, every, dt = 400, 100, 1e-3
     with make(pkg, planets, "split_central", dt=dt, resident=1) as e:
