@@ -423,11 +423,12 @@ def test_tracked_energy_wave_layouts(pkg):
 
 def test_packed_resident_mixed_shapes(pkg, orc):
     """1201 planetary systems of four shapes (3 or 8 planets, 0 to 100 test
-    particles) in the packed layout: two systems per workgroup, sorted by
-    massive count, so one workgroup pads a 4-body system's records to a
-    9-body one's.  100 steps of 20 sampled systems (both systems of that
-    workgroup among them) against the oracle's run loop, and tracked
-    energies against one wave per system."""
+    particles) in the packed layout: several systems per workgroup (5 on 256
+    CUs), sorted by massive count, so the workgroup that holds the last
+    4-body system (1200) and the first 9-body one (1) pads the former's
+    records to the latter's.  100 steps of 20 sampled systems (those two
+    among them) against the oracle's run loop, and tracked energies against
+    one wave per system."""
     shapes = [(3, 0), (8, 64), (3, 100), (8, 20)]
     parts = []
     for k in range(1201):
