@@ -26,6 +26,12 @@ struct nb_batch {
     double *pk_tiny = nullptr;    // [blocks][2] r^2 offsets (FP32, FP64)
     int pk_nblk = 0, pk_threads = 0, pk_smax = 0, pk_recs = 0;
     bool pk_stale = true;
+    // pinned host staging of the tracked energies (a pageable device-to-host
+    // copy costs about 12 ms the first time in a process)
+    double *h_pinned = nullptr;
+    size_t h_pinned_n = 0;
+    double *e_dev = nullptr;      // device buffer of the tracked energies
+    size_t e_dev_n = 0;
     std::vector<double> hG, hh, hdt;
     int64_t *off = nullptr;       // [nsys+1]
     int32_t *sys_of = nullptr;    // system of each particle

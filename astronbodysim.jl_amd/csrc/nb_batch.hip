@@ -1162,6 +1162,32 @@ static int bforce(nb_batch *b, bool with_pot, bool allow_timing, bool pot_only =
     return NB_OK;
 }
 
+// Pinned host staging of at least n doubles (grown on demand).
+static int ensure_pinned(nb_batch *b, size_t n) {
+    if (b->h_pinned_n >= n) return NB_OK;
+    if (b->h_pinned) NB_HIP(hipHostFree(b->h_pinned));
+    b->h_pinned = nullptr;
+    b->h_pinned_n = 0;
+    hipError_t e = hipHostMalloc((void **)&b->h_pinned, sizeof(double) * n, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        b->h_pinned = nullptr;
+        set_error("hipHostMalloc(%zu bytes) failed: %s", sizeof(double) * n, hipGetErrorString(e));
+        return NB_ERR_HIP;
+    }
+    b->h_pinned_n = n;
+    return NB_OK;
+}
+
+// Device buffer of at least n tracked energies (grown on demand).
+static int ensure_edev(nb_batch *b, size_t n) {
+    if (b->e_dev_n >= n) return NB_OK;
+    dfree(b->e_dev);
+    b->e_dev_n = 0;
+    NB_TRY(dalloc(&b->e_dev, sizeof(double) * n));
+    b->e_dev_n = n;
+    return NB_OK;
+}
+
 static int bkick_drift(nb_batch *b, double kick_coef, bool drift) {
     if (b->n == 0) return NB_OK;
     k_batch_kick_drift<<<blocks_for(b->n), 256, 0, b->stream>>>(
@@ -1651,6 +1677,8 @@ int nb_batch_destroy(nb_batch *b) {
     dfree(b->items); dfree(b->counter); dfree(b->stage); dfree(b->red); dfree(b->wsrc);
     dfree(b->sort_tmp); dfree(b->keys); dfree(b->idx); dfree(b->sysbox); dfree(b->spare);
     dfree(b->pk_lane); dfree(b->pk_meta); dfree(b->pk_blk); dfree(b->pk_tiny);
+    if (b->h_pinned) (void)hipHostFree(b->h_pinned);
+    dfree(b->e_dev);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return NB_OK;
@@ -1726,6 +1754,8 @@ int nb_batch_upload(nb_batch *b, const double *pos, const double *vel, const dou
             b->h_mmax[(size_t)q] = std::max(b->h_mmax[(size_t)q], std::fabs(mass[i]));
         }
     b->pk_stale = true;
+    NB_TRY(ensure_pinned(b, (size_t)std::max(b->nsys, 1) * 128));
+    NB_TRY(ensure_edev(b, (size_t)std::max(b->nsys, 1) * 128));
     b->uploaded = true;
     NB_TRY(nb_batch_set_params(b, b->hG.data(), b->hh.data(), b->hdt.data()));
     NB_TRY(rebuild_central(b));
@@ -1830,8 +1860,8 @@ int nb_batch_kdk_steps_tracked(nb_batch *b, int64_t nsteps, int32_t every, doubl
     const size_t ns = (size_t)b->nsys;
     if (slots == 0) return batch_steps(b, nsteps, true);
     if (resident_ok(b)) NB_TRY(batch_unsort(b));
-    double *dev = nullptr;
-    NB_TRY(dalloc(&dev, sizeof(double) * ns * (size_t)slots));
+    NB_TRY(ensure_edev(b, ns * (size_t)slots));
+    double *dev = b->e_dev;
     int rc = NB_OK;
     if (resident_ok(b)) {
         rc = resident_steps(b, nsteps, every, dev, (int)slots);
@@ -1860,21 +1890,22 @@ int nb_batch_kdk_steps_tracked(nb_batch *b, int64_t nsteps, int32_t every, doubl
                     energy[s * (size_t)slots + (size_t)k] = host[s * 8] + 0.5 * host[s * 8 + 1];
         }
         if (rc == NB_OK && nsteps % every) rc = batch_steps(b, nsteps % every, true);
-        dfree(dev);
         return rc;
     }
+    if (rc == NB_OK) rc = ensure_pinned(b, ns * (size_t)slots);
     if (rc == NB_OK) {
-        hipError_t e = hipMemcpyAsync(energy, dev, sizeof(double) * ns * (size_t)slots,
+        hipError_t e = hipMemcpyAsync(b->h_pinned, dev, sizeof(double) * ns * (size_t)slots,
                                       hipMemcpyDeviceToHost, b->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
         if (e != hipSuccess) {
             set_error("energy readback: %s", hipGetErrorString(e));
             rc = NB_ERR_HIP;
+        } else {
+            std::memcpy(energy, b->h_pinned, sizeof(double) * ns * (size_t)slots);
         }
     }
     if (rc == NB_OK && nsteps % every) rc = resident_steps(b, nsteps % every, 0, nullptr, 0);
     (void)hipStreamSynchronize(b->stream);
-    dfree(dev);
     return rc;
 }
 

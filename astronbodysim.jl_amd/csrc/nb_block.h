@@ -145,6 +145,30 @@ __device__ __forceinline__ void tile_f32_hilo(const float4 *__restrict__ s,
     }
 }
 
+// A source for the near tiles of FP64-state blocks on far path 0: its FP64
+// position relative to the block origin and its mass.
+struct SrcF64 {
+    double x, y, z;
+    float m, pad;
+};
+
+template <int TPT, bool CHECKED, bool POT>
+__device__ __forceinline__ void tile_f32_d(const SrcF64 *__restrict__ s, const double (&xi)[TPT],
+                                           const double (&yi)[TPT], const double (&zi)[TPT],
+                                           const float (&h2c)[TPT], const float (&h2)[TPT],
+                                           const float (&hinv)[TPT], const float (&hinv3)[TPT],
+                                           float (&ax)[TPT], float (&ay)[TPT], float (&az)[TPT],
+                                           float (&aq)[TPT]) {
+#pragma unroll 2
+    for (int jj = 0; jj < BS; ++jj) {
+        const SrcF64 v = s[jj];
+#pragma unroll
+        for (int k = 0; k < TPT; ++k)
+            pair_f32_d<CHECKED, POT>(v.x, v.y, v.z, v.m, xi[k], yi[k], zi[k], h2c[k], h2[k], hinv[k],
+                                     hinv3[k], ax[k], ay[k], az[k], aq[k]);
+    }
+}
+
 // Tile classification (wave-uniform), from the Euclidean distance d between
 // a source tile's box and the box of the wave's targets:
 //   d <= h_max (+ a resolution margin)      -> "soft": softening-checked pairs
@@ -216,12 +240,18 @@ constexpr float FARW_MASSLESS = 1.329228e36f;   // 2^120
 // FP32 positions differ by an FP32 subtraction with a relative error of at
 // most 2^-24, as small as the hi/lo difference's, for 12 VALU per pair
 // instead of 18.  sjl then holds (x, y, z, m) of each source.
+// D64NEAR (FP64 state on far path 0): the near tiles take the sources' FP64
+// positions relative to the origin (sjd) and form each separation with one
+// FP64 subtraction and a conversion per axis (pair_f32_d): 15 VALU per pair
+// instead of the hi/lo loop's 18, at the same accuracy; sjl is then unused.
 template <typename T, int MF = 0>
 struct BlockShared {
     static constexpr bool HILO = sizeof(T) == 8 || MF == FAR_MFMA_BF16;
     static constexpr bool ABSNEAR = sizeof(T) == 4 && MF == FAR_MFMA_BF16;
+    static constexpr bool D64NEAR = sizeof(T) == 8 && MF == FAR_VALU;
     float4 sj[2][BS];
-    float4 sjl[HILO ? 2 : 1][HILO ? BS : 1];
+    float4 sjl[HILO && !D64NEAR ? 2 : 1][HILO && !D64NEAR ? BS : 1];
+    SrcF64 sjd[D64NEAR ? 2 : 1][D64NEAR ? BS : 1];
     static constexpr bool COLS = MF == 1 || MF == 2;   // the MFMA far paths
     float sB[COLS ? 2 : 1][COLS ? 4 : 1][COLS ? BS : 1];   // MFMA B columns, component-major
     float sQ[MF == 2 ? 2 : 1][MF == 2 ? BS : 1];     // |x'_j|^2 of each source
@@ -665,6 +695,7 @@ __device__ __forceinline__ void force_block(
     double (&Az)[TPT], double (&Aq)[TPT], bool (&valid)[TPT]) {
     constexpr bool HILO = BlockShared<T, MF>::HILO;
     constexpr bool ABSNEAR = BlockShared<T, MF>::ABSNEAR;
+    constexpr bool D64NEAR = BlockShared<T, MF>::D64NEAR;
     static_assert(!(MF == FAR_MFMA_SUM || MF == FAR_MFMA_R2) || (HILO && TPT == 1 && !POT),
                   "MFMA far tiles: SPLIT, 1 target/lane, acc only");
     static_assert(MF != FAR_WEIGHTED || !POT, "weighted far tiles: acc only");
@@ -692,6 +723,7 @@ __device__ __forceinline__ void force_block(
 
     float xi[TPT], yi[TPT], zi[TPT], xil[TPT], yil[TPT], zil[TPT];
     float xa[TPT], ya[TPT], za[TPT];   // ABSNEAR: the FP32 positions themselves
+    double xd[TPT], yd[TPT], zd[TPT];  // D64NEAR: FP64 positions relative to the origin
     float h2c[TPT], h2[TPT], hinv[TPT], hinv3[TPT];
     float ax[TPT], ay[TPT], az[TPT], aq[TPT];
 #pragma unroll
@@ -706,6 +738,9 @@ __device__ __forceinline__ void force_block(
         xa[k] = (float)px[k];
         ya[k] = (float)py[k];
         za[k] = (float)pz[k];
+        xd[k] = dx;
+        yd[k] = dy;
+        zd[k] = dz;
         const float hf = (float)ph[k];
         h2[k] = hf * hf;
         h2c[k] = fmaxf(h2[k], FLT_MIN);
@@ -794,6 +829,8 @@ __device__ __forceinline__ void force_block(
             sh.sj[buf][tid] = make_float4(hx, hy, hz, (ABSNEAR && !ACC) ? cw : (float)rm);
             if (ABSNEAR)
                 sh.sjl[HILO ? buf : 0][HILO ? tid : 0] = make_float4((float)rx, (float)ry, (float)rz, (float)rm);
+            else if (D64NEAR)
+                sh.sjd[D64NEAR ? buf : 0][D64NEAR ? tid : 0] = SrcF64{dx, dy, dz, (float)rm, 0.0f};
             else if (HILO)
                 sh.sjl[HILO ? buf : 0][HILO ? tid : 0] =
                     make_float4((float)(dx - (double)hx), (float)(dy - (double)hy),
@@ -834,6 +871,9 @@ __device__ __forceinline__ void force_block(
             sh.sj[buf][tid] = make_float4(PAD_COORD, PAD_COORD, PAD_COORD, 0.0f);
             if (ABSNEAR)
                 sh.sjl[HILO ? buf : 0][HILO ? tid : 0] = make_float4(PAD_COORD, PAD_COORD, PAD_COORD, 0.0f);
+            else if (D64NEAR)
+                sh.sjd[D64NEAR ? buf : 0][D64NEAR ? tid : 0] =
+                    SrcF64{(double)PAD_COORD, (double)PAD_COORD, (double)PAD_COORD, 0.0f, 0.0f};
             else if (HILO)
                 sh.sjl[HILO ? buf : 0][HILO ? tid : 0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if constexpr (MF == 1 || MF == 2) {
@@ -879,12 +919,18 @@ __device__ __forceinline__ void force_block(
             if (ABSNEAR)
                 tile_f32<TPT, true, POT>(sh.sjl[HILO ? buf : 0], xa, ya, za, h2c, h2, hinv, hinv3, ax, ay, az,
                                          aq);
+            else if (D64NEAR)
+                tile_f32_d<TPT, true, POT>(sh.sjd[D64NEAR ? buf : 0], xd, yd, zd, h2c, h2, hinv, hinv3, ax, ay,
+                                           az, aq);
             else if (HILO)
                 tile_f32_hilo<TPT, true, POT>(sh.sj[buf], sh.sjl[HILO ? buf : 0], xi, yi, zi, xil,
                                               yil, zil, h2c, h2, hinv, hinv3, ax, ay, az, aq);
             else
                 tile_f32<TPT, true, POT>(sh.sj[buf], xi, yi, zi, h2c, h2, hinv, hinv3, ax, ay, az,
                                          aq);
+        } else if (D64NEAR && (m_close & tbit)) {
+            tile_f32_d<TPT, false, POT>(sh.sjd[D64NEAR ? buf : 0], xd, yd, zd, h2c, h2, hinv, hinv3, ax, ay, az,
+                                        aq);
         } else if (HILO && !ABSNEAR && (m_close & tbit)) {
             tile_f32_hilo<TPT, false, POT>(sh.sj[buf], sh.sjl[HILO ? buf : 0], xi, yi, zi, xil,
                                            yil, zil, h2c, h2, hinv, hinv3, ax, ay, az, aq);

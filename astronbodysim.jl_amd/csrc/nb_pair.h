@@ -112,6 +112,31 @@ __device__ __forceinline__ void pair_f32_hilo(float sx, float sy, float sz, floa
     if (POT) aq += q;
 }
 
+// One FP32 pair whose separation comes from FP64 coordinates (both relative
+// to the block origin): dx = fl32(xj - xi) carries a relative error of at most
+// 2^-24 (plus 2^-53 (|xi| + |xj|) / |dx| from the FP64 subtraction), as the
+// hi/lo difference does, with one FP64 subtraction and one conversion per axis
+// instead of three FP32 operations.
+template <bool CHECKED, bool POT>
+__device__ __forceinline__ void pair_f32_d(double sx, double sy, double sz, float sm, double xi,
+                                           double yi, double zi, float h2c, float h2, float hinv,
+                                           float hinv3, float &ax, float &ay, float &az, float &aq) {
+    const float dx = (float)(sx - xi);
+    const float dy = (float)(sy - yi);
+    const float dz = (float)(sz - zi);
+    float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+    float ri = __builtin_amdgcn_rsqf(r2);
+    float q = sm * ri;
+    float fac = q * (ri * ri);
+    if (CHECKED) {
+        if (__builtin_expect(r2 < h2c, 0)) spline_f32(r2, sm, h2, hinv, hinv3, fac, q);
+    }
+    ax = __builtin_fmaf(fac, dx, ax);
+    ay = __builtin_fmaf(fac, dy, ay);
+    az = __builtin_fmaf(fac, dz, az);
+    if (POT) aq += q;
+}
+
 // ---------------------------------------------------------------------------
 // FP64 pair (correctness mode): the reference's formula in double.
 // ---------------------------------------------------------------------------
