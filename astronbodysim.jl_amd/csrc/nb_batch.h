@@ -23,7 +23,7 @@ struct nb_batch {
     int32_t *pk_lane = nullptr;   // [blocks][threads] particle, -1 for an idle lane
     int32_t *pk_meta = nullptr;   // [blocks][threads] (system in block << 16) | first flag | record slot
     int32_t *pk_blk = nullptr;    // [blocks][stride] block table
-    double *pk_tiny = nullptr;    // [blocks][2] r^2 offsets (FP32, FP64)
+    double *pk_tiny = nullptr;    // [blocks] largest |mass| (sets the r^2 offsets)
     int pk_nblk = 0, pk_threads = 0, pk_smax = 0, pk_recs = 0;
     bool pk_stale = true;
     // pinned host staging of the tracked energies (a pageable device-to-host

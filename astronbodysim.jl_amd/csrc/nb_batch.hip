@@ -10,12 +10,12 @@
 //    is the same block routine as the one-system kernel (force_block), with
 //    the system's own source range, per-system Morton order, FP64 origin and
 //    hi/lo near tiles.  KDK steps are captured in HIP graphs.
-//  * resident path (systems whose massive particles plus a share of their
-//    massless ones fit 256 lanes of one to four wavefronts): one block per
-//    system keeps its particles in registers and runs many KDK steps in a
-//    single launch, the sources broadcast through LDS; central bodies use FP64
-//    pair terms, the other massive bodies FP32 hi/lo terms relative to the
-//    system's first massive particle.
+//  * resident path (small systems): the particles stay in registers and many
+//    KDK steps run in a single launch, the sources broadcast through LDS;
+//    central bodies use FP64 pair terms, the other massive bodies FP32 pair
+//    terms on separations taken from FP64 positions.  Several systems share a
+//    workgroup (k_batch_resident_packed), or one system takes one to four
+//    wavefronts (k_batch_resident).
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -341,24 +341,22 @@ __device__ __forceinline__ double rsq_f64(double x) {
     return y;
 }
 
-// FP32 hi/lo pair of the resident kernel: source (sh, sl, sm), target k of
-// the lane.  SOFT: the spline inside h.  Otherwise Newtonian, with r^2 + tiny
-// in place of r^2 (tiny from resident_tiny): the self pair (r = 0) then has a
-// finite factor times dx = 0, so it contributes nothing without a select, and
-// every pair with r^2 above about 2^24 tiny keeps its FP32 r^2 unchanged.
+// FP32 pair of the resident kernel from FP64 positions: source (sx, sy, sz,
+// sm), target (x, y, z); the separation is fl32 of the FP64 difference
+// (pair_f32_d).  SOFT: the spline inside h.  Otherwise Newtonian, with
+// r^2 + tiny in place of r^2 (tiny from resident_tiny): the self pair (r = 0)
+// then has a finite factor times dx = 0, so it contributes nothing without a
+// select, and every pair with r^2 above about 2^24 tiny keeps its FP32 r^2.
 template <bool SOFT>
-__device__ __forceinline__ void res_pair_f32(float sxh, float syh, float szh, float sxl, float syl,
-                                             float szl, float smf, float xh, float yh, float zh,
-                                             float xl, float yl, float zl, float h2cf, float h2f,
-                                             float hinvf, float hinv3f, float tiny, float &fx, float &fy,
-                                             float &fz, float &fq) {
+__device__ __forceinline__ void res_pair_f32(double sx, double sy, double sz, float smf, double x, double y,
+                                             double z, float h2cf, float h2f, float hinvf, float hinv3f,
+                                             float tiny, float &fx, float &fy, float &fz, float &fq) {
     if constexpr (SOFT) {
-        pair_f32_hilo<true, false>(sxh, syh, szh, smf, sxl, syl, szl, xh, yh, zh, xl, yl, zl, h2cf,
-                                   h2f, hinvf, hinv3f, fx, fy, fz, fq);
+        pair_f32_d<true, false>(sx, sy, sz, smf, x, y, z, h2cf, h2f, hinvf, hinv3f, fx, fy, fz, fq);
     } else {
-        const float dx = (sxh - xh) + (sxl - xl);
-        const float dy = (syh - yh) + (syl - yl);
-        const float dz = (szh - zh) + (szl - zl);
+        const float dx = (float)(sx - x);
+        const float dy = (float)(sy - y);
+        const float dz = (float)(sz - z);
         const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, tiny)));
         const float ri = __builtin_amdgcn_rsqf(r2);
         const float fac = smf * (ri * ri) * ri;
@@ -393,9 +391,8 @@ struct ResState {
     double tiny64;
 };
 
-// One source of a resident wave in LDS, 32 B: a central body as FP64
-// (x, y, z, m), any other as FP32 (x_hi, y_hi, z_hi, m) and (x_lo, y_lo, z_lo, 0)
-// relative to the wave's origin.
+// One source of a resident wave in LDS, 32 B: FP64 (x, y, z, m) for a central
+// body; FP64 (x, y, z) and the FP32 mass for any other (src_f32).
 struct ResSrc {
     float4 a, b;
 };
@@ -411,6 +408,17 @@ __device__ __forceinline__ void f64x2_of(float4 f, double &u, double &v) {
                                        __builtin_bit_cast(unsigned, f.x));
     v = __builtin_bit_cast(double, ((unsigned long long)__builtin_bit_cast(unsigned, f.w) << 32) |
                                        __builtin_bit_cast(unsigned, f.z));
+}
+// The record of a source with FP32 pair terms, and its reading.
+__device__ __forceinline__ ResSrc src_f32(double x, double y, double z, double m) {
+    const float4 zb = f64x2_bits(z, 0.0);
+    return ResSrc{f64x2_bits(x, y), make_float4(zb.x, zb.y, (float)m, 0.0f)};
+}
+__device__ __forceinline__ void src_f32_of(const ResSrc &r, double &x, double &y, double &z, float &m) {
+    double unused;
+    f64x2_of(r.a, x, y);
+    f64x2_of(make_float4(r.b.x, r.b.y, 0.0f, 0.0f), z, unused);
+    m = r.b.z;
 }
 
 // nsteps KDK steps of one wave's particles; SOFT (h > 0) is wave-uniform and
@@ -437,19 +445,10 @@ __device__ __forceinline__ void resident_run(const ResArgs &a, ResState<NSLOT> &
         for (int k = 0; k < NSLOT; ++k) {
             r.X[k] += r.VX[k] * dt; r.Y[k] += r.VY[k] * dt; r.Z[k] += r.VZ[k] * dt;
         }
-        // origin: the system's first massive particle (the central body of a
-        // planetary system)
-        const double ox = readlane_f64(r.X[0], 0), oy = readlane_f64(r.Y[0], 0), oz = readlane_f64(r.Z[0], 0);
-        float xh[NSLOT], yh[NSLOT], zh[NSLOT], xl[NSLOT], yl[NSLOT], zl[NSLOT];
         float fx[NSLOT], fy[NSLOT], fz[NSLOT], fq[NSLOT];
         double Ax[NSLOT], Ay[NSLOT], Az[NSLOT], Aq[NSLOT];
 #pragma unroll
         for (int k = 0; k < NSLOT; ++k) {
-            const double dx = r.X[k] - ox, dy = r.Y[k] - oy, dz = r.Z[k] - oz;
-            xh[k] = (float)dx; yh[k] = (float)dy; zh[k] = (float)dz;
-            xl[k] = (float)(dx - (double)xh[k]);
-            yl[k] = (float)(dy - (double)yh[k]);
-            zl[k] = (float)(dz - (double)zh[k]);
             fx[k] = fy[k] = fz[k] = fq[k] = 0.0f;
             Ax[k] = Ay[k] = Az[k] = Aq[k] = 0.0;
             const int p = lane + 64 * k;
@@ -457,8 +456,7 @@ __device__ __forceinline__ void resident_run(const ResArgs &a, ResState<NSLOT> &
                 src[p].a = f64x2_bits(r.X[k], r.Y[k]);
                 src[p].b = f64x2_bits(r.Z[k], r.M[k]);
             } else if (p < nm) {
-                src[p].a = make_float4(xh[k], yh[k], zh[k], (float)r.M[k]);
-                src[p].b = make_float4(xl[k], yl[k], zl[k], 0.0f);
+                src[p] = src_f32(r.X[k], r.Y[k], r.Z[k], r.M[k]);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -481,18 +479,19 @@ __device__ __forceinline__ void resident_run(const ResArgs &a, ResState<NSLOT> &
                 }
             }
         }
-        // the other massive bodies: FP32 hi/lo terms, two sources per trip
+        // the other massive bodies: FP32 terms, two sources per trip
         // (position nm holds a massless record for an odd count)
         for (int j = nc; j < nm; j += 2) {
-            const float4 ah = src[j].a, al = src[j].b, bh = src[j + 1].a, bl = src[j + 1].b;
+            double ax_, ay_, az_, bx_, by_, bz_;
+            float am, bm;
+            src_f32_of(src[j], ax_, ay_, az_, am);
+            src_f32_of(src[j + 1], bx_, by_, bz_, bm);
 #pragma unroll
             for (int kt = 0; kt < NSLOT; ++kt) {
-                res_pair_f32<SOFT>(ah.x, ah.y, ah.z, al.x, al.y, al.z, ah.w, xh[kt], yh[kt], zh[kt], xl[kt],
-                                   yl[kt], zl[kt], h2cf, h2f, hinvf, hinv3f, tiny32, fx[kt], fy[kt], fz[kt],
-                                   fq[kt]);
-                res_pair_f32<SOFT>(bh.x, bh.y, bh.z, bl.x, bl.y, bl.z, bh.w, xh[kt], yh[kt], zh[kt], xl[kt],
-                                   yl[kt], zl[kt], h2cf, h2f, hinvf, hinv3f, tiny32, fx[kt], fy[kt], fz[kt],
-                                   fq[kt]);
+                res_pair_f32<SOFT>(ax_, ay_, az_, am, r.X[kt], r.Y[kt], r.Z[kt], h2cf, h2f, hinvf, hinv3f, tiny32,
+                                   fx[kt], fy[kt], fz[kt], fq[kt]);
+                res_pair_f32<SOFT>(bx_, by_, bz_, bm, r.X[kt], r.Y[kt], r.Z[kt], h2cf, h2f, hinvf, hinv3f, tiny32,
+                                   fx[kt], fy[kt], fz[kt], fq[kt]);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -552,11 +551,10 @@ __device__ __forceinline__ void resident_run(const ResArgs &a, ResState<NSLOT> &
 // counted in the caller's order.  Massless particles are no sources, so each
 // wave integrates its own copy of the massive ones (identical instructions on
 // identical data, hence identical values) and the waves never exchange data.
-// Each massive particle is a source: its coordinates are broadcast from the
-// lane that holds it with v_readlane.  Pair terms of the central bodies (the
-// first ncentral particles; ALL_F64: of every source) are FP64; the others are
-// FP32 hi/lo terms relative to the first massive particle.  Drift and kick
-// are FP64.  Every `energy_every` steps wave 0, which holds every massive
+// Each massive particle is a source, published to LDS each step.  Pair terms
+// of the central bodies (the first ncentral particles; ALL_F64: of every
+// source) are FP64; the others are FP32 on the fl32 of the FP64 separation.
+// Drift and kick are FP64.  Every `energy_every` steps wave 0, which holds every massive
 // particle, writes the system's total energy (velocities synchronised to the
 // step, FP64 potential over the massive sources; massless particles add
 // nothing) to energy[s * energy_slots + k].
@@ -627,8 +625,8 @@ __global__ __launch_bounds__(256) void k_batch_resident(ResArgs a, int nsteps) {
 #pragma unroll
     for (int k = 0; k < NSLOT; ++k) m_max = fmax(m_max, r.act[k] ? fabs(r.M[k]) : 0.0);
     resident_tiny(wave_max(m_max), r.tiny32, r.tiny64);
-    // 64 NSLOT + 1 records per wave: position nm is a massless record at the
-    // origin (it pads an odd count of FP32 sources)
+    // 64 NSLOT + 1 records per wave: position nm is a massless zero record
+    // (it pads an odd count of FP32 sources)
     extern __shared__ ResSrc res_src[];
     ResSrc *src = res_src + wave * (64 * NSLOT + 1);
     if (lane == 0) {
@@ -655,23 +653,20 @@ __global__ __launch_bounds__(256) void k_batch_resident(ResArgs a, int nsteps) {
 // particles fill the first wavefronts (each system's within one wave), their
 // massless particles the waves after, so a planetary system's 64 test
 // particles take one full wave and the massive waves serve several systems.
-// Each step every massive lane publishes its 32-byte source record (ResSrc:
-// FP64 for a central body, FP32 hi/lo relative to its system's origin
-// otherwise) into one of two LDS buffers; one workgroup barrier later every
-// lane reads its own system's records.  The buffer alternates with the step,
+// Each step every massive lane publishes its 32-byte source record (ResSrc)
+// into one of two LDS buffers; one workgroup barrier later every lane reads
+// its own system's records.  The buffer alternates with the step,
 // so a step's writes never meet the previous step's reads (every lane has
 // passed this step's barrier only after those reads) and one barrier per step
 // suffices.  A system's records: slots 0 .. NCB - 1 for its central bodies,
 // NCB .. NMB - 1 for the others, massless zero records where it has fewer
 // (NCB, NMB: the block's largest counts), and one more zero record that pads
-// an odd FP32 count.  The origin of a system is the position its first
-// massive particle had when the launch began: the hi/lo split represents
-// x - origin exactly to about 2^-48 |x - origin| for any nearby origin, so it
-// need not follow the particle.
+// an odd FP32 count.  Each particle's arithmetic is the one-system resident
+// kernel's, so both layouts give identical positions.
 struct PackArgs {
     const int32_t *lane, *meta, *blk;
-    const double *tiny;
-    int smax;        // systems per block at most (block table stride 5 + 3 smax)
+    const double *mmax;   // largest |mass| of each block's systems
+    int smax;        // systems per block at most (block table stride 5 + 2 smax)
     int recs;        // records per system
     const double *Gs, *hs, *dts;
     double *x, *y, *z, *vx, *vy, *vz;
@@ -683,8 +678,7 @@ struct PackArgs {
 };
 
 // Block table: [0] NCB, [1] NMB, [2] systems in the block, [3] softened,
-// [4] unused, then smax system ids, smax origin particles (-1: none) and smax
-// massive counts.  meta: bits 16+ system in block, bit 15 first massive lane
+// [4] unused, then smax system ids and smax massive counts.  meta: bits 16+ system in block, bit 15 first massive lane
 // of its system, bits 0-14 record slot (0x7fff: massless).
 constexpr int PK_NONE = 0x7fff;
 
@@ -697,16 +691,14 @@ __device__ __forceinline__ void packed_run(const PackArgs &a, const int32_t *bi,
     const bool first = (meta >> 15) & 1;
     const bool act = p >= 0, massive = act && slot != PK_NONE;
     const int s = bi[5 + sl];
-    const int orig = bi[5 + a.smax + sl];
-    const int nm_s = bi[5 + 2 * a.smax + sl];
+    const int nm_s = bi[5 + a.smax + sl];
     const double G = a.Gs[s], dt = a.dts[s], hsys = a.hs[s];
     const double hinv = SOFT ? 1.0 / hsys : 0.0, hinv3 = hinv * hinv * hinv;
     const float hf = (float)hsys, h2f = hf * hf, h2cf = fmaxf(h2f, FLT_MIN);
     const float hinvf = SOFT ? 1.0f / hf : 0.0f, hinv3f = hinvf * hinvf * hinvf;
-    const float tiny32 = (float)a.tiny[2 * blockIdx.x];
-    const double tiny64 = a.tiny[2 * blockIdx.x + 1];
-    const double ox = orig >= 0 ? a.x[orig] : 0.0, oy = orig >= 0 ? a.y[orig] : 0.0,
-                 oz = orig >= 0 ? a.z[orig] : 0.0;
+    float tiny32;
+    double tiny64;
+    resident_tiny(a.mmax[blockIdx.x], tiny32, tiny64);
     double X = act ? a.x[p] : 0.0, Y = act ? a.y[p] : 0.0, Z = act ? a.z[p] : 0.0;
     double VX = act ? a.vx[p] : 0.0, VY = act ? a.vy[p] : 0.0, VZ = act ? a.vz[p] : 0.0;
     double AX = act ? a.ax[p] : 0.0, AY = act ? a.ay[p] : 0.0, AZ = act ? a.az[p] : 0.0;
@@ -717,16 +709,12 @@ __device__ __forceinline__ void packed_run(const PackArgs &a, const int32_t *bi,
     for (int step = 0; step < nsteps; ++step) {
         ResSrc *src = rec + (step & 1) * nrec;
         X += VX * dt; Y += VY * dt; Z += VZ * dt;
-        const double dx0 = X - ox, dy0 = Y - oy, dz0 = Z - oz;
-        const float xh = (float)dx0, yh = (float)dy0, zh = (float)dz0;
-        const float xl = (float)(dx0 - (double)xh), yl = (float)(dy0 - (double)yh), zl = (float)(dz0 - (double)zh);
         if (massive) {
             if (slot < NCB) {
                 src[rb + slot].a = f64x2_bits(X, Y);
                 src[rb + slot].b = f64x2_bits(Z, M);
             } else {
-                src[rb + slot].a = make_float4(xh, yh, zh, (float)M);
-                src[rb + slot].b = make_float4(xl, yl, zl, 0.0f);
+                src[rb + slot] = src_f32(X, Y, Z, M);
             }
         }
         __syncthreads();
@@ -747,11 +735,12 @@ __device__ __forceinline__ void packed_run(const PackArgs &a, const int32_t *bi,
         }
         float fx = 0.0f, fy = 0.0f, fz = 0.0f, fq = 0.0f;
         for (int j = NCB; j < NMB; j += 2) {
-            const float4 ah = src[rb + j].a, al = src[rb + j].b, bh = src[rb + j + 1].a, bl = src[rb + j + 1].b;
-            res_pair_f32<SOFT>(ah.x, ah.y, ah.z, al.x, al.y, al.z, ah.w, xh, yh, zh, xl, yl, zl, h2cf, h2f,
-                               hinvf, hinv3f, tiny32, fx, fy, fz, fq);
-            res_pair_f32<SOFT>(bh.x, bh.y, bh.z, bl.x, bl.y, bl.z, bh.w, xh, yh, zh, xl, yl, zl, h2cf, h2f,
-                               hinvf, hinv3f, tiny32, fx, fy, fz, fq);
+            double ax_, ay_, az_, bx_, by_, bz_;
+            float am, bm;
+            src_f32_of(src[rb + j], ax_, ay_, az_, am);
+            src_f32_of(src[rb + j + 1], bx_, by_, bz_, bm);
+            res_pair_f32<SOFT>(ax_, ay_, az_, am, X, Y, Z, h2cf, h2f, hinvf, hinv3f, tiny32, fx, fy, fz, fq);
+            res_pair_f32<SOFT>(bx_, by_, bz_, bm, X, Y, Z, h2cf, h2f, hinvf, hinv3f, tiny32, fx, fy, fz, fq);
         }
         AX = G * (Ax + (double)fx);
         AY = G * (Ay + (double)fy);
@@ -804,7 +793,7 @@ __device__ __forceinline__ void packed_run(const PackArgs &a, const int32_t *bi,
 // first.  Every lane, idle ones included, takes every barrier.
 __global__ __launch_bounds__(1024) void k_batch_resident_packed(PackArgs a, int nsteps) {
     extern __shared__ ResSrc pk_src[];
-    const int32_t *bi = a.blk + (int64_t)blockIdx.x * (5 + 3 * a.smax);
+    const int32_t *bi = a.blk + (int64_t)blockIdx.x * (5 + 2 * a.smax);
     const int nrec = bi[2] * a.recs;
     ResSrc *rec = pk_src, *erec = pk_src + 2 * nrec;
     double *es = reinterpret_cast<double *>(pk_src + 3 * nrec);
@@ -1343,10 +1332,10 @@ static int build_packed(nb_batch *b) {
             nfb_max = std::max(nfb_max, b->h_nm[(size_t)q] - nc[q]);
         }
     }
-    const int threads = 64 * maxw, stride = 5 + 3 * smax, recs = ncb_max + nfb_max + 1;
+    const int threads = 64 * maxw, stride = 5 + 2 * smax, recs = ncb_max + nfb_max + 1;
     const size_t nb = blks.size();
     std::vector<int32_t> lane(nb * threads, -1), meta(nb * threads, (int32_t)PK_NONE), blk(nb * stride, -1);
-    std::vector<double> tiny(2 * nb);
+    std::vector<double> mmax_b(nb);
     for (size_t kb = 0; kb < nb; ++kb) {
         const Blk &k = blks[kb];
         int ncb = 0, nfb = 0;
@@ -1358,10 +1347,7 @@ static int build_packed(nb_batch *b) {
         }
         int32_t *bi = &blk[kb * stride];
         bi[0] = ncb; bi[1] = ncb + nfb; bi[2] = (int32_t)k.sys.size(); bi[3] = soft(k.sys[0]); bi[4] = 0;
-        const double m = std::max(mmax, 1.0);
-        const double c32 = std::cbrt(4.0 * m / (double)FLT_MAX), c64 = std::cbrt(4.0 * m / DBL_MAX);
-        tiny[2 * kb] = (double)(float)(c32 * c32);
-        tiny[2 * kb + 1] = c64 * c64;
+        mmax_b[kb] = mmax;
         // massive waves first (each system's massive particles in one wave),
         // then the massless particles
         int mw_used = 0, cur_w = 0, in_w = 0;
@@ -1369,8 +1355,7 @@ static int build_packed(nb_batch *b) {
             const int32_t q = k.sys[j];
             const int nm = b->h_nm[(size_t)q];
             bi[5 + j] = q;
-            bi[5 + smax + j] = -1;
-            bi[5 + 2 * smax + j] = nm;
+            bi[5 + smax + j] = nm;
             if (nm == 0) continue;
             if (in_w + nm > 64) { ++cur_w; in_w = 0; }
             int mlane = 64 * cur_w + in_w;
@@ -1380,7 +1365,6 @@ static int build_packed(nb_batch *b) {
             bool firstm = true;
             for (int64_t i = b->h_off[q]; i < b->h_off[q + 1]; ++i) {
                 if (!b->h_massive[(size_t)i]) continue;
-                if (bi[5 + smax + j] < 0) bi[5 + smax + j] = (int32_t)i;
                 const bool central = f64 || i - b->h_off[q] < ncentral;
                 const int slot = central ? ci++ : ncb + fi++;
                 lane[kb * threads + mlane] = (int32_t)i;
@@ -1404,11 +1388,11 @@ static int build_packed(nb_batch *b) {
     NB_TRY(dalloc(&b->pk_lane, sizeof(int32_t) * lane.size()));
     NB_TRY(dalloc(&b->pk_meta, sizeof(int32_t) * meta.size()));
     NB_TRY(dalloc(&b->pk_blk, sizeof(int32_t) * blk.size()));
-    NB_TRY(dalloc(&b->pk_tiny, sizeof(double) * tiny.size()));
+    NB_TRY(dalloc(&b->pk_tiny, sizeof(double) * mmax_b.size()));
     NB_HIP(hipMemcpy(b->pk_lane, lane.data(), sizeof(int32_t) * lane.size(), hipMemcpyHostToDevice));
     NB_HIP(hipMemcpy(b->pk_meta, meta.data(), sizeof(int32_t) * meta.size(), hipMemcpyHostToDevice));
     NB_HIP(hipMemcpy(b->pk_blk, blk.data(), sizeof(int32_t) * blk.size(), hipMemcpyHostToDevice));
-    NB_HIP(hipMemcpy(b->pk_tiny, tiny.data(), sizeof(double) * tiny.size(), hipMemcpyHostToDevice));
+    NB_HIP(hipMemcpy(b->pk_tiny, mmax_b.data(), sizeof(double) * mmax_b.size(), hipMemcpyHostToDevice));
     b->pk_nblk = (int)nb;
     b->pk_threads = threads;
     b->pk_smax = smax;
@@ -1440,7 +1424,7 @@ static int resident_steps(nb_batch *b, int64_t nsteps, int every, double *energy
     if (packed_ok(b)) {
         if (b->pk_stale) NB_TRY(build_packed(b));
         PackArgs pa{};
-        pa.lane = b->pk_lane; pa.meta = b->pk_meta; pa.blk = b->pk_blk; pa.tiny = b->pk_tiny;
+        pa.lane = b->pk_lane; pa.meta = b->pk_meta; pa.blk = b->pk_blk; pa.mmax = b->pk_tiny;
         pa.smax = b->pk_smax; pa.recs = b->pk_recs;
         pa.Gs = b->Gs; pa.hs = b->hs; pa.dts = b->dts;
         pa.x = b->x; pa.y = b->y; pa.z = b->z; pa.vx = b->vx; pa.vy = b->vy; pa.vz = b->vz;
