@@ -1232,11 +1232,12 @@ static void resident_shape(const nb_batch *b, int &waves, int &nslot) {
     }
 }
 
-// The packed layout applies when no wave count is forced and every system
-// has at most 64 massive particles and fits one workgroup (one massive wave
-// plus its massless particles in at most 15 more).
+// The packed layout applies when no wave count is forced, the particle
+// indices fit its 32-bit lane tables, and every system has at most 64 massive
+// particles and fits one workgroup (one massive wave plus its massless
+// particles in at most 15 more).
 static bool packed_ok(const nb_batch *b) {
-    if (b->tun.resident_waves != 0 || b->h_nm.size() != (size_t)b->nsys) return false;
+    if (b->tun.resident_waves != 0 || b->h_nm.size() != (size_t)b->nsys || b->n >= INT32_MAX) return false;
     for (int32_t q = 0; q < b->nsys; ++q) {
         const int64_t ns = b->h_off[q + 1] - b->h_off[q], nm = b->h_nm[(size_t)q];
         if (nm > 64 || 1 + (ns - nm + 63) / 64 > 16) return false;
@@ -1268,10 +1269,16 @@ static int build_packed(nb_batch *b) {
         if (nc[p] != nc[q]) return nc[p] < nc[q];
         return b->h_nm[(size_t)p] < b->h_nm[(size_t)q];
     });
-    // records per system are at most 65 (32 B each, three buffers): S also
-    // keeps the workgroup's LDS within 64 KiB
-    int rmax = 1;
-    for (int32_t q = 0; q < ns_sys; ++q) rmax = std::max(rmax, b->h_nm[(size_t)q] + 1);
+    // S also keeps the workgroup's LDS (three buffers of 32-byte records)
+    // within 64 KiB.  Records per system: the largest central count plus the
+    // largest other massive count plus one (at most 129), which may come from
+    // different systems.
+    int nc_max = 0, nf_max = 0;
+    for (int32_t q = 0; q < ns_sys; ++q) {
+        nc_max = std::max(nc_max, nc[q]);
+        nf_max = std::max(nf_max, b->h_nm[(size_t)q] - nc[q]);
+    }
+    const int rmax = nc_max + nf_max + 1;
     const int s_lds = std::max(1, (65536 - 1024 * (int)sizeof(double)) / (3 * rmax * (int)sizeof(ResSrc)));
     // S minimises the estimated makespan ceil(workgroups / CUs) x waves per
     // workgroup (the kernel is bound by VALU issue, and a CU runs its

@@ -462,6 +462,49 @@ def test_packed_resident_mixed_shapes(pkg, orc):
         assert np.max(np.abs(d["vel"][sl] - vo)) < 1e-8, s
 
 
+def test_packed_resident_mixed_central_counts(pkg, orc):
+    """Two central bodies per system (set_central(2): the star and particle
+    1), where a quarter of the systems have a massless test particle at index
+    1 and so one central body.  Sorted by central count, the workgroup that
+    holds the last one-central system (1203) and the first two-central ones
+    (0, 2, ...) pads the former's FP64 slots with a zero record.  Every system
+    is bit-identical to one wave per system (which has no padding), and a
+    sample follows the oracle's run loop."""
+    shapes = [(3, 0), (8, 64), (3, 100), (8, 20)]
+    parts = []
+    for k in range(1204):
+        npl, nt = shapes[k % 4]
+        e = pkg.ics.planetary_ensemble(nsys=1, seed=3000 + k, nplanets=npl, ntest=nt)
+        pos, vel, mass = e["pos"], e["vel"], e["mass"]
+        if k % 4 == 3:   # a test particle at index 1
+            idx = np.arange(len(mass))
+            idx[[1, 10]] = idx[[10, 1]]
+            pos, vel, mass = pos[idx], vel[idx], mass[idx]
+        parts.append((pos, vel, mass))
+    off = np.concatenate([[0], np.cumsum([len(p[2]) for p in parts])]).astype(np.int64)
+    ens = {"offsets": off, "pos": np.concatenate([p[0] for p in parts]),
+           "vel": np.concatenate([p[1] for p in parts]), "mass": np.concatenate([p[2] for p in parts])}
+    steps, dt = 100, 1e-3
+    out = {}
+    for w in (0, 1):
+        with make(pkg, ens, "split_central", dt=dt, resident=1, resident_waves=w) as e:
+            e.set_central(2)
+            e.compute_force()
+            e.half_kick(+1)
+            e.kdk_steps(steps, use_graph=True)
+            e.half_kick(-1)
+            out[w] = e.download()
+    for k in ("pos", "vel", "acc"):
+        assert np.array_equal(out[0][k], out[1][k]), k
+    for s in (0, 1, 2, 3, 1199, 1203):
+        sl = slice(off[s], off[s + 1])
+        xo, vo, _, _ = orc.run_const_kdk(ens["pos"][sl], ens["vel"][sl], ens["mass"][sl], [0.0] * 6,
+                                         1.0, t_begin=0.0, t_end=steps * dt, dt=dt,
+                                         between_snapshots=1.0, skip_self=True)
+        assert np.max(np.abs(out[0]["pos"][sl] - xo)) < 1e-9, s
+        assert np.max(np.abs(out[0]["vel"][sl] - vo)) < 1e-8, s
+
+
 def test_tracked_energy(pkg, orc, planets):
     steps, every, dt = 400, 100, 1e-3
     with make(pkg, planets, "split_central", dt=dt, resident=1) as e:
