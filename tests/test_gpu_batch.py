@@ -505,6 +505,54 @@ def test_packed_resident_mixed_central_counts(pkg, orc):
         assert np.max(np.abs(out[0]["vel"][sl] - vo)) < 1e-8, s
 
 
+@pytest.mark.parametrize("mode", ["split", "fp64"])
+def test_packed_resident_softened_clusters(pkg, orc, mode):
+    """1100 softened Plummer clusters of 8 to 64 bodies (h = 0.01 a, every
+    body massive) in the packed layout (several clusters per massive wave),
+    with one cluster in four given 30 massless test particles: the softened
+    branch of the resident kernel (the spline inside h, in FP32 pairs for
+    split precision and in FP64 for the FP64 mode).  Bit-identical to one
+    wave per system; a sample follows the oracle's softened run loop for
+    100 steps, and the tracked energies agree."""
+    ens = pkg.ics.cluster_ensemble(nsys=1100, seed=17, nmin=8, nmax=64)
+    off, pos, vel, mass = ens["offsets"], ens["pos"], ens["vel"], ens["mass"]
+    rng = np.random.default_rng(4)
+    parts, hs = [], []
+    for s in range(len(off) - 1):
+        sl = slice(off[s], off[s + 1])
+        p, v, m = pos[sl], vel[sl], mass[sl]
+        if s % 4 == 1:
+            tp = rng.normal(0.0, ens["a"][s], (30, 3))
+            p, v, m = np.vstack([p, tp]), np.vstack([v, np.zeros((30, 3))]), np.concatenate([m, np.zeros(30)])
+        parts.append((p, v, m))
+    off2 = np.concatenate([[0], np.cumsum([len(q[2]) for q in parts])]).astype(np.int64)
+    e2 = {"offsets": off2, "pos": np.concatenate([q[0] for q in parts]),
+          "vel": np.concatenate([q[1] for q in parts]), "mass": np.concatenate([q[2] for q in parts]),
+          "h": ens["h"]}
+    steps, every, dt = 100, 50, 1e-3
+    out = {}
+    for w in (0, 1):
+        with make(pkg, e2, mode, dt=dt, resident=1, resident_waves=w) as e:
+            e.compute_force()
+            e.half_kick(+1)
+            E = e.kdk_steps_tracked(steps, every)
+            e.half_kick(-1)
+            out[w] = (e.download(), E)
+    d = out[0][0]
+    for k in ("pos", "vel", "acc"):
+        assert np.array_equal(d[k], out[1][0][k]), k
+    assert np.all(np.abs(out[1][1] - out[0][1]) <= 1e-13 * np.abs(out[0][1]))
+    # FP32 pairs: a relative force error of about 1e-7 moves a body by about
+    # 0.5e-7 a t^2 ~ 5e-10 in 0.1 time units; close encounters amplify it
+    tol_x, tol_v = (1e-8, 1e-7) if mode == "split" else (1e-12, 1e-11)
+    for s in (0, 1, 2, 5, 500, 1099):
+        sl = slice(off2[s], off2[s + 1])
+        xo, vo, _, _ = orc.run_const_kdk(e2["pos"][sl], e2["vel"][sl], e2["mass"][sl], [ens["h"][s]] * 6,
+                                         1.0, t_begin=0.0, t_end=steps * dt, dt=dt, between_snapshots=1.0)
+        assert np.max(np.abs(d["pos"][sl] - xo)) < tol_x, s
+        assert np.max(np.abs(d["vel"][sl] - vo)) < tol_v, s
+
+
 def test_tracked_energy(pkg, orc, planets):
     steps, every, dt = 400, 100, 1e-3
     with make(pkg, planets, "split_central", dt=dt, resident=1) as e:
