@@ -131,7 +131,9 @@ def write_gadget2(filename: str, header: HeaderGadget2, Pos, Vel, ID, Mass, Coll
         _block(f, "POS", vec(Pos), format2, endian)
         _block(f, "VEL", vec(Vel), format2, endian)
         _block(f, "ID", idb, format2, endian)
-        var = np.array([header.mass[t - 1] == 0.0 for t in ct], dtype=bool)
+        # (vectorised: the writer runs beside the simulation's thread, and a
+        # per-particle Python loop here would hold the interpreter lock)
+        var = np.asarray(header.mass, dtype=np.float64)[ct - 1] == 0.0
         if var.any():
             m = np.asarray(Mass, dtype=np.float64)[order][var]
             _block(f, "MASS", m.astype(f32).tobytes(), format2, endian)
@@ -218,8 +220,8 @@ def read_gadget2(filename: str):
     out["VEL"] = np.frombuffer(vel, dtype=f4).astype(np.float64).reshape(n, 3)
     idt = np.dtype(endian + ("u8" if len(idr) == 8 * n else "u4"))
     out["ID"] = np.frombuffer(idr, dtype=idt).astype(np.int64)
-    var = np.array([header.mass[t - 1] == 0.0 for t in ct], dtype=bool)
-    mass = np.array([header.mass[t - 1] for t in ct], dtype=np.float64)
+    mass = np.asarray(header.mass, dtype=np.float64)[ct.astype(np.int64) - 1]
+    var = mass == 0.0
     if var.any():
         mb = get(4, "MASS")
         if mb is None or len(mb) != 4 * int(var.sum()):
