@@ -421,7 +421,7 @@ int nb_adaptive_compute_force(nb_ctx *c, int64_t t_int, int64_t *nactive) {
     void *gx = gaz + n;
     void *gy = (char *)gx + (c->is_double ? 8 : 4) * (size_t)n;
     void *gz = (char *)gy + (c->is_double ? 8 : 4) * (size_t)n;
-    NB_TRY(launch_tile_boxes(c->x, c->y, c->z, c->is_double, n, c->tile_box, s));
+    NB_TRY(launch_tile_boxes(c->x, c->y, c->z, c->is_double, n, c->tile_box, c->sub_box, s));
     if (c->is_double)
         k_gather_targets<double><<<gridn(na), 256, 0, s>>>(
             (const double *)c->x, (const double *)c->y, (const double *)c->z, c->h, c->ad.idx,
@@ -446,7 +446,7 @@ int nb_adaptive_compute_force(nb_ctx *c, int64_t t_int, int64_t *nactive) {
         a.th = gh; a.nt = na;
         a.sx = (const double *)c->x; a.sy = (const double *)c->y; a.sz = (const double *)c->z;
         a.sm = (const double *)c->m; a.ns = n;
-        a.tile_box = c->tile_box; a.use_origin = 1;
+        a.tile_box = c->tile_box; a.sub_box = c->sub_box; a.use_origin = 1;
         a.nseg = nseg; a.seg_len = tiles_per_seg * BS; a.nblk_i = (int)nblk; a.G = c->G;
         a.ax = gax; a.ay = gay; a.az = gaz; a.pot = nullptr; a.part = c->part;
         a.skip_src = (c->mode == NB_MODE_SPLIT_CENTRAL && c->ncentral > 0) ? c->central : nullptr;
@@ -467,7 +467,7 @@ int nb_adaptive_compute_force(nb_ctx *c, int64_t t_int, int64_t *nactive) {
             // FP64 central terms for the active targets
             ForceArgs<double> b = a;
             b.sx = c->cx; b.sy = c->cy; b.sz = c->cz; b.sm = c->cm; b.ns = c->ncentral;
-            b.skip_src = nullptr; b.tile_box = nullptr; b.accumulate = 1;
+            b.skip_src = nullptr; b.tile_box = nullptr; b.sub_box = nullptr; b.accumulate = 1;
             b.nseg = 1; b.part = nullptr;
             b.seg_len = ((c->ncentral + BS - 1) / BS) * BS;
             NB_TRY(ctx_gather_central(c));
@@ -479,7 +479,7 @@ int nb_adaptive_compute_force(nb_ctx *c, int64_t t_int, int64_t *nactive) {
         a.th = gh; a.nt = na;
         a.sx = (const float *)c->x; a.sy = (const float *)c->y; a.sz = (const float *)c->z;
         a.sm = (const float *)c->m; a.ns = n;
-        a.tile_box = c->tile_box; a.use_origin = 0;
+        a.tile_box = c->tile_box; a.sub_box = c->sub_box; a.use_origin = 0;
         a.nseg = nseg; a.seg_len = tiles_per_seg * BS; a.nblk_i = (int)nblk; a.G = c->G;
         a.ax = gax; a.ay = gay; a.az = gaz; a.pot = nullptr; a.part = c->part;
         const int fp = c->tun.far_path == FAR_MFMA_BF16 ? FAR_MFMA_BF16 : FAR_VALU;

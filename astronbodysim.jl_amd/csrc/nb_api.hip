@@ -223,7 +223,7 @@ int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = tru
     if (allow_sort) NB_TRY(maybe_sort(c, 1));
     c->force_calls++;
     c->calls_since_sort++;
-    NB_TRY(launch_tile_boxes(c->x, c->y, c->z, c->is_double, n, c->tile_box, s));
+    NB_TRY(launch_tile_boxes(c->x, c->y, c->z, c->is_double, n, c->tile_box, c->sub_box, s));
     const bool f64 = c->mode == NB_MODE_FP64;
     const int tpt = f64 ? 1 : c->tun.targets_per_thread;
     Geometry g = geometry(n, n, tpt, c->tun.j_segments);
@@ -237,6 +237,7 @@ int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = tru
         a.sx = (const double *)c->x; a.sy = (const double *)c->y; a.sz = (const double *)c->z;
         a.sm = (const double *)c->m; a.ns = n;
         a.tile_box = c->tile_box;
+        a.sub_box = c->sub_box;
         a.use_origin = 1;
         a.nseg = g.nseg; a.seg_len = g.seg_len; a.nblk_i = g.nblk_i;
         a.G = c->G;
@@ -257,6 +258,7 @@ int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = tru
         a.sx = (const float *)c->x; a.sy = (const float *)c->y; a.sz = (const float *)c->z;
         a.sm = (const float *)c->m; a.ns = n;
         a.tile_box = c->tile_box;
+        a.sub_box = c->sub_box;
         a.use_origin = 0;
         a.nseg = g.nseg; a.seg_len = g.seg_len; a.nblk_i = g.nblk_i;
         a.G = c->G;
@@ -425,6 +427,7 @@ int nb_create(nb_ctx **out, int64_t n, int32_t precision_mode) {
             (rc = dalloc(&c->az, sizeof(double) * nn)) ||
             (rc = dalloc(&c->pot, sizeof(double) * nn)) ||
             (rc = dalloc(&c->tile_box, sizeof(double) * 8 * (size_t)((n + BS - 1) / BS + 1))) ||
+            (rc = dalloc(&c->sub_box, sizeof(double) * 8 * NQ * (size_t)((n + BS - 1) / BS + 1))) ||
             (rc = dalloc(&c->stage, sizeof(double) * 3 * nn)) ||
             (rc = dalloc(&c->wsrc, sizeof(float) * nn)) ||
             (rc = dalloc(&c->red, sizeof(double) * (ENERGY_BLOCKS * 8 + 8))) ||
@@ -474,7 +477,7 @@ int nb_destroy(nb_ctx *c) {
     dfree(c->m); dfree(c->h); dfree(c->coll);
     dfree(c->ax); dfree(c->ay); dfree(c->az); dfree(c->pot);
     dfree(c->perm);
-    dfree(c->tile_box); dfree(c->part); dfree(c->stage); dfree(c->red); dfree(c->wsrc);
+    dfree(c->tile_box); dfree(c->sub_box); dfree(c->part); dfree(c->stage); dfree(c->red); dfree(c->wsrc);
     dfree(c->time); dfree(c->flag); dfree(c->central);
     dfree(c->cx); dfree(c->cy); dfree(c->cz); dfree(c->cm); dfree(c->cidx);
     dfree(c->ti_end); dfree(c->ti_beg);
@@ -656,7 +659,7 @@ static int probe_points(nb_ctx *c, int64_t npts, const double *pts, double h, do
         hipError_t e = hipMemcpyAsync(aos, pts, sizeof(double) * 3 * (size_t)npts,
                                       hipMemcpyHostToDevice, s);
         if (e != hipSuccess) { set_error("probe upload: %s", hipGetErrorString(e)); rc = NB_ERR_HIP; break; }
-        if ((rc = launch_tile_boxes(c->x, c->y, c->z, c->is_double, c->n, c->tile_box, s))) break;
+        if ((rc = launch_tile_boxes(c->x, c->y, c->z, c->is_double, c->n, c->tile_box, c->sub_box, s))) break;
         Geometry g = geometry(npts, c->n, 1, 1);
         if (c->is_double) {
             if ((rc = to_soa(aos, nullptr, npts, tx, ty, tz, true, s))) break;
@@ -664,7 +667,7 @@ static int probe_points(nb_ctx *c, int64_t npts, const double *pts, double h, do
             a.tx = tx; a.ty = ty; a.tz = tz; a.th = nullptr; a.th_const = h; a.nt = npts;
             a.sx = (const double *)c->x; a.sy = (const double *)c->y; a.sz = (const double *)c->z;
             a.sm = (const double *)c->m; a.ns = c->n;
-            a.tile_box = c->tile_box; a.use_origin = 1;
+            a.tile_box = c->tile_box; a.sub_box = c->sub_box; a.use_origin = 1;
             a.nseg = 1; a.seg_len = g.seg_len; a.nblk_i = g.nblk_i; a.G = c->G;
             a.ax = oax; a.ay = oay; a.az = oaz; a.pot = opot; a.accumulate = 0;
             rc = c->mode == NB_MODE_FP64 ? launch_force_f64(a, with_pot, s)
@@ -678,7 +681,7 @@ static int probe_points(nb_ctx *c, int64_t npts, const double *pts, double h, do
             a.tx = fx; a.ty = fy; a.tz = fz; a.th = nullptr; a.th_const = h; a.nt = npts;
             a.sx = (const float *)c->x; a.sy = (const float *)c->y; a.sz = (const float *)c->z;
             a.sm = (const float *)c->m; a.ns = c->n;
-            a.tile_box = c->tile_box; a.use_origin = 0;
+            a.tile_box = c->tile_box; a.sub_box = c->sub_box; a.use_origin = 0;
             a.nseg = 1; a.seg_len = g.seg_len; a.nblk_i = g.nblk_i; a.G = c->G;
             a.ax = oax; a.ay = oay; a.az = oaz; a.pot = opot; a.accumulate = 0;
             if ((rc = launch_force_f32(a, 1, with_pot, s))) break;

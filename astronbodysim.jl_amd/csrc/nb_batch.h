@@ -49,6 +49,7 @@ struct nb_batch {
     int64_t *tile_base = nullptr; // [nsys+1]
     int32_t *tile_sys = nullptr;  // [ntiles]
     double *tile_box = nullptr;   // [ntiles][8]
+    double *sub_box = nullptr;    // [ntiles][NQ][8] boxes of the tile quarters
     // work items of the ragged kernel: (system << 32) | target tile, by cost
     int64_t nitems = 0;
     int64_t *items = nullptr;

@@ -55,7 +55,7 @@ __global__ __launch_bounds__(BS) void k_batch_tile_boxes(const double *__restric
                                                         const int64_t *__restrict__ off,
                                                         const int64_t *__restrict__ tile_base,
                                                         const int32_t *__restrict__ tile_sys,
-                                                        double *__restrict__ box) {
+                                                        double *__restrict__ box, double *__restrict__ sub) {
     __shared__ double red[6][NW];
     const int64_t t = blockIdx.x;
     const int s = tile_sys[t];
@@ -76,7 +76,11 @@ __global__ __launch_bounds__(BS) void k_batch_tile_boxes(const double *__restric
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 0) {
 #pragma unroll
-        for (int d = 0; d < 3; ++d) { red[d][w] = lo[d]; red[3 + d][w] = hi[d]; }
+        for (int d = 0; d < 3; ++d) {
+            red[d][w] = lo[d]; red[3 + d][w] = hi[d];
+            sub[(t * NQ + w) * 8 + d] = lo[d];   // the quarter's box (empty: lo > hi)
+            sub[(t * NQ + w) * 8 + 3 + d] = hi[d];
+        }
     }
     __syncthreads();
     if (threadIdx.x < 6) {
@@ -91,6 +95,7 @@ struct BatchArgs {
     const int64_t *off;
     const int64_t *tile_base;
     const double *tile_box;
+    const double *sub_box;        // [ntiles][NQ][8]: boxes of the tile quarters
     const double *Gs;
     const double *x, *y, *z, *m, *h;
     const float *w;               // source weights (weighted far paths) or nullptr
@@ -132,7 +137,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MF == 4 ? 4 
         // force_block starts with a barrier, so item_sh is read by every
         // thread before thread 0 overwrites it in the next iteration.
         force_block<double, TPT, POT, MF, ACC>(sh, a.x, a.y, a.z, a.h, 0.0, i0, o1, a.x, a.y, a.z, a.m, a.w, o0,
-                                      o1, a.tile_box + a.tile_base[s] * 8, a.skip_src, true, Ax,
+                                      o1, a.tile_box + a.tile_base[s] * 8,
+                                      a.sub_box + a.tile_base[s] * (NQ * 8), a.skip_src, true, Ax,
                                       Ay, Az, Aq, valid);
         const double G = a.Gs[s];
 #pragma unroll
@@ -1082,7 +1088,7 @@ static int bforce(nb_batch *b, bool with_pot, bool allow_timing, bool pot_only =
     NB_TRY(build_items(b, tpt));
     b->calls_since_sort++;
     BatchArgs a{};
-    a.off = b->off; a.tile_base = b->tile_base; a.tile_box = b->tile_box; a.Gs = b->Gs;
+    a.off = b->off; a.tile_base = b->tile_base; a.tile_box = b->tile_box; a.sub_box = b->sub_box; a.Gs = b->Gs;
     a.x = b->x; a.y = b->y; a.z = b->z; a.m = b->m; a.h = b->h;
     a.skip_src = (b->mode == NB_MODE_SPLIT_CENTRAL && b->ncentral > 0) ? b->central : nullptr;
     a.ax = b->ax; a.ay = b->ay; a.az = b->az; a.pot = b->pot;
@@ -1098,7 +1104,8 @@ static int bforce(nb_batch *b, bool with_pot, bool allow_timing, bool pot_only =
         NB_TRY(launch_ok("k_batch_force_f64"));
     } else {
         k_batch_tile_boxes<<<(unsigned)b->ntiles, BS, 0, s>>>(b->x, b->y, b->z, b->off,
-                                                              b->tile_base, b->tile_sys, b->tile_box);
+                                                              b->tile_base, b->tile_sys, b->tile_box,
+                                                              b->sub_box);
         NB_TRY(launch_ok("k_batch_tile_boxes"));
         pot_only = pot_only && with_pot;
         // potential alone on far path 4: m/r = rsq(w r^2) m^(2/3)
@@ -1618,6 +1625,7 @@ int nb_batch_create(nb_batch **out, int32_t nsys, const int64_t *offsets, int32_
             (rc = dalloc(&b->tile_base, sizeof(int64_t) * (ns + 1))) ||
             (rc = dalloc(&b->tile_sys, sizeof(int32_t) * (size_t)std::max<int64_t>(b->ntiles, 1))) ||
             (rc = dalloc(&b->tile_box, sizeof(double) * 8 * (size_t)std::max<int64_t>(b->ntiles, 1))) ||
+            (rc = dalloc(&b->sub_box, sizeof(double) * 8 * NQ * (size_t)std::max<int64_t>(b->ntiles, 1))) ||
             (rc = dalloc(&b->counter, sizeof(unsigned) * 2)) ||
             (rc = dalloc(&b->stage, sizeof(double) * 3 * n)) ||
             (rc = dalloc(&b->wsrc, sizeof(float) * std::max<int64_t>(n, 1))) ||
@@ -1664,7 +1672,7 @@ int nb_batch_destroy(nb_batch *b) {
     dfree(b->x); dfree(b->y); dfree(b->z); dfree(b->vx); dfree(b->vy); dfree(b->vz);
     dfree(b->m); dfree(b->h); dfree(b->ax); dfree(b->ay); dfree(b->az); dfree(b->pot);
     dfree(b->perm); dfree(b->central); dfree(b->cidx);
-    dfree(b->tile_base); dfree(b->tile_sys); dfree(b->tile_box);
+    dfree(b->tile_base); dfree(b->tile_sys); dfree(b->tile_box); dfree(b->sub_box);
     dfree(b->items); dfree(b->counter); dfree(b->stage); dfree(b->red); dfree(b->wsrc);
     dfree(b->sort_tmp); dfree(b->keys); dfree(b->idx); dfree(b->sysbox); dfree(b->spare);
     dfree(b->pk_lane); dfree(b->pk_meta); dfree(b->pk_blk); dfree(b->pk_tiny);

@@ -106,7 +106,8 @@ __device__ void block_stats(const double (&px)[TPT], const double (&py)[TPT], co
 // placed far away, so both tile paths can run a fixed trip count.
 constexpr float PAD_COORD = 1.0e18f;
 
-template <int TPT, bool CHECKED, bool POT>
+// NS sources from s: a whole tile (BS) or one 64-source quarter of it.
+template <int TPT, bool CHECKED, bool POT, int NS = BS>
 __device__ __forceinline__ void tile_f32(const float4 *__restrict__ s, const float (&xi)[TPT],
                                          const float (&yi)[TPT], const float (&zi)[TPT],
                                          const float (&h2c)[TPT], const float (&h2)[TPT],
@@ -114,7 +115,7 @@ __device__ __forceinline__ void tile_f32(const float4 *__restrict__ s, const flo
                                          float (&ax)[TPT], float (&ay)[TPT], float (&az)[TPT],
                                          float (&aq)[TPT]) {
 #pragma unroll 4
-    for (int jj = 0; jj < BS; ++jj) {
+    for (int jj = 0; jj < NS; ++jj) {
         const float4 v = s[jj];
 #pragma unroll
         for (int k = 0; k < TPT; ++k)
@@ -123,7 +124,7 @@ __device__ __forceinline__ void tile_f32(const float4 *__restrict__ s, const flo
     }
 }
 
-template <int TPT, bool CHECKED, bool POT>
+template <int TPT, bool CHECKED, bool POT, int NS = BS>
 __device__ __forceinline__ void tile_f32_hilo(const float4 *__restrict__ s,
                                               const float4 *__restrict__ sl,
                                               const float (&xi)[TPT], const float (&yi)[TPT],
@@ -134,7 +135,7 @@ __device__ __forceinline__ void tile_f32_hilo(const float4 *__restrict__ s,
                                               float (&ax)[TPT], float (&ay)[TPT],
                                               float (&az)[TPT], float (&aq)[TPT]) {
 #pragma unroll 2
-    for (int jj = 0; jj < BS; ++jj) {
+    for (int jj = 0; jj < NS; ++jj) {
         const float4 v = s[jj];
         const float4 w = sl[jj];
 #pragma unroll
@@ -152,7 +153,7 @@ struct SrcF64 {
     float m, pad;
 };
 
-template <int TPT, bool CHECKED, bool POT>
+template <int TPT, bool CHECKED, bool POT, int NS = BS>
 __device__ __forceinline__ void tile_f32_d(const SrcF64 *__restrict__ s, const double (&xi)[TPT],
                                            const double (&yi)[TPT], const double (&zi)[TPT],
                                            const float (&h2c)[TPT], const float (&h2)[TPT],
@@ -160,7 +161,7 @@ __device__ __forceinline__ void tile_f32_d(const SrcF64 *__restrict__ s, const d
                                            float (&ax)[TPT], float (&ay)[TPT], float (&az)[TPT],
                                            float (&aq)[TPT]) {
 #pragma unroll 2
-    for (int jj = 0; jj < BS; ++jj) {
+    for (int jj = 0; jj < NS; ++jj) {
         const SrcF64 v = s[jj];
 #pragma unroll
         for (int k = 0; k < TPT; ++k)
@@ -526,6 +527,14 @@ __device__ __forceinline__ void tile_mf3(const float4 *__restrict__ s, const nb_
     for (int jc = 16; jc < BS; jc += 16) mf3_chunk<false>(s + jc, sa[jc + c16][grp], grp, Bg, S);
 }
 
+// One 64-source quarter of a tile (sources 64q..64q+63), added into S.
+__device__ __forceinline__ void quarter_mf3(const float4 *__restrict__ s, const nb_bf8 (*__restrict__ sa)[4],
+                                            int lane, int q, const nb_bf8 (&Bg)[4], float (&S)[4][4]) {
+    const int c16 = lane & 15, grp = lane >> 4;
+#pragma unroll 1
+    for (int jc = SB * q; jc < SB * q + SB; jc += 16) mf3_chunk<false>(s + jc, sa[jc + c16][grp], grp, Bg, S);
+}
+
 // Potential only (no accelerations): with q = rsq(w_j r^2) = m_j^(1/3) / r
 // from the same product, m_j / r = q c_j where c_j = m_j^(2/3) = 1 / w_j
 // (0 for massless sources) rides in the w slot of the lo coordinates: rsq and
@@ -555,6 +564,14 @@ __device__ __forceinline__ void tile_mf3_pot(const float4 *__restrict__ sl,
     mf3_pot_chunk<true>(sl, sa[c16][grp], grp, Bg, S);
 #pragma unroll 1
     for (int jc = 16; jc < BS; jc += 16) mf3_pot_chunk<false>(sl + jc, sa[jc + c16][grp], grp, Bg, S);
+}
+
+__device__ __forceinline__ void quarter_mf3_pot(const float4 *__restrict__ sl,
+                                                const nb_bf8 (*__restrict__ sa)[4], int lane, int q,
+                                                const nb_bf8 (&Bg)[4], float (&S)[4]) {
+    const int c16 = lane & 15, grp = lane >> 4;
+#pragma unroll 1
+    for (int jc = SB * q; jc < SB * q + SB; jc += 16) mf3_pot_chunk<false>(sl + jc, sa[jc + c16][grp], grp, Bg, S);
 }
 
 // FAR_MFMA_BF16 fold: target 16g + c sits in lanes c, 16 + c, 32 + c, 48 + c
@@ -681,7 +698,11 @@ __device__ __forceinline__ int64_t target_of(int64_t i0, int k, int tid) {
 
 // One block's share of an all-pairs sum: targets target_of(i0, k, tid) (k < TPT,
 // i < i_end) against the sources [j0, j1).  tile_box points at the box of
-// the BS-source tile that starts at j0 (tiles are BS-aligned from j0).
+// the BS-source tile that starts at j0 (tiles are BS-aligned from j0);
+// sub_box, when not null, at the boxes of that tile's NQ quarters (SB sources
+// each, [tile][NQ][8]).  With far path 0 (and 4 on FP32 state) a wave then
+// classifies each quarter on its own, so a tile that only touches the wave's
+// targets with one corner runs the cheaper arithmetic on its other quarters.
 // Returns FP64 sums (without G) in Ax..Aq and the target validity.  With
 // ACC = false (and POT) only Aq is meaningful: the kernel writes the
 // potential alone, so the acceleration arithmetic is dead and removed.
@@ -691,7 +712,7 @@ __device__ __forceinline__ void force_block(
     const T *__restrict__ tz, const double *__restrict__ th, double th_const, int64_t i0,
     int64_t i_end, const T *__restrict__ sx, const T *__restrict__ sy, const T *__restrict__ sz,
     const T *__restrict__ sm, const float *__restrict__ sw, int64_t j0, int64_t j1, const double *__restrict__ tile_box,
-    const uint8_t *__restrict__ skip_src, bool use_origin, double (&Ax)[TPT], double (&Ay)[TPT],
+    const double *__restrict__ sub_box, const uint8_t *__restrict__ skip_src, bool use_origin, double (&Ax)[TPT], double (&Ay)[TPT],
     double (&Az)[TPT], double (&Aq)[TPT], bool (&valid)[TPT]) {
     constexpr bool HILO = BlockShared<T, MF>::HILO;
     constexpr bool ABSNEAR = BlockShared<T, MF>::ABSNEAR;
@@ -758,7 +779,14 @@ __device__ __forceinline__ void force_block(
     // the B column of target 16g + (l & 15).
     constexpr bool ROWS = BlockShared<T, MF>::ROWS;
     nb_bf8 Bg[4];
-    uint64_t m_soft = 0, m_close = 0, m_far = 0;   // classes of tiles 64k..64k+63
+    // Classes of the quarters of tiles 16k..16k+15: bit 4 (t - 16k) + q.  Far
+    // path 0, and far path 4 with FP32 state, classify quarters; the others
+    // classify whole tiles (the four bits of a tile agree).  (The FP64-state
+    // path-4 kernel would spill inside its far loop with the quarter code, and
+    // its near tiles are few: the far reach grows with the block's extent.)
+    constexpr bool QUARTERS = MF == FAR_VALU || (MF == FAR_MFMA_BF16 && ABSNEAR);
+    const bool by_quarter = QUARTERS && sub_box != nullptr;
+    uint64_t m_soft = 0, m_close = 0, m_far = 0;
     if constexpr (ROWS) {
         const int lane = tid & 63;
         const double dx = px[0] - ox, dy = py[0] - oy, dz = pz[0] - oz;
@@ -899,13 +927,16 @@ __device__ __forceinline__ void force_block(
                 if (WEIGHTS) rw = sw ? sw[j] : (rm > 0 ? (float)(1.0 / cbrt((double)rm * (double)rm)) : 0.0f);
             }
         }
-        // Tile classes of the next 64 tiles, one per lane, as wave-wide bit
-        // masks (the per-tile test is then scalar).
-        if ((it & 63) == 0) {
-            const int64_t t = it + (tid & 63);
-            bool c_soft = false, c_close = false, c_far = false;
-            if (j0 + t * BS < j1) {
-                const double g2 = tile_gap2(tile_box + t * 8, wg);
+        // Quarter classes of the next 16 tiles, one per lane, as wave-wide bit
+        // masks (the per-tile test is then scalar).  A quarter past the end of
+        // the sources holds padding only, which every path sums to zero.
+        if ((it & 15) == 0) {
+            const int lane = tid & 63;
+            const int64_t t = it + (lane >> 2);
+            const int q = lane & 3;
+            bool c_soft = false, c_close = false, c_far = true;
+            if (j0 + t * BS + (by_quarter ? q * SB : 0) < j1) {
+                const double g2 = tile_gap2(by_quarter ? sub_box + (t * NQ + q) * 8 : tile_box + t * 8, wg);
                 c_soft = g2 <= soft_d2;
                 c_close = !c_soft && g2 <= close_d2;
                 c_far = g2 > wg.farw_d2;
@@ -914,8 +945,76 @@ __device__ __forceinline__ void force_block(
             m_close = __builtin_amdgcn_ballot_w64(c_close);
             m_far = __builtin_amdgcn_ballot_w64(c_far);
         }
-        const uint64_t tbit = 1ull << (it & 63);
-        if (m_soft & tbit) {
+        const int tsh = 4 * (it & 15);
+        const unsigned q_soft = (unsigned)(m_soft >> tsh) & 15u;
+        const unsigned q_close = (unsigned)(m_close >> tsh) & 15u;
+        const unsigned q_far = (unsigned)(m_far >> tsh) & 15u;
+        // far quarters with arithmetic of their own (on path 0 far is mid)
+        const unsigned q_fown = ROWS ? q_far : 0u;
+        if (by_quarter && q_soft != 15u && q_close != 15u && q_fown != 15u && (q_soft | q_close | q_fown) != 0u) {
+            // a tile of mixed quarters: its far quarters first (their sums
+            // folded before the near quarters' FP32 sums start)
+            if constexpr (QUARTERS) {
+                if constexpr (ROWS) {
+                    if (q_fown != 0u) {
+                        float S3[4][4], Sq[4];
+#pragma unroll
+                        for (int g = 0; g < 4; ++g) {
+                            Sq[g] = 0.0f;
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) S3[g][c] = 0.0f;
+                        }
+#pragma unroll 1
+                        for (int q = 0; q < NQ; ++q) {
+                            if (!(q_fown & (1u << q))) continue;
+                            if constexpr (!ACC)
+                                quarter_mf3_pot(ABSNEAR ? sh.sj[buf] : sh.sjl[HILO ? buf : 0], sh.sA[0], tid & 63,
+                                                q, Bg, Sq);
+                            else
+                                quarter_mf3(sh.sj[buf], sh.sA[0], tid & 63, q, Bg, S3);
+                        }
+                        if constexpr (!ACC)
+                            Aq[0] += (double)fold_mf3_one(Sq);
+                        else
+                            fold_mf3(S3, px[0] - ox, py[0] - oy, pz[0] - oz, Ax[0], Ay[0], Az[0]);
+                    }
+                }
+#pragma unroll 1
+                for (int q = 0; q < NQ; ++q) {
+                    const unsigned qb = 1u << q;
+                    if (q_fown & qb) continue;
+                    const int o = q * SB;
+                    if (q_soft & qb) {
+                        if (ABSNEAR)
+                            tile_f32<TPT, true, POT, SB>(sh.sjl[HILO ? buf : 0] + (HILO ? o : 0), xa, ya, za, h2c,
+                                                         h2, hinv, hinv3, ax, ay, az, aq);
+                        else if (D64NEAR)
+                            tile_f32_d<TPT, true, POT, SB>(sh.sjd[D64NEAR ? buf : 0] + (D64NEAR ? o : 0), xd, yd,
+                                                           zd, h2c, h2, hinv, hinv3, ax, ay, az, aq);
+                        else if (HILO)
+                            tile_f32_hilo<TPT, true, POT, SB>(sh.sj[buf] + o, sh.sjl[HILO ? buf : 0] + (HILO ? o : 0),
+                                                              xi, yi, zi, xil, yil, zil, h2c, h2, hinv, hinv3,
+                                                              ax, ay, az, aq);
+                        else
+                            tile_f32<TPT, true, POT, SB>(sh.sj[buf] + o, xi, yi, zi, h2c, h2, hinv, hinv3, ax, ay,
+                                                         az, aq);
+                    } else if (D64NEAR && (q_close & qb)) {
+                        tile_f32_d<TPT, false, POT, SB>(sh.sjd[D64NEAR ? buf : 0] + (D64NEAR ? o : 0), xd, yd, zd,
+                                                        h2c, h2, hinv, hinv3, ax, ay, az, aq);
+                    } else if (HILO && !ABSNEAR && (q_close & qb)) {
+                        tile_f32_hilo<TPT, false, POT, SB>(sh.sj[buf] + o, sh.sjl[HILO ? buf : 0] + (HILO ? o : 0),
+                                                           xi, yi, zi, xil, yil, zil, h2c, h2, hinv, hinv3, ax,
+                                                           ay, az, aq);
+                    } else if (ABSNEAR) {
+                        tile_f32<TPT, false, POT, SB>(sh.sjl[HILO ? buf : 0] + (HILO ? o : 0), xa, ya, za, h2c, h2,
+                                                      hinv, hinv3, ax, ay, az, aq);
+                    } else {
+                        tile_f32<TPT, false, POT, SB>(sh.sj[buf] + o, xi, yi, zi, h2c, h2, hinv, hinv3, ax, ay,
+                                                      az, aq);
+                    }
+                }
+            }
+        } else if (q_soft == 15u) {
             if (ABSNEAR)
                 tile_f32<TPT, true, POT>(sh.sjl[HILO ? buf : 0], xa, ya, za, h2c, h2, hinv, hinv3, ax, ay, az,
                                          aq);
@@ -928,13 +1027,13 @@ __device__ __forceinline__ void force_block(
             else
                 tile_f32<TPT, true, POT>(sh.sj[buf], xi, yi, zi, h2c, h2, hinv, hinv3, ax, ay, az,
                                          aq);
-        } else if (D64NEAR && (m_close & tbit)) {
+        } else if (D64NEAR && q_close == 15u) {
             tile_f32_d<TPT, false, POT>(sh.sjd[D64NEAR ? buf : 0], xd, yd, zd, h2c, h2, hinv, hinv3, ax, ay, az,
                                         aq);
-        } else if (HILO && !ABSNEAR && (m_close & tbit)) {
+        } else if (HILO && !ABSNEAR && q_close == 15u) {
             tile_f32_hilo<TPT, false, POT>(sh.sj[buf], sh.sjl[HILO ? buf : 0], xi, yi, zi, xil,
                                            yil, zil, h2c, h2, hinv, hinv3, ax, ay, az, aq);
-        } else if (ROWS && (m_far & tbit)) {
+        } else if (ROWS && q_far == 15u) {
             if constexpr (!ACC) {
                 float Sq[4];
                 tile_mf3_pot(ABSNEAR ? sh.sj[buf] : sh.sjl[HILO ? buf : 0], sh.sA[0], tid & 63, Bg, Sq);
@@ -945,7 +1044,7 @@ __device__ __forceinline__ void force_block(
                 fold_mf3(S3, px[0] - ox, py[0] - oy, pz[0] - oz, Ax[0], Ay[0], Az[0]);
             }
             continue;   // no FP32 VALU partials to flush
-        } else if (WFAR && (m_far & tbit)) {
+        } else if (WFAR && q_far == 15u) {
             tile_w<TPT>(sh.sj[buf], sh.sW[WFAR ? buf : 0], sh.sw[WFAR ? buf : 0], wbx, wby, wbz, wci,
                         Sx, Sy, Sz, Sm);
 #pragma unroll

@@ -70,6 +70,7 @@ struct nb_ctx {
     double *ax = nullptr, *ay = nullptr, *az = nullptr, *pot = nullptr;
     int64_t *perm = nullptr;      // internal -> caller index (nullptr: identity)
     double *tile_box = nullptr;
+    double *sub_box = nullptr;    // [ntiles][NQ][8] boxes of the tile quarters
     double *part = nullptr;
     int part_cap = 0;             // segments the partial buffer holds
     double *stage = nullptr;      // 3n doubles

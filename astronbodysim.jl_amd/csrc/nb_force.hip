@@ -37,7 +37,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MF == 4 ? 4 
     double Ax[TPT], Ay[TPT], Az[TPT], Aq[TPT];
     bool valid[TPT];
     force_block<T, TPT, POT, MF, ACC>(sh, a.tx, a.ty, a.tz, a.th, a.th_const, i0, a.nt, a.sx, a.sy, a.sz,
-                             a.sm, a.sw, j0, j1, a.tile_box + (j0 / BS) * 8, a.skip_src,
+                             a.sm, a.sw, j0, j1, a.tile_box + (j0 / BS) * 8,
+                             a.sub_box ? a.sub_box + (j0 / BS) * (NQ * 8) : nullptr, a.skip_src,
                              a.use_origin != 0, Ax, Ay, Az, Aq, valid);
 #pragma unroll
     for (int k = 0; k < TPT; ++k) {
@@ -139,11 +140,13 @@ __global__ void k_finalize(const double *__restrict__ part, int nseg, int64_t n,
     if (with_pot) pot[i] = -G * sq;
 }
 
-// Bounding box of each BS-source tile (lo xyz, hi xyz).
+// Bounding box of each BS-source tile (lo xyz, hi xyz) and, when sub is not
+// null, of each of its NQ quarters (the sources of one wave; an empty quarter
+// gets lo = DBL_MAX, hi = -DBL_MAX).
 template <typename T>
 __global__ __launch_bounds__(BS) void k_tile_boxes(const T *__restrict__ x, const T *__restrict__ y,
                                                    const T *__restrict__ z, int64_t n,
-                                                   double *__restrict__ box) {
+                                                   double *__restrict__ box, double *__restrict__ sub) {
     __shared__ double red[6][NW];
     const int64_t j = (int64_t)blockIdx.x * BS + threadIdx.x;
     const bool v = j < n;
@@ -162,6 +165,11 @@ __global__ __launch_bounds__(BS) void k_tile_boxes(const T *__restrict__ x, cons
     if (lane == 0) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) { red[d][w] = lo[d]; red[3 + d][w] = hi[d]; }
+    }
+    if (sub && lane == 0) {
+        double *o = sub + ((int64_t)blockIdx.x * NQ + w) * 8;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) { o[d] = lo[d]; o[3 + d] = hi[d]; }
     }
     __syncthreads();
     if (threadIdx.x < 6) {
@@ -185,15 +193,15 @@ static int check_launch(const char *what) {
 }
 
 int launch_tile_boxes(const void *x, const void *y, const void *z, int is_double, int64_t n,
-                      double *box, hipStream_t s) {
+                      double *box, double *sub_box, hipStream_t s) {
     if (n <= 0) return NB_OK;
     const int64_t ntiles = (n + BS - 1) / BS;
     if (is_double)
         k_tile_boxes<double><<<(unsigned)ntiles, BS, 0, s>>>((const double *)x, (const double *)y,
-                                                            (const double *)z, n, box);
+                                                            (const double *)z, n, box, sub_box);
     else
         k_tile_boxes<float><<<(unsigned)ntiles, BS, 0, s>>>((const float *)x, (const float *)y,
-                                                           (const float *)z, n, box);
+                                                           (const float *)z, n, box, sub_box);
     return check_launch("k_tile_boxes");
 }
 

@@ -11,6 +11,10 @@ namespace nb {
 
 // Source tile = one LDS stage = one block of BS threads.
 constexpr int BS = 256;
+// Tile quarters: the SB sources one wave stages.  Their boxes let a wave
+// classify a tile quarter by quarter.
+constexpr int SB = 64;
+constexpr int NQ = BS / SB;
 
 void set_error(const char *fmt, ...);
 void clear_error();
@@ -42,6 +46,7 @@ struct ForceArgs {
     const T *sx, *sy, *sz, *sm;  // sources
     int64_t ns;
     const double *tile_box;  // [ntiles][8]: lo xyz, hi xyz, (unused) of each BS-source tile
+    const double *sub_box;   // [ntiles][NQ][8]: the same for each tile's quarters, or nullptr
     int use_origin;          // 1: per-block FP64 origin (SPLIT), 0: absolute FP32 coordinates
     int nseg;                // source segments (grid = nblk_i * nseg)
     int64_t seg_len;         // sources per segment (multiple of BS)
@@ -63,7 +68,7 @@ constexpr int FAR_MFMA_BF16 = 4;  // bf16-split r^2 on the matrix cores, sum on 
 
 // Launchers (nb_force.hip)
 int launch_tile_boxes(const void *x, const void *y, const void *z, int is_double, int64_t n,
-                      double *box, hipStream_t s);
+                      double *box, double *sub_box, hipStream_t s);
 // acc = false (with pot) writes the potential alone.
 // far_path FAR_MFMA_BF16 (tpt 1, forces or potential alone) runs the FP32
 // state relative to a per-block FP64 origin; any other value is far path 0.

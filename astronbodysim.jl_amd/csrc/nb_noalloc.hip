@@ -120,7 +120,7 @@ extern "C" int nb_compute_force_noalloc(int32_t precision_mode, int64_t n, const
     if (f64) {
         NB_TRY(launch_force_f64(a, with_pot, s));
     } else {
-        NB_TRY(launch_tile_boxes(x, y, z, 1, n, box, s));
+        NB_TRY(launch_tile_boxes(x, y, z, 1, n, box, nullptr, s));
         NB_TRY(launch_force_split(a, 1, with_pot, s, FAR_VALU, true));
     }
     if (nseg > 1) NB_TRY(launch_finalize(part, nseg, n, G, ax, ay, az, pot, with_pot, s, true));

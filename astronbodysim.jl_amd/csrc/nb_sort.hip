@@ -121,7 +121,7 @@ int morton_reorder(nb_ctx *c) {
         S.tmp_bytes = bytes;
     }
     const int64_t ntiles = (n + BS - 1) / BS;
-    NB_TRY(launch_tile_boxes(c->x, c->y, c->z, c->is_double, n, c->tile_box, s));
+    NB_TRY(launch_tile_boxes(c->x, c->y, c->z, c->is_double, n, c->tile_box, c->sub_box, s));
     k_global_box<<<1, 256, 0, s>>>(c->tile_box, ntiles, S.box);
     NB_TRY(check_launch("k_global_box"));
     const unsigned g = (unsigned)((n + 255) / 256);
