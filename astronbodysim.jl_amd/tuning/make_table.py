@@ -2,9 +2,9 @@
 (tuning/mi355x.json): autotune one seeded system per size and precision mode
 and save what the autotuner measured.
 
-    python -m astronbodysim.jl_amd.tuning.make_table [out.json] [--modes=split,fp32,fp64]
+    python astronbodysim.jl_amd/tuning/make_table.py [out.json] [--modes=split,fp32,fp64]
 
-(or run this file directly).  Sizes 2^12 .. 2^20 in the split mode on
+(the package directory's name has a dot, so it does not run with -m).  Sizes 2^12 .. 2^20 in the split mode on
 two-Plummer collisions (the benchmark's geometry), 2^12 .. 2^18 in the fp32
 mode and 2^12 .. 2^15 in the fp64 mode on Plummer spheres.  With --modes= only those modes are
 measured again and the table's other entries are kept.  Needs the GPU; a
