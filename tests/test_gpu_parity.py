@@ -253,9 +253,9 @@ def test_collision_far_tiles_ragged(pkg, orc, far):
 
 @pytest.mark.parametrize("far", [0, 4])
 def test_segment_longer_than_one_class_group(pkg, orc, far):
-    """One source segment of 79 tiles: the kernel classifies tiles 64 at a
-    time (one lane each), so this covers a full group of 64 and a partial
-    second group of 15."""
+    """One source segment of 79 tiles: the kernel classifies tile quarters
+    16 tiles at a time (one lane each), so this covers four full groups of 16
+    and a partial fifth group of 15."""
     pos, vel, mass = pkg.ics.collision(10000, seeds=(7, 8))
     assert (len(mass) + 255) // 256 == 79
     soft = [0.002] * 6
@@ -323,14 +323,40 @@ def test_far_outlier_particle(pkg, orc, far):
     assert np.median(e) <= 2e-7 and np.percentile(e, 99) <= 2e-6 and e[-1] <= 1e-6
 
 
+@pytest.mark.parametrize("mode,far", [("fp32", 4), ("fp32", 0), ("split", 0), ("split", 4)])
+def test_tile_quarters_mixed_and_partial(pkg, orc, mode, far):
+    """Tiles whose 64-source quarters fall in different classes for a wave
+    (a softened Plummer core next to far shells), and a last source tile of
+    40 particles whose three other quarters hold padding only: against the
+    oracle (on the FP32-rounded inputs for the FP32 state), with three source
+    segments so that each segment ends in a partial class group."""
+    pos, vel, mass = pkg.ics.plummer(8232, seed=11)
+    assert len(mass) % 256 == 40
+    soft = [0.02] * 6
+    if mode == "fp32":
+        p32 = pos.astype(np.float32).astype(np.float64)
+        m32 = mass.astype(np.float32).astype(np.float64)
+        ref = orc.compute_force(p32, m32, soft)
+    else:
+        ref = orc.compute_force(pos, mass, soft)
+    with make_ctx(pkg, pos, vel, mass, soft, mode, tuning=(1, 3, 16, far)) as ctx:
+        ctx.compute_force()
+        acc = ctx.download(pos=False, vel=False)["acc"]
+    e = orc.rel_err(acc, ref)
+    print(f"{mode} far_path={far}: median {np.median(e):.3e} p99 {np.percentile(e, 99):.3e} max {e.max():.3e}")
+    assert np.all(np.isfinite(acc))
+    assert np.median(e) <= 2e-7 and np.percentile(e, 99) <= 2e-6 and e.max() <= 1e-4
+
+
 @pytest.mark.parametrize("geometry,far,nseg,sort", [
     ("plummer", 0, 1, 0), ("plummer", 4, 1, 0), ("plummer", 4, 5, 1), ("collision", 0, 4, 16),
     ("collision", 4, 4, 16), ("translated", 4, 4, 16)])
 def test_fp32_state_far_paths(pkg, orc, geometry, far, nseg, sort):
     """The FP32 drop-in mode (the reference's floattype = Float32 GPU runs)
     with far path 0 and with the bf16 matrix-core far tiles, which take the
-    FP32 positions relative to a per-block FP64 origin (hi/lo near tiles):
-    against the oracle on the FP32-rounded inputs."""
+    FP32 positions relative to a per-block FP64 origin (near tiles take the
+    FP32 positions themselves): against the oracle on the FP32-rounded
+    inputs."""
     if geometry == "plummer":
         pos, vel, mass = pkg.ics.plummer(4096, seed=1)
         soft = [0.01] * 6
