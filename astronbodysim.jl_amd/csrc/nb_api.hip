@@ -238,6 +238,7 @@ int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = tru
         a.sm = (const double *)c->m; a.ns = n;
         a.tile_box = c->tile_box;
         a.sub_box = c->sub_box;
+        a.tgeo = c->tgeo;
         a.use_origin = 1;
         a.nseg = g.nseg; a.seg_len = g.seg_len; a.nblk_i = g.nblk_i;
         a.G = c->G;
@@ -259,6 +260,7 @@ int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = tru
         a.sm = (const float *)c->m; a.ns = n;
         a.tile_box = c->tile_box;
         a.sub_box = c->sub_box;
+        a.tgeo = c->tgeo;
         a.use_origin = 0;
         a.nseg = g.nseg; a.seg_len = g.seg_len; a.nblk_i = g.nblk_i;
         a.G = c->G;
@@ -428,6 +430,7 @@ int nb_create(nb_ctx **out, int64_t n, int32_t precision_mode) {
             (rc = dalloc(&c->pot, sizeof(double) * nn)) ||
             (rc = dalloc(&c->tile_box, sizeof(double) * 8 * (size_t)((n + BS - 1) / BS + 1))) ||
             (rc = dalloc(&c->sub_box, sizeof(double) * 8 * NQ * (size_t)((n + BS - 1) / BS + 1))) ||
+            (rc = dalloc(&c->tgeo, sizeof(double) * TG_STRIDE * (size_t)((n + BS - 1) / BS + 1))) ||
             (rc = dalloc(&c->stage, sizeof(double) * 3 * nn)) ||
             (rc = dalloc(&c->wsrc, sizeof(float) * nn)) ||
             (rc = dalloc(&c->red, sizeof(double) * (ENERGY_BLOCKS * 8 + 8))) ||
@@ -477,7 +480,7 @@ int nb_destroy(nb_ctx *c) {
     dfree(c->m); dfree(c->h); dfree(c->coll);
     dfree(c->ax); dfree(c->ay); dfree(c->az); dfree(c->pot);
     dfree(c->perm);
-    dfree(c->tile_box); dfree(c->sub_box); dfree(c->part); dfree(c->stage); dfree(c->red); dfree(c->wsrc);
+    dfree(c->tile_box); dfree(c->sub_box); dfree(c->tgeo); dfree(c->part); dfree(c->stage); dfree(c->red); dfree(c->wsrc);
     dfree(c->time); dfree(c->flag); dfree(c->central);
     dfree(c->cx); dfree(c->cy); dfree(c->cz); dfree(c->cm); dfree(c->cidx);
     dfree(c->ti_end); dfree(c->ti_beg);

@@ -688,6 +688,33 @@ __device__ __forceinline__ double tile_gap2(const double *__restrict__ tb, const
     return gx * gx + gy * gy + gz * gz;
 }
 
+// A target block's geometry as force_block derives it (block_stats and
+// wave_geometry), stored once per force call when the block's targets meet
+// several source segments: the origin, then per wave lo xyz, hi xyz and the
+// three squared class distances (TG_STRIDE doubles per block).
+static_assert(TG_STRIDE >= 3 + 9 * NW, "target geometry record");
+
+__device__ __forceinline__ void store_target_geom(double *__restrict__ g, double ox, double oy, double oz,
+                                                  const WaveGeom &w, int wave, int lane) {
+    if (lane != 0) return;
+    if (wave == 0) { g[0] = ox; g[1] = oy; g[2] = oz; }
+    double *gw = g + 3 + 9 * wave;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) { gw[d] = w.lo[d]; gw[3 + d] = w.hi[d]; }
+    gw[6] = w.soft_d2; gw[7] = w.close_d2; gw[8] = w.farw_d2;
+}
+
+__device__ __forceinline__ WaveGeom load_target_geom(const double *__restrict__ g, int wave) {
+    const double *gw = g + 3 + 9 * wave;
+    WaveGeom w;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) { w.lo[d] = uniform_f64(gw[d]); w.hi[d] = uniform_f64(gw[3 + d]); }
+    w.soft_d2 = uniform_f64(gw[6]);
+    w.close_d2 = uniform_f64(gw[7]);
+    w.farw_d2 = uniform_f64(gw[8]);
+    return w;
+}
+
 // Target k (< TPT) of thread tid in a block whose targets start at i0: each
 // wave owns 64*TPT consecutive targets, so that a wave's targets (sorted in
 // Morton order) stay compact and its tile classes stay tight.
@@ -703,6 +730,8 @@ __device__ __forceinline__ int64_t target_of(int64_t i0, int k, int tid) {
 // each, [tile][NQ][8]).  With far path 0 (and 4 on FP32 state) a wave then
 // classifies each quarter on its own, so a tile that only touches the wave's
 // targets with one corner runs the cheaper arithmetic on its other quarters.
+// tgeo, when not null, holds this target block's origin and wave geometry
+// (TG_STRIDE doubles written by k_target_geom from the same targets).
 // Returns FP64 sums (without G) in Ax..Aq and the target validity.  With
 // ACC = false (and POT) only Aq is meaningful: the kernel writes the
 // potential alone, so the acceleration arithmetic is dead and removed.
@@ -712,7 +741,8 @@ __device__ __forceinline__ void force_block(
     const T *__restrict__ tz, const double *__restrict__ th, double th_const, int64_t i0,
     int64_t i_end, const T *__restrict__ sx, const T *__restrict__ sy, const T *__restrict__ sz,
     const T *__restrict__ sm, const float *__restrict__ sw, int64_t j0, int64_t j1, const double *__restrict__ tile_box,
-    const double *__restrict__ sub_box, const uint8_t *__restrict__ skip_src, bool use_origin, double (&Ax)[TPT], double (&Ay)[TPT],
+    const double *__restrict__ sub_box, const double *__restrict__ tgeo, const uint8_t *__restrict__ skip_src,
+    bool use_origin, double (&Ax)[TPT], double (&Ay)[TPT],
     double (&Az)[TPT], double (&Aq)[TPT], bool (&valid)[TPT]) {
     constexpr bool HILO = BlockShared<T, MF>::HILO;
     constexpr bool ABSNEAR = BlockShared<T, MF>::ABSNEAR;
@@ -737,10 +767,17 @@ __device__ __forceinline__ void force_block(
         pz[k] = valid[k] ? (double)tz[i] : 0.0;
         ph[k] = valid[k] ? (th ? th[i] : th_const) : 0.0;
     }
-    BlockStats st;
-    block_stats<TPT>(px, py, pz, ph, valid, use_origin, sh.red, st);
-    // block-uniform: kept in scalar registers
-    const double ox = uniform_f64(st.ox), oy = uniform_f64(st.oy), oz = uniform_f64(st.oz);
+    // The block's origin and its waves' geometry: read when stored for this
+    // target block (tgeo), else derived here (block-uniform, kept in scalar
+    // registers).
+    double ox, oy, oz;
+    if (tgeo) {
+        ox = uniform_f64(tgeo[0]); oy = uniform_f64(tgeo[1]); oz = uniform_f64(tgeo[2]);
+    } else {
+        BlockStats st;
+        block_stats<TPT>(px, py, pz, ph, valid, use_origin, sh.red, st);
+        ox = uniform_f64(st.ox); oy = uniform_f64(st.oy); oz = uniform_f64(st.oz);
+    }
 
     float xi[TPT], yi[TPT], zi[TPT], xil[TPT], yil[TPT], zil[TPT];
     float xa[TPT], ya[TPT], za[TPT];   // ABSNEAR: the FP32 positions themselves
@@ -772,7 +809,8 @@ __device__ __forceinline__ void force_block(
     }
     // Tiles are classified per wave (the branch is wave-uniform).
     // (ABSNEAR has no close class: every near tile takes the FP32 positions)
-    const WaveGeom wg = wave_geometry<TPT>(px, py, pz, ph, valid, ox, oy, oz, HILO && !ABSNEAR);
+    const WaveGeom wg = tgeo ? load_target_geom(tgeo, __builtin_amdgcn_readfirstlane(tid >> 6))
+                             : wave_geometry<TPT>(px, py, pz, ph, valid, ox, oy, oz, HILO && !ABSNEAR);
     const double soft_d2 = wg.soft_d2, close_d2 = wg.close_d2;
     const double close_d = sqrt(close_d2);
     // FAR_MFMA_BF16: lane l holds, for target group g, the K slice (l >> 4) of

@@ -71,6 +71,7 @@ struct nb_ctx {
     int64_t *perm = nullptr;      // internal -> caller index (nullptr: identity)
     double *tile_box = nullptr;
     double *sub_box = nullptr;    // [ntiles][NQ][8] boxes of the tile quarters
+    double *tgeo = nullptr;       // [target blocks][TG_STRIDE] geometry workspace of the force kernel
     double *part = nullptr;
     int part_cap = 0;             // segments the partial buffer holds
     double *stage = nullptr;      // 3n doubles

@@ -38,7 +38,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MF == 4 ? 4 
     bool valid[TPT];
     force_block<T, TPT, POT, MF, ACC>(sh, a.tx, a.ty, a.tz, a.th, a.th_const, i0, a.nt, a.sx, a.sy, a.sz,
                              a.sm, a.sw, j0, j1, a.tile_box + (j0 / BS) * 8,
-                             a.sub_box ? a.sub_box + (j0 / BS) * (NQ * 8) : nullptr, a.skip_src,
+                             a.sub_box ? a.sub_box + (j0 / BS) * (NQ * 8) : nullptr,
+                             a.tgeo ? a.tgeo + (int64_t)ib * TG_STRIDE : nullptr, a.skip_src,
                              a.use_origin != 0, Ax, Ay, Az, Aq, valid);
 #pragma unroll
     for (int k = 0; k < TPT; ++k) {
@@ -61,6 +62,33 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MF == 4 ? 4 
             if (POT) p[3 * a.nt + i] = Aq[k];
         }
     }
+}
+
+// The origin and wave geometry of each target block (force_block's first
+// steps), stored once per force call for launches whose target blocks meet
+// several source segments.  hilo: the force kernel's near tiles have a
+// close class (FP64 state).
+template <typename T, int TPT>
+__global__ __launch_bounds__(BS) void k_target_geom(ForceArgs<T> a, bool hilo) {
+    __shared__ double red[11][NW];
+    const int tid = threadIdx.x;
+    const int64_t i0 = (int64_t)blockIdx.x * (BS * TPT);
+    double px[TPT], py[TPT], pz[TPT], ph[TPT];
+    bool valid[TPT];
+#pragma unroll
+    for (int k = 0; k < TPT; ++k) {
+        const int64_t i = target_of<TPT>(i0, k, tid);
+        valid[k] = i < a.nt;
+        px[k] = valid[k] ? (double)a.tx[i] : 0.0;
+        py[k] = valid[k] ? (double)a.ty[i] : 0.0;
+        pz[k] = valid[k] ? (double)a.tz[i] : 0.0;
+        ph[k] = valid[k] ? (a.th ? a.th[i] : a.th_const) : 0.0;
+    }
+    BlockStats st;
+    block_stats<TPT>(px, py, pz, ph, valid, a.use_origin != 0, red, st);
+    const double ox = uniform_f64(st.ox), oy = uniform_f64(st.oy), oz = uniform_f64(st.oz);
+    const WaveGeom wg = wave_geometry<TPT>(px, py, pz, ph, valid, ox, oy, oz, hilo);
+    store_target_geom(a.tgeo + (int64_t)blockIdx.x * TG_STRIDE, ox, oy, oz, wg, tid >> 6, tid & 63);
 }
 
 // FP64-pair kernel (correctness mode, and the FP64 central-body terms):
@@ -205,14 +233,34 @@ int launch_tile_boxes(const void *x, const void *y, const void *z, int is_double
     return check_launch("k_tile_boxes");
 }
 
+// With several source segments, store each target block's geometry first
+// (a.tgeo); otherwise the force kernel derives it (tgeo = nullptr).
+template <typename T>
+static int prepare_target_geom(ForceArgs<T> &a, int tpt, hipStream_t s) {
+    if (!a.tgeo || a.nseg <= 1 || a.nblk_i <= 0) {
+        a.tgeo = nullptr;
+        return NB_OK;
+    }
+    const bool hilo = sizeof(T) == 8;
+    if (tpt == 1) k_target_geom<T, 1><<<(unsigned)a.nblk_i, BS, 0, s>>>(a, hilo);
+    else if (tpt == 2) k_target_geom<T, 2><<<(unsigned)a.nblk_i, BS, 0, s>>>(a, hilo);
+    else if (tpt == 4) k_target_geom<T, 4><<<(unsigned)a.nblk_i, BS, 0, s>>>(a, hilo);
+    else {
+        a.tgeo = nullptr;
+        return NB_OK;
+    }
+    return check_launch("k_target_geom");
+}
+
 template <typename T, int MF>
-static int launch_f32_impl(const ForceArgs<T> &a, int tpt, bool pot, hipStream_t s, bool acc) {
+static int launch_f32_impl(ForceArgs<T> a, int tpt, bool pot, hipStream_t s, bool acc) {
     const int64_t nb = (int64_t)a.nblk_i * a.nseg;
     if (nb <= 0) return NB_OK;
     if (nb > INT32_MAX) {
         set_error("force grid too large (%lld blocks)", (long long)nb);
         return NB_ERR_INVALID;
     }
+    NB_TRY(prepare_target_geom(a, tpt, s));
     dim3 g((unsigned)nb), b(BS);
 #define NB_L(TPT_, POT_) k_force_f32<T, TPT_, POT_, (POT_ ? 0 : MF)><<<g, b, 0, s>>>(a)
 #define NB_LP(TPT_) k_force_f32<T, TPT_, true, 0, false><<<g, b, 0, s>>>(a)
@@ -242,24 +290,28 @@ int launch_force_f32(const ForceArgs<float> &a, int tpt, bool pot, hipStream_t s
         }
         ForceArgs<float> b = a;
         b.use_origin = 1;
+        NB_TRY(prepare_target_geom(b, 1, s));
         if (pot) k_force_f32<float, 1, true, FAR_MFMA_BF16, false><<<(unsigned)nb, BS, 0, s>>>(b);
         else k_force_f32<float, 1, false, FAR_MFMA_BF16><<<(unsigned)nb, BS, 0, s>>>(b);
         return check_launch("k_force_f32<float, bf16 mfma>");
     }
     return launch_f32_impl<float, FAR_VALU>(a, tpt, pot, s, acc);
 }
-int launch_force_split(const ForceArgs<double> &a, int tpt, bool pot, hipStream_t s, int far_path,
+int launch_force_split(const ForceArgs<double> &a_in, int tpt, bool pot, hipStream_t s, int far_path,
                        bool acc) {
     if (!acc && !pot) acc = true;
+    ForceArgs<double> a = a_in;
     if (far_path == FAR_MFMA_BF16 && tpt == 1 && pot && !acc) {
         const int64_t nb = (int64_t)a.nblk_i * a.nseg;
         if (nb <= 0) return NB_OK;
+        NB_TRY(prepare_target_geom(a, tpt, s));
         k_force_f32<double, 1, true, FAR_MFMA_BF16, false><<<(unsigned)nb, BS, 0, s>>>(a);
         return check_launch("k_force_f32<bf16 mfma, potential>");
     }
     if ((far_path == FAR_MFMA_SUM || far_path == FAR_MFMA_R2) && tpt == 1 && !pot) {
         const int64_t nb = (int64_t)a.nblk_i * a.nseg;
         if (nb <= 0) return NB_OK;
+        NB_TRY(prepare_target_geom(a, tpt, s));
         if (far_path == FAR_MFMA_R2) k_force_f32<double, 1, false, FAR_MFMA_R2><<<(unsigned)nb, BS, 0, s>>>(a);
         else k_force_f32<double, 1, false, FAR_MFMA_SUM><<<(unsigned)nb, BS, 0, s>>>(a);
         return check_launch("k_force_f32<mfma>");
@@ -267,6 +319,7 @@ int launch_force_split(const ForceArgs<double> &a, int tpt, bool pot, hipStream_
     if (far_path == FAR_MFMA_BF16 && tpt == 1 && !pot) {
         const int64_t nb = (int64_t)a.nblk_i * a.nseg;
         if (nb <= 0) return NB_OK;
+        NB_TRY(prepare_target_geom(a, tpt, s));
         k_force_f32<double, 1, false, FAR_MFMA_BF16><<<(unsigned)nb, BS, 0, s>>>(a);
         return check_launch("k_force_f32<bf16 mfma>");
     }

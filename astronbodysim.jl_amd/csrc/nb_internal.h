@@ -15,6 +15,9 @@ constexpr int BS = 256;
 // classify a tile quarter by quarter.
 constexpr int SB = 64;
 constexpr int NQ = BS / SB;
+// Doubles per target block of the force kernel's geometry workspace (the
+// block origin, then nine per wave).
+constexpr int TG_STRIDE = 3 + 9 * (BS / 64) + 1;
 
 void set_error(const char *fmt, ...);
 void clear_error();
@@ -47,6 +50,7 @@ struct ForceArgs {
     int64_t ns;
     const double *tile_box;  // [ntiles][8]: lo xyz, hi xyz, (unused) of each BS-source tile
     const double *sub_box;   // [ntiles][NQ][8]: the same for each tile's quarters, or nullptr
+    double *tgeo;            // [nblk_i][TG_STRIDE] workspace for the target blocks' geometry, or nullptr
     int use_origin;          // 1: per-block FP64 origin (SPLIT), 0: absolute FP32 coordinates
     int nseg;                // source segments (grid = nblk_i * nseg)
     int64_t seg_len;         // sources per segment (multiple of BS)
