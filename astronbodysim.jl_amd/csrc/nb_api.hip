@@ -109,6 +109,17 @@ struct Geometry {
     int64_t seg_len;
 };
 
+// The bf16 B-operand workspace of the matrix-core far path (16 dwords per
+// target), allocated when a launch with several source segments first needs
+// it.
+int ensure_tbg(nb_ctx *c, const Geometry &g) {
+    if (c->tbg || g.nseg <= 1 || g.tpt != 1 || c->tun.far_path != FAR_MFMA_BF16 ||
+        c->mode == NB_MODE_FP64)
+        return NB_OK;
+    drop_graph(c);
+    return dalloc(&c->tbg, sizeof(unsigned) * 16 * (size_t)BS * (size_t)((c->n + BS - 1) / BS + 1));
+}
+
 Geometry geometry(int64_t nt, int64_t ns, int tpt, int want_seg) {
     Geometry g;
     g.tpt = tpt;
@@ -228,6 +239,7 @@ int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = tru
     const int tpt = f64 ? 1 : c->tun.targets_per_thread;
     Geometry g = geometry(n, n, tpt, c->tun.j_segments);
     NB_TRY(ensure_part(c, g.nseg));
+    NB_TRY(ensure_tbg(c, g));
     hipEvent_t e0 = nullptr;
     if (allow_timing) NB_TRY(record_start(c, &e0));
     if (c->is_double) {
@@ -239,6 +251,7 @@ int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = tru
         a.tile_box = c->tile_box;
         a.sub_box = c->sub_box;
         a.tgeo = c->tgeo;
+        a.tbg = c->tbg;
         a.use_origin = 1;
         a.nseg = g.nseg; a.seg_len = g.seg_len; a.nblk_i = g.nblk_i;
         a.G = c->G;
@@ -261,6 +274,7 @@ int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = tru
         a.tile_box = c->tile_box;
         a.sub_box = c->sub_box;
         a.tgeo = c->tgeo;
+        a.tbg = c->tbg;
         a.use_origin = 0;
         a.nseg = g.nseg; a.seg_len = g.seg_len; a.nblk_i = g.nblk_i;
         a.G = c->G;
@@ -480,7 +494,7 @@ int nb_destroy(nb_ctx *c) {
     dfree(c->m); dfree(c->h); dfree(c->coll);
     dfree(c->ax); dfree(c->ay); dfree(c->az); dfree(c->pot);
     dfree(c->perm);
-    dfree(c->tile_box); dfree(c->sub_box); dfree(c->tgeo); dfree(c->part); dfree(c->stage); dfree(c->red); dfree(c->wsrc);
+    dfree(c->tile_box); dfree(c->sub_box); dfree(c->tgeo); dfree(c->tbg); dfree(c->part); dfree(c->stage); dfree(c->red); dfree(c->wsrc);
     dfree(c->time); dfree(c->flag); dfree(c->central);
     dfree(c->cx); dfree(c->cy); dfree(c->cz); dfree(c->cm); dfree(c->cidx);
     dfree(c->ti_end); dfree(c->ti_beg);
@@ -794,6 +808,7 @@ int nb_kdk_steps(nb_ctx *c, int64_t nsteps, double dt, double t_begin, int32_t u
             Geometry g = geometry(c->n, c->n, c->mode == NB_MODE_FP64 ? 1 : c->tun.targets_per_thread,
                                   c->tun.j_segments);
             NB_TRY(ensure_part(c, g.nseg));
+            NB_TRY(ensure_tbg(c, g));
             NB_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             int rc = NB_OK;
             for (int64_t q = 0; q < chunk && rc == NB_OK; ++q)

@@ -138,7 +138,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MF == 4 ? 4 
         // thread before thread 0 overwrites it in the next iteration.
         force_block<double, TPT, POT, MF, ACC>(sh, a.x, a.y, a.z, a.h, 0.0, i0, o1, a.x, a.y, a.z, a.m, a.w, o0,
                                       o1, a.tile_box + a.tile_base[s] * 8,
-                                      a.sub_box + a.tile_base[s] * (NQ * 8), nullptr, a.skip_src, true, Ax,
+                                      a.sub_box + a.tile_base[s] * (NQ * 8), nullptr, nullptr, a.skip_src, true,
+                                      Ax,
                                       Ay, Az, Aq, valid);
         const double G = a.Gs[s];
 #pragma unroll

@@ -484,6 +484,26 @@ __device__ __forceinline__ nb_bf8 shfl_bf8(nb_bf8 v, int src) {
     return __builtin_bit_cast(nb_bf8, u);
 }
 
+// The B operands of lane `lane` for the four 16-target groups of its wave
+// (FAR_MFMA_BF16): lane l holds, for group g, the K slice (l >> 4) of the B
+// column of target 16g + (l & 15).  (dx, dy, dz): the lane's target
+// relative to the block origin.  Evaluated without FMA contraction, so that
+// the force kernel and k_target_geom produce the same operands.
+__device__ __forceinline__ void target_b_operands(double dx, double dy, double dz, int lane,
+                                                  nb_bf8 (&Bg)[4]) {
+#pragma clang fp contract(off)
+    nb_bf8 mine[4];
+    bf16_target_col((float)dx, (float)dy, (float)dz, (float)(dx * dx + dy * dy + dz * dz), mine);
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+#pragma unroll
+        for (int sl = 0; sl < 4; ++sl) {
+            const nb_bf8 t = shfl_bf8(mine[sl], 16 * g + (lane & 15));
+            if (sl == (lane >> 4)) Bg[g] = t;
+        }
+    }
+}
+
 // Farthest tiles with FAR_MFMA_BF16.  Chunk of 16 sources jc..jc+15: MFMA g gives
 // lane l the w r^2 of targets 16g + (l & 15) (the B columns) against sources
 // jc + 4(l >> 4) + v in element v (the A rows); the lane then accumulates
@@ -731,7 +751,8 @@ __device__ __forceinline__ int64_t target_of(int64_t i0, int k, int tid) {
 // classifies each quarter on its own, so a tile that only touches the wave's
 // targets with one corner runs the cheaper arithmetic on its other quarters.
 // tgeo, when not null, holds this target block's origin and wave geometry
-// (TG_STRIDE doubles written by k_target_geom from the same targets).
+// (TG_STRIDE doubles written by k_target_geom from the same targets), and
+// tbg, when not null, its lanes' bf16 B operands (16 dwords per lane).
 // Returns FP64 sums (without G) in Ax..Aq and the target validity.  With
 // ACC = false (and POT) only Aq is meaningful: the kernel writes the
 // potential alone, so the acceleration arithmetic is dead and removed.
@@ -741,8 +762,8 @@ __device__ __forceinline__ void force_block(
     const T *__restrict__ tz, const double *__restrict__ th, double th_const, int64_t i0,
     int64_t i_end, const T *__restrict__ sx, const T *__restrict__ sy, const T *__restrict__ sz,
     const T *__restrict__ sm, const float *__restrict__ sw, int64_t j0, int64_t j1, const double *__restrict__ tile_box,
-    const double *__restrict__ sub_box, const double *__restrict__ tgeo, const uint8_t *__restrict__ skip_src,
-    bool use_origin, double (&Ax)[TPT], double (&Ay)[TPT],
+    const double *__restrict__ sub_box, const double *__restrict__ tgeo, const unsigned *__restrict__ tbg,
+    const uint8_t *__restrict__ skip_src, bool use_origin, double (&Ax)[TPT], double (&Ay)[TPT],
     double (&Az)[TPT], double (&Aq)[TPT], bool (&valid)[TPT]) {
     constexpr bool HILO = BlockShared<T, MF>::HILO;
     constexpr bool ABSNEAR = BlockShared<T, MF>::ABSNEAR;
@@ -826,17 +847,12 @@ __device__ __forceinline__ void force_block(
     const bool by_quarter = QUARTERS && sub_box != nullptr;
     uint64_t m_soft = 0, m_close = 0, m_far = 0;
     if constexpr (ROWS) {
-        const int lane = tid & 63;
-        const double dx = px[0] - ox, dy = py[0] - oy, dz = pz[0] - oz;
-        nb_bf8 mine[4];
-        bf16_target_col(xi[0], yi[0], zi[0], (float)(dx * dx + dy * dy + dz * dz), mine);
+        if (tbg) {   // stored for this target block with its geometry
+            const nb_u4v *b = reinterpret_cast<const nb_u4v *>(tbg) + (int64_t)tid * 4;
 #pragma unroll
-        for (int g = 0; g < 4; ++g) {
-#pragma unroll
-            for (int sl = 0; sl < 4; ++sl) {
-                const nb_bf8 t = shfl_bf8(mine[sl], 16 * g + (lane & 15));
-                if (sl == (lane >> 4)) Bg[g] = t;
-            }
+            for (int g = 0; g < 4; ++g) Bg[g] = __builtin_bit_cast(nb_bf8, b[g]);
+        } else {
+            target_b_operands(px[0] - ox, py[0] - oy, pz[0] - oz, tid & 63, Bg);
         }
     }
     // weighted far tiles: per-target operands and S-form sums

@@ -72,6 +72,7 @@ struct nb_ctx {
     double *tile_box = nullptr;
     double *sub_box = nullptr;    // [ntiles][NQ][8] boxes of the tile quarters
     double *tgeo = nullptr;       // [target blocks][TG_STRIDE] geometry workspace of the force kernel
+    unsigned *tbg = nullptr;      // [target blocks][BS][16] bf16 B operands (far path 4), on first need
     double *part = nullptr;
     int part_cap = 0;             // segments the partial buffer holds
     double *stage = nullptr;      // 3n doubles

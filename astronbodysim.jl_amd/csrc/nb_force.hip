@@ -39,7 +39,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MF == 4 ? 4 
     force_block<T, TPT, POT, MF, ACC>(sh, a.tx, a.ty, a.tz, a.th, a.th_const, i0, a.nt, a.sx, a.sy, a.sz,
                              a.sm, a.sw, j0, j1, a.tile_box + (j0 / BS) * 8,
                              a.sub_box ? a.sub_box + (j0 / BS) * (NQ * 8) : nullptr,
-                             a.tgeo ? a.tgeo + (int64_t)ib * TG_STRIDE : nullptr, a.skip_src,
+                             a.tgeo ? a.tgeo + (int64_t)ib * TG_STRIDE : nullptr,
+                             a.tbg ? a.tbg + (int64_t)ib * BS * 16 : nullptr, a.skip_src,
                              a.use_origin != 0, Ax, Ay, Az, Aq, valid);
 #pragma unroll
     for (int k = 0; k < TPT; ++k) {
@@ -68,7 +69,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MF == 4 ? 4 
 // steps), stored once per force call for launches whose target blocks meet
 // several source segments.  hilo: the force kernel's near tiles have a
 // close class (FP64 state).
-template <typename T, int TPT>
+template <typename T, int TPT, bool ROWS>
 __global__ __launch_bounds__(BS) void k_target_geom(ForceArgs<T> a, bool hilo) {
     __shared__ double red[11][NW];
     const int tid = threadIdx.x;
@@ -89,6 +90,13 @@ __global__ __launch_bounds__(BS) void k_target_geom(ForceArgs<T> a, bool hilo) {
     const double ox = uniform_f64(st.ox), oy = uniform_f64(st.oy), oz = uniform_f64(st.oz);
     const WaveGeom wg = wave_geometry<TPT>(px, py, pz, ph, valid, ox, oy, oz, hilo);
     store_target_geom(a.tgeo + (int64_t)blockIdx.x * TG_STRIDE, ox, oy, oz, wg, tid >> 6, tid & 63);
+    if constexpr (ROWS) {   // the lanes' bf16 B operands (TPT 1)
+        nb_bf8 Bg[4];
+        target_b_operands(px[0] - ox, py[0] - oy, pz[0] - oz, tid & 63, Bg);
+        nb_u4v *b = reinterpret_cast<nb_u4v *>(a.tbg + ((int64_t)blockIdx.x * BS + tid) * 16);
+#pragma unroll
+        for (int g = 0; g < 4; ++g) b[g] = __builtin_bit_cast(nb_u4v, Bg[g]);
+    }
 }
 
 // FP64-pair kernel (correctness mode, and the FP64 central-body terms):
@@ -235,18 +243,24 @@ int launch_tile_boxes(const void *x, const void *y, const void *z, int is_double
 
 // With several source segments, store each target block's geometry first
 // (a.tgeo); otherwise the force kernel derives it (tgeo = nullptr).
+// rows: the force kernel takes bf16 matrix-core far tiles (TPT 1), whose
+// B operands are stored as well (a.tbg).
 template <typename T>
-static int prepare_target_geom(ForceArgs<T> &a, int tpt, hipStream_t s) {
+static int prepare_target_geom(ForceArgs<T> &a, int tpt, hipStream_t s, bool rows = false) {
     if (!a.tgeo || a.nseg <= 1 || a.nblk_i <= 0) {
         a.tgeo = nullptr;
+        a.tbg = nullptr;
         return NB_OK;
     }
     const bool hilo = sizeof(T) == 8;
-    if (tpt == 1) k_target_geom<T, 1><<<(unsigned)a.nblk_i, BS, 0, s>>>(a, hilo);
-    else if (tpt == 2) k_target_geom<T, 2><<<(unsigned)a.nblk_i, BS, 0, s>>>(a, hilo);
-    else if (tpt == 4) k_target_geom<T, 4><<<(unsigned)a.nblk_i, BS, 0, s>>>(a, hilo);
+    if (!rows || tpt != 1 || !a.tbg) a.tbg = nullptr;
+    if (tpt == 1 && a.tbg) k_target_geom<T, 1, true><<<(unsigned)a.nblk_i, BS, 0, s>>>(a, hilo);
+    else if (tpt == 1) k_target_geom<T, 1, false><<<(unsigned)a.nblk_i, BS, 0, s>>>(a, hilo);
+    else if (tpt == 2) k_target_geom<T, 2, false><<<(unsigned)a.nblk_i, BS, 0, s>>>(a, hilo);
+    else if (tpt == 4) k_target_geom<T, 4, false><<<(unsigned)a.nblk_i, BS, 0, s>>>(a, hilo);
     else {
         a.tgeo = nullptr;
+        a.tbg = nullptr;
         return NB_OK;
     }
     return check_launch("k_target_geom");
@@ -290,7 +304,7 @@ int launch_force_f32(const ForceArgs<float> &a, int tpt, bool pot, hipStream_t s
         }
         ForceArgs<float> b = a;
         b.use_origin = 1;
-        NB_TRY(prepare_target_geom(b, 1, s));
+        NB_TRY(prepare_target_geom(b, 1, s, true));
         if (pot) k_force_f32<float, 1, true, FAR_MFMA_BF16, false><<<(unsigned)nb, BS, 0, s>>>(b);
         else k_force_f32<float, 1, false, FAR_MFMA_BF16><<<(unsigned)nb, BS, 0, s>>>(b);
         return check_launch("k_force_f32<float, bf16 mfma>");
@@ -304,7 +318,7 @@ int launch_force_split(const ForceArgs<double> &a_in, int tpt, bool pot, hipStre
     if (far_path == FAR_MFMA_BF16 && tpt == 1 && pot && !acc) {
         const int64_t nb = (int64_t)a.nblk_i * a.nseg;
         if (nb <= 0) return NB_OK;
-        NB_TRY(prepare_target_geom(a, tpt, s));
+        NB_TRY(prepare_target_geom(a, tpt, s, true));
         k_force_f32<double, 1, true, FAR_MFMA_BF16, false><<<(unsigned)nb, BS, 0, s>>>(a);
         return check_launch("k_force_f32<bf16 mfma, potential>");
     }
@@ -319,7 +333,7 @@ int launch_force_split(const ForceArgs<double> &a_in, int tpt, bool pot, hipStre
     if (far_path == FAR_MFMA_BF16 && tpt == 1 && !pot) {
         const int64_t nb = (int64_t)a.nblk_i * a.nseg;
         if (nb <= 0) return NB_OK;
-        NB_TRY(prepare_target_geom(a, tpt, s));
+        NB_TRY(prepare_target_geom(a, tpt, s, true));
         k_force_f32<double, 1, false, FAR_MFMA_BF16><<<(unsigned)nb, BS, 0, s>>>(a);
         return check_launch("k_force_f32<bf16 mfma>");
     }

@@ -51,6 +51,7 @@ struct ForceArgs {
     const double *tile_box;  // [ntiles][8]: lo xyz, hi xyz, (unused) of each BS-source tile
     const double *sub_box;   // [ntiles][NQ][8]: the same for each tile's quarters, or nullptr
     double *tgeo;            // [nblk_i][TG_STRIDE] workspace for the target blocks' geometry, or nullptr
+    unsigned *tbg;           // [nblk_i][BS][16] workspace for their bf16 B operands, or nullptr
     int use_origin;          // 1: per-block FP64 origin (SPLIT), 0: absolute FP32 coordinates
     int nseg;                // source segments (grid = nblk_i * nseg)
     int64_t seg_len;         // sources per segment (multiple of BS)
