@@ -106,6 +106,42 @@ __device__ void block_stats(const double (&px)[TPT], const double (&py)[TPT], co
 // placed far away, so both tile paths can run a fixed trip count.
 constexpr float PAD_COORD = 1.0e18f;
 
+// Softening-checked pairs, GR sources at a time: the Newtonian terms of all
+// of them first, then one wave-level test for a pair inside some target's h
+// (rare even in these tiles), then the sums in source order.  One branch per
+// group instead of one per pair keeps the group's arithmetic in one block for
+// the scheduler; the results are those of pair_f32 / pair_f32_d pair by pair.
+template <int TPT>
+struct CheckedGroup {
+    static constexpr int GR = TPT >= 4 ? 1 : 4 / TPT;
+};
+
+template <int TPT, bool POT, int GR>
+__device__ __forceinline__ void checked_group_sums(const float (&dx)[GR][TPT], const float (&dy)[GR][TPT],
+                                                   const float (&dz)[GR][TPT], const float (&r2)[GR][TPT],
+                                                   float (&fac)[GR][TPT], float (&q)[GR][TPT],
+                                                   const float (&m)[GR], bool any, const float (&h2c)[TPT],
+                                                   const float (&h2)[TPT], const float (&hinv)[TPT],
+                                                   const float (&hinv3)[TPT], float (&ax)[TPT], float (&ay)[TPT],
+                                                   float (&az)[TPT], float (&aq)[TPT]) {
+    if (__builtin_expect(any, 0)) {
+#pragma unroll
+        for (int g = 0; g < GR; ++g)
+#pragma unroll
+            for (int k = 0; k < TPT; ++k)
+                if (r2[g][k] < h2c[k]) spline_f32(r2[g][k], m[g], h2[k], hinv[k], hinv3[k], fac[g][k], q[g][k]);
+    }
+#pragma unroll
+    for (int g = 0; g < GR; ++g)
+#pragma unroll
+        for (int k = 0; k < TPT; ++k) {
+            ax[k] = __builtin_fmaf(fac[g][k], dx[g][k], ax[k]);
+            ay[k] = __builtin_fmaf(fac[g][k], dy[g][k], ay[k]);
+            az[k] = __builtin_fmaf(fac[g][k], dz[g][k], az[k]);
+            if (POT) aq[k] += q[g][k];
+        }
+}
+
 // NS sources from s: a whole tile (BS) or one 64-source quarter of it.
 template <int TPT, bool CHECKED, bool POT, int NS = BS>
 __device__ __forceinline__ void tile_f32(const float4 *__restrict__ s, const float (&xi)[TPT],
@@ -114,6 +150,32 @@ __device__ __forceinline__ void tile_f32(const float4 *__restrict__ s, const flo
                                          const float (&hinv)[TPT], const float (&hinv3)[TPT],
                                          float (&ax)[TPT], float (&ay)[TPT], float (&az)[TPT],
                                          float (&aq)[TPT]) {
+    if constexpr (CHECKED && CheckedGroup<TPT>::GR > 1) {
+        constexpr int GR = CheckedGroup<TPT>::GR;
+#pragma unroll 1
+        for (int jj = 0; jj < NS; jj += GR) {
+            float dx[GR][TPT], dy[GR][TPT], dz[GR][TPT], r2[GR][TPT], fac[GR][TPT], q[GR][TPT], m[GR];
+            bool any = false;
+#pragma unroll
+            for (int g = 0; g < GR; ++g) {
+                const float4 v = s[jj + g];
+                m[g] = v.w;
+#pragma unroll
+                for (int k = 0; k < TPT; ++k) {
+                    dx[g][k] = v.x - xi[k];
+                    dy[g][k] = v.y - yi[k];
+                    dz[g][k] = v.z - zi[k];
+                    r2[g][k] = __builtin_fmaf(dx[g][k], dx[g][k], __builtin_fmaf(dy[g][k], dy[g][k], dz[g][k] * dz[g][k]));
+                    const float ri = __builtin_amdgcn_rsqf(r2[g][k]);
+                    q[g][k] = v.w * ri;
+                    fac[g][k] = q[g][k] * (ri * ri);
+                    any = any || r2[g][k] < h2c[k];
+                }
+            }
+            checked_group_sums<TPT, POT, GR>(dx, dy, dz, r2, fac, q, m, any, h2c, h2, hinv, hinv3, ax, ay, az, aq);
+        }
+        return;
+    }
 #pragma unroll 4
     for (int jj = 0; jj < NS; ++jj) {
         const float4 v = s[jj];
@@ -160,6 +222,32 @@ __device__ __forceinline__ void tile_f32_d(const SrcF64 *__restrict__ s, const d
                                            const float (&hinv)[TPT], const float (&hinv3)[TPT],
                                            float (&ax)[TPT], float (&ay)[TPT], float (&az)[TPT],
                                            float (&aq)[TPT]) {
+    if constexpr (CHECKED && CheckedGroup<TPT>::GR > 1) {
+        constexpr int GR = CheckedGroup<TPT>::GR;
+#pragma unroll 1
+        for (int jj = 0; jj < NS; jj += GR) {
+            float dx[GR][TPT], dy[GR][TPT], dz[GR][TPT], r2[GR][TPT], fac[GR][TPT], q[GR][TPT], m[GR];
+            bool any = false;
+#pragma unroll
+            for (int g = 0; g < GR; ++g) {
+                const SrcF64 v = s[jj + g];
+                m[g] = v.m;
+#pragma unroll
+                for (int k = 0; k < TPT; ++k) {
+                    dx[g][k] = (float)(v.x - xi[k]);
+                    dy[g][k] = (float)(v.y - yi[k]);
+                    dz[g][k] = (float)(v.z - zi[k]);
+                    r2[g][k] = __builtin_fmaf(dx[g][k], dx[g][k], __builtin_fmaf(dy[g][k], dy[g][k], dz[g][k] * dz[g][k]));
+                    const float ri = __builtin_amdgcn_rsqf(r2[g][k]);
+                    q[g][k] = v.m * ri;
+                    fac[g][k] = q[g][k] * (ri * ri);
+                    any = any || r2[g][k] < h2c[k];
+                }
+            }
+            checked_group_sums<TPT, POT, GR>(dx, dy, dz, r2, fac, q, m, any, h2c, h2, hinv, hinv3, ax, ay, az, aq);
+        }
+        return;
+    }
 #pragma unroll 2
     for (int jj = 0; jj < NS; ++jj) {
         const SrcF64 v = s[jj];
