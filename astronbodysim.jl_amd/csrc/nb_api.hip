@@ -109,13 +109,22 @@ struct Geometry {
     int64_t seg_len;
 };
 
-// The bf16 B-operand workspace of the matrix-core far path (16 dwords per
-// target), allocated when a launch with several source segments first needs
-// it.
+// Workspaces of the matrix-core far path, allocated when a launch first needs
+// them: the flags of the split far/near passes (one byte per block) and, with
+// several source segments, the bf16 B operands (16 dwords per target).
 int ensure_tbg(nb_ctx *c, const Geometry &g) {
-    if (c->tbg || g.nseg <= 1 || g.tpt != 1 || c->tun.far_path != FAR_MFMA_BF16 ||
-        c->mode == NB_MODE_FP64)
-        return NB_OK;
+    if (g.tpt != 1 || c->tun.far_path != FAR_MFMA_BF16 || c->mode == NB_MODE_FP64) return NB_OK;
+    const size_t nf = (size_t)g.nblk_i * (size_t)g.nseg;
+    if (nf > c->nflag_cap) {
+        drop_graph(c);
+        dfree(c->nflag);
+        c->nflag = nullptr;
+        c->nflag_cap = 0;
+        NB_TRY(dalloc(&c->nflag, nf));
+        NB_HIP(hipMemsetAsync(c->nflag, 0, nf, c->stream));   // before any capture
+        c->nflag_cap = nf;
+    }
+    if (c->tbg || g.nseg <= 1) return NB_OK;
     drop_graph(c);
     return dalloc(&c->tbg, sizeof(unsigned) * 16 * (size_t)BS * (size_t)((c->n + BS - 1) / BS + 1));
 }
@@ -252,6 +261,7 @@ int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = tru
         a.sub_box = c->sub_box;
         a.tgeo = c->tgeo;
         a.tbg = c->tbg;
+        a.nflag = c->nflag_cap >= (size_t)g.nblk_i * (size_t)g.nseg ? c->nflag : nullptr;
         a.use_origin = 1;
         a.nseg = g.nseg; a.seg_len = g.seg_len; a.nblk_i = g.nblk_i;
         a.G = c->G;
@@ -275,6 +285,7 @@ int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = tru
         a.sub_box = c->sub_box;
         a.tgeo = c->tgeo;
         a.tbg = c->tbg;
+        a.nflag = c->nflag_cap >= (size_t)g.nblk_i * (size_t)g.nseg ? c->nflag : nullptr;
         a.use_origin = 0;
         a.nseg = g.nseg; a.seg_len = g.seg_len; a.nblk_i = g.nblk_i;
         a.G = c->G;
@@ -494,7 +505,7 @@ int nb_destroy(nb_ctx *c) {
     dfree(c->m); dfree(c->h); dfree(c->coll);
     dfree(c->ax); dfree(c->ay); dfree(c->az); dfree(c->pot);
     dfree(c->perm);
-    dfree(c->tile_box); dfree(c->sub_box); dfree(c->tgeo); dfree(c->tbg); dfree(c->part); dfree(c->stage); dfree(c->red); dfree(c->wsrc);
+    dfree(c->tile_box); dfree(c->sub_box); dfree(c->tgeo); dfree(c->tbg); dfree(c->nflag); dfree(c->part); dfree(c->stage); dfree(c->red); dfree(c->wsrc);
     dfree(c->time); dfree(c->flag); dfree(c->central);
     dfree(c->cx); dfree(c->cy); dfree(c->cz); dfree(c->cm); dfree(c->cidx);
     dfree(c->ti_end); dfree(c->ti_beg);

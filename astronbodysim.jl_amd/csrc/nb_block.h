@@ -333,7 +333,9 @@ constexpr float FARW_MASSLESS = 1.329228e36f;   // 2^120
 // positions relative to the origin (sjd) and form each separation with one
 // FP64 subtraction and a conversion per axis (pair_f32_d): 15 VALU per pair
 // instead of the hi/lo loop's 18, at the same accuracy; sjl is then unused.
-template <typename T, int MF = 0>
+// NOROWS: the near pass of a split bf16 launch, which stages no bf16 rows
+// (half the LDS, so more of its blocks fit on a CU).
+template <typename T, int MF = 0, bool NOROWS = false>
 struct BlockShared {
     static constexpr bool HILO = sizeof(T) == 8 || MF == FAR_MFMA_BF16;
     static constexpr bool ABSNEAR = sizeof(T) == 4 && MF == FAR_MFMA_BF16;
@@ -348,8 +350,9 @@ struct BlockShared {
     float4 sW[WFAR ? 2 : 1][WFAR ? BS : 1];          // (w x', w y', w z', w |x'|^2)
     float sw[WFAR ? 2 : 1][WFAR ? BS : 1];           // w = m^(-2/3)
     static constexpr bool ROWS = HILO && MF == 4;
-    nb_bf8 sA[1][ROWS ? BS : 1][4];                   // bf16 A rows (K = 32), single-buffered
+    nb_bf8 sA[1][ROWS && !NOROWS ? BS : 1][4];        // bf16 A rows (K = 32), single-buffered
     double red[11][NW];
+    unsigned need[2][NW];   // near pass: tiles of the next 16 that a wave needs
 };
 
 typedef float nb_f4v __attribute__((ext_vector_type(4)));
@@ -844,9 +847,14 @@ __device__ __forceinline__ int64_t target_of(int64_t i0, int k, int tid) {
 // Returns FP64 sums (without G) in Ax..Aq and the target validity.  With
 // ACC = false (and POT) only Aq is meaningful: the kernel writes the
 // potential alone, so the acceleration arithmetic is dead and removed.
-template <typename T, int TPT, bool POT, int MF = 0, bool ACC = true>
-__device__ __forceinline__ void force_block(
-    BlockShared<T, MF> &sh, const T *__restrict__ tx, const T *__restrict__ ty,
+// PASS splits the bf16 matrix-core launch (forces only) in two: PASS_FAR sums
+// the far tiles (quarters) only and returns whether the wave left near ones
+// out; PASS_NEAR sums the near tiles only.  Each pass's kernel then holds
+// only its own loops: the far loop runs faster without the near loops
+// beside it.
+template <typename T, int TPT, bool POT, int MF = 0, bool ACC = true, int PASS = PASS_ALL>
+__device__ __forceinline__ bool force_block(
+    BlockShared<T, MF, PASS == PASS_NEAR> &sh, const T *__restrict__ tx, const T *__restrict__ ty,
     const T *__restrict__ tz, const double *__restrict__ th, double th_const, int64_t i0,
     int64_t i_end, const T *__restrict__ sx, const T *__restrict__ sy, const T *__restrict__ sz,
     const T *__restrict__ sm, const float *__restrict__ sw, int64_t j0, int64_t j1, const double *__restrict__ tile_box,
@@ -862,7 +870,9 @@ __device__ __forceinline__ void force_block(
     static_assert(MF != FAR_MFMA_BF16 || (HILO && TPT == 1 && (!POT || !ACC)),
                   "bf16 matrix-core far tiles: SPLIT, 1 target/lane, acc or potential alone");
     static_assert(ACC || POT, "nothing to compute");
+    static_assert(PASS == PASS_ALL || (MF == FAR_MFMA_BF16 && ACC && !POT), "split passes: bf16 far path, forces");
     const int tid = threadIdx.x;
+    bool had_near = false;   // PASS_FAR: near tiles of this wave left to PASS_NEAR
     __syncthreads();  // the previous call's readers of sh are done
     double S[4] = {0.0, 0.0, 0.0, 0.0};
 
@@ -934,7 +944,7 @@ __device__ __forceinline__ void force_block(
     constexpr bool QUARTERS = MF == FAR_VALU || (MF == FAR_MFMA_BF16 && ABSNEAR);
     const bool by_quarter = QUARTERS && sub_box != nullptr;
     uint64_t m_soft = 0, m_close = 0, m_far = 0;
-    if constexpr (ROWS) {
+    if constexpr (ROWS && PASS != PASS_NEAR) {
         if (tbg) {   // stored for this target block with its geometry
             const nb_u4v *b = reinterpret_cast<const nb_u4v *>(tbg) + (int64_t)tid * 4;
 #pragma unroll
@@ -973,7 +983,7 @@ __device__ __forceinline__ void force_block(
     }
     // Register prefetch of the first tile (with the source weight when the
     // weighted far paths are compiled in).
-    constexpr bool WEIGHTS = WFAR || ROWS;
+    constexpr bool WEIGHTS = WFAR || (ROWS && PASS != PASS_NEAR);
     T rx = 0, ry = 0, rz = 0, rm = 0;
     float rw = 0.0f;
     bool rv = false;
@@ -986,10 +996,56 @@ __device__ __forceinline__ void force_block(
         }
     }
     int it = 0;
+    unsigned blk_need = 0;   // near pass: tiles of the current 16 that some wave needs
+    int nstaged = 0;         // near pass: tiles staged so far (their parity picks the buffer)
     for (int64_t base = j0; base < j1; base += BS, ++it) {
-        const int buf = it & 1;
+        int buf = it & 1;
+        if constexpr (PASS == PASS_NEAR) {
+            // classes first: a tile that is far for every wave is not staged
+            if ((it & 15) == 0) {
+                const int lane = tid & 63;
+                const int64_t t = it + (lane >> 2);
+                const int q = lane & 3;
+                bool c_soft = false, c_close = false, c_far = true;
+                if (j0 + t * BS + (by_quarter ? q * SB : 0) < j1) {
+                    const double g2 = tile_gap2(by_quarter ? sub_box + (t * NQ + q) * 8 : tile_box + t * 8, wg);
+                    c_soft = g2 <= soft_d2;
+                    c_close = !c_soft && g2 <= close_d2;
+                    c_far = g2 > wg.farw_d2;
+                }
+                m_soft = __builtin_amdgcn_ballot_w64(c_soft);
+                m_close = __builtin_amdgcn_ballot_w64(c_close);
+                m_far = __builtin_amdgcn_ballot_w64(c_far);
+            }
+            // one exchange per 16 tiles: bit t of a wave's word = tile it + t
+            // has a quarter that is not far for it (two slots, so a wave that
+            // runs ahead over skipped tiles never overwrites a word being read)
+            if ((it & 15) == 0) {
+                unsigned mine = 0;
+#pragma unroll
+                for (int t = 0; t < 16; ++t)
+                    mine |= (((unsigned)(m_far >> (4 * t)) & 15u) != 15u ? 1u : 0u) << t;
+                if ((tid & 63) == 0) sh.need[(it >> 4) & 1][tid >> 6] = mine;
+                __syncthreads();
+                unsigned all = 0;
+#pragma unroll
+                for (int w = 0; w < NW; ++w) all |= sh.need[(it >> 4) & 1][w];
+                blk_need = __builtin_amdgcn_readfirstlane(all);
+            }
+            if (!((blk_need >> (it & 15)) & 1u)) {
+                const int64_t j = base + BS + tid;   // keep the register prefetch one tile ahead
+                rv = j < j1 && !(skip_src && skip_src[j]);
+                if (rv) { rx = sx[j]; ry = sy[j]; rz = sz[j]; rm = sm[j]; }
+                continue;
+            }
+            // Staged tiles alternate buffers: a buffer is written again only
+            // after the next staged tile's barrier, which every wave reaches
+            // once done with the tile that used it (skipped tiles have none).
+            buf = nstaged & 1;
+            ++nstaged;
+        }
         // the single-buffered bf16 rows: the last tile's readers finish first
-        if constexpr (ROWS) if (it > 0) __syncthreads();
+        if constexpr (ROWS && PASS != PASS_NEAR) if (it > 0) __syncthreads();
         if (rv) {
             const double dx = (double)rx - ox, dy = (double)ry - oy, dz = (double)rz - oz;
             const float hx = (float)dx, hy = (float)dy, hz = (float)dz;
@@ -997,7 +1053,9 @@ __device__ __forceinline__ void force_block(
             // the lo record (ABSNEAR: of the hi record)
             const float cw = (ROWS && !ACC && rw > 0.0f) ? __builtin_amdgcn_rcpf(rw) : 0.0f;
             sh.sj[buf][tid] = make_float4(hx, hy, hz, (ABSNEAR && !ACC) ? cw : (float)rm);
-            if (ABSNEAR)
+            if (PASS == PASS_FAR) {
+                // the far tiles read sj and the rows only
+            } else if (ABSNEAR)
                 sh.sjl[HILO ? buf : 0][HILO ? tid : 0] = make_float4((float)rx, (float)ry, (float)rz, (float)rm);
             else if (D64NEAR)
                 sh.sjd[D64NEAR ? buf : 0][D64NEAR ? tid : 0] = SrcF64{dx, dy, dz, (float)rm, 0.0f};
@@ -1013,7 +1071,7 @@ __device__ __forceinline__ void force_block(
                 sh.sB[buf][3][tid] = (float)md;
             }
             if constexpr (MF == 2) sh.sQ[MF == 2 ? buf : 0][MF == 2 ? tid : 0] = (float)(dx * dx + dy * dy + dz * dz);
-            if constexpr (ROWS) {
+            if constexpr (ROWS && PASS != PASS_NEAR) {
                 if (rw > 0.0f) {
                     const double q2 = (double)hx * hx + (double)hy * hy + (double)hz * hz;
                     bf16_source_row(rw * hx, rw * hy, rw * hz, (float)((double)rw * q2), rw,
@@ -1039,7 +1097,8 @@ __device__ __forceinline__ void force_block(
             }
         } else {
             sh.sj[buf][tid] = make_float4(PAD_COORD, PAD_COORD, PAD_COORD, 0.0f);
-            if (ABSNEAR)
+            if (PASS == PASS_FAR) {
+            } else if (ABSNEAR)
                 sh.sjl[HILO ? buf : 0][HILO ? tid : 0] = make_float4(PAD_COORD, PAD_COORD, PAD_COORD, 0.0f);
             else if (D64NEAR)
                 sh.sjd[D64NEAR ? buf : 0][D64NEAR ? tid : 0] =
@@ -1054,7 +1113,7 @@ __device__ __forceinline__ void force_block(
             }
             // |PAD|^2: r^2 ~ 3e36 keeps r^-3 at zero (and finite) for padding
             if constexpr (MF == 2) sh.sQ[MF == 2 ? buf : 0][MF == 2 ? tid : 0] = 3.0f * PAD_COORD * PAD_COORD;
-            if constexpr (ROWS) bf16_massless_row(sh.sA[0][ROWS ? tid : 0]);
+            if constexpr (ROWS && PASS != PASS_NEAR) bf16_massless_row(sh.sA[0][ROWS ? tid : 0]);
             if constexpr (WFAR) {
                 sh.sW[WFAR ? buf : 0][WFAR ? tid : 0] = make_float4(0.0f, 0.0f, 0.0f, FARW_MASSLESS);
                 sh.sw[WFAR ? buf : 0][WFAR ? tid : 0] = 0.0f;
@@ -1072,7 +1131,7 @@ __device__ __forceinline__ void force_block(
         // Quarter classes of the next 16 tiles, one per lane, as wave-wide bit
         // masks (the per-tile test is then scalar).  A quarter past the end of
         // the sources holds padding only, which every path sums to zero.
-        if ((it & 15) == 0) {
+        if (PASS != PASS_NEAR && (it & 15) == 0) {
             const int lane = tid & 63;
             const int64_t t = it + (lane >> 2);
             const int q = lane & 3;
@@ -1093,11 +1152,33 @@ __device__ __forceinline__ void force_block(
         const unsigned q_far = (unsigned)(m_far >> tsh) & 15u;
         // far quarters with arithmetic of their own (on path 0 far is mid)
         const unsigned q_fown = ROWS ? q_far : 0u;
+        if constexpr (PASS == PASS_FAR) {
+            if (q_far != 15u) had_near = true;
+            if (q_far == 15u || (by_quarter && q_far != 0u)) {
+                float S3[4][4];
+                if (q_far == 15u) {
+                    tile_mf3(sh.sj[buf], sh.sA[0], tid & 63, Bg, S3);
+                } else {
+#pragma unroll
+                    for (int g = 0; g < 4; ++g)
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) S3[g][c] = 0.0f;
+#pragma unroll 1
+                    for (int q = 0; q < NQ; ++q)
+                        if (q_far & (1u << q)) quarter_mf3(sh.sj[buf], sh.sA[0], tid & 63, q, Bg, S3);
+                }
+                fold_mf3(S3, px[0] - ox, py[0] - oy, pz[0] - oz, Ax[0], Ay[0], Az[0]);
+            }
+            continue;
+        }
+        if constexpr (PASS == PASS_NEAR) {
+            if (q_far == 15u) continue;   // summed by PASS_FAR
+        }
         if (by_quarter && q_soft != 15u && q_close != 15u && q_fown != 15u && (q_soft | q_close | q_fown) != 0u) {
             // a tile of mixed quarters: its far quarters first (their sums
             // folded before the near quarters' FP32 sums start)
             if constexpr (QUARTERS) {
-                if constexpr (ROWS) {
+                if constexpr (ROWS && PASS != PASS_NEAR) {
                     if (q_fown != 0u) {
                         float S3[4][4], Sq[4];
 #pragma unroll
@@ -1175,7 +1256,7 @@ __device__ __forceinline__ void force_block(
         } else if (HILO && !ABSNEAR && q_close == 15u) {
             tile_f32_hilo<TPT, false, POT>(sh.sj[buf], sh.sjl[HILO ? buf : 0], xi, yi, zi, xil,
                                            yil, zil, h2c, h2, hinv, hinv3, ax, ay, az, aq);
-        } else if (ROWS && q_far == 15u) {
+        } else if (ROWS && PASS == PASS_ALL && q_far == 15u) {
             if constexpr (!ACC) {
                 float Sq[4];
                 tile_mf3_pot(ABSNEAR ? sh.sj[buf] : sh.sjl[HILO ? buf : 0], sh.sA[0], tid & 63, Bg, Sq);
@@ -1233,6 +1314,7 @@ __device__ __forceinline__ void force_block(
         Ay[0] += col[1] - yd * col[3];
         Az[0] += col[2] - zd * col[3];
     }
+    return had_near;
 }
 
 }  // namespace nb

@@ -24,11 +24,17 @@ namespace nb {
 // with the bf16 far tiles, the SPLIT layout of the FP32 positions).  MF
 // selects the matrix-core far tiles (TPT 1, forces or potentials alone).
 // ACC = false writes the potential only.
-template <typename T, int TPT, bool POT, int MF = 0, bool ACC = true>
+// PASS: every tile (PASS_ALL), or the two halves of a split bf16 launch:
+// PASS_FAR writes the far tiles' sums and flags the blocks that left near
+// tiles out (a.nflag, which holds zeros between launches), then PASS_NEAR
+// adds the near tiles' sums of the flagged blocks to the same outputs and
+// clears their flags.  (A stale flag would only add a zero sum.)
+template <typename T, int TPT, bool POT, int MF = 0, bool ACC = true, int PASS = PASS_ALL>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MF == 4 ? 4 : 3))) void k_force_f32(ForceArgs<T> a) {
-    __shared__ BlockShared<T, MF> sh;
+    __shared__ BlockShared<T, MF, PASS == PASS_NEAR> sh;
     const int tid = threadIdx.x;
     const int logical = xcd_remap(blockIdx.x, gridDim.x);
+    if (PASS == PASS_NEAR && !a.nflag[logical]) return;   // block-uniform
     const int seg = logical / a.nblk_i;
     const int ib = logical - seg * a.nblk_i;
     const int64_t i0 = (int64_t)ib * (BS * TPT);
@@ -36,17 +42,33 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MF == 4 ? 4 
     const int64_t j1 = (j0 + a.seg_len < a.ns) ? j0 + a.seg_len : a.ns;
     double Ax[TPT], Ay[TPT], Az[TPT], Aq[TPT];
     bool valid[TPT];
-    force_block<T, TPT, POT, MF, ACC>(sh, a.tx, a.ty, a.tz, a.th, a.th_const, i0, a.nt, a.sx, a.sy, a.sz,
+    const bool had_near = force_block<T, TPT, POT, MF, ACC, PASS>(sh, a.tx, a.ty, a.tz, a.th, a.th_const, i0, a.nt, a.sx, a.sy, a.sz,
                              a.sm, a.sw, j0, j1, a.tile_box + (j0 / BS) * 8,
                              a.sub_box ? a.sub_box + (j0 / BS) * (NQ * 8) : nullptr,
                              a.tgeo ? a.tgeo + (int64_t)ib * TG_STRIDE : nullptr,
                              a.tbg ? a.tbg + (int64_t)ib * BS * 16 : nullptr, a.skip_src,
                              a.use_origin != 0, Ax, Ay, Az, Aq, valid);
+    // (plain byte stores: any wave that left near tiles out sets the flag;
+    // a block-wide reduction here cost the far pass a few percent)
+    if (PASS == PASS_FAR && had_near && (tid & 63) == 0) a.nflag[logical] = 1;
+    // every wave has read the flag: force_block passed barriers since
+    if (PASS == PASS_NEAR && tid == 0) a.nflag[logical] = 0;
 #pragma unroll
     for (int k = 0; k < TPT; ++k) {
         if (!valid[k]) continue;
         const int64_t i = target_of<TPT>(i0, k, tid);
-        if (a.nseg == 1) {
+        if (PASS == PASS_NEAR) {   // forces only: add to the far pass's outputs
+            if (a.nseg == 1) {
+                a.ax[i] += a.G * Ax[k];
+                a.ay[i] += a.G * Ay[k];
+                a.az[i] += a.G * Az[k];
+            } else {
+                double *p = a.part + (int64_t)seg * 4 * a.nt;
+                p[i] += Ax[k];
+                p[a.nt + i] += Ay[k];
+                p[2 * a.nt + i] += Az[k];
+            }
+        } else if (a.nseg == 1) {
             if (ACC) {
                 a.ax[i] = a.G * Ax[k];
                 a.ay[i] = a.G * Ay[k];
@@ -62,6 +84,22 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MF == 4 ? 4 
             }
             if (POT) p[3 * a.nt + i] = Aq[k];
         }
+    }
+}
+
+// The bf16 matrix-core force launch (forces only, 1 target per lane): one
+// kernel, or the far and near passes when the caller provides the flags.
+// The split pays where near tiles are few (FP64 state: the collision at 2^20
+// has 1.8 % of its pairs in them, and its far loop alone runs about 10 %
+// faster); with FP32 state (the FP32 Plummer sphere at 2^16: 18 %) the near
+// pass cost more than it saved, so that launch stays in one kernel.
+template <typename T>
+static void launch_bf16_forces(const ForceArgs<T> &a, int64_t nb, hipStream_t s) {
+    if (a.nflag && sizeof(T) == 8) {
+        k_force_f32<T, 1, false, FAR_MFMA_BF16, true, PASS_FAR><<<(unsigned)nb, BS, 0, s>>>(a);
+        k_force_f32<T, 1, false, FAR_MFMA_BF16, true, PASS_NEAR><<<(unsigned)nb, BS, 0, s>>>(a);
+    } else {
+        k_force_f32<T, 1, false, FAR_MFMA_BF16><<<(unsigned)nb, BS, 0, s>>>(a);
     }
 }
 
@@ -306,7 +344,7 @@ int launch_force_f32(const ForceArgs<float> &a, int tpt, bool pot, hipStream_t s
         b.use_origin = 1;
         NB_TRY(prepare_target_geom(b, 1, s, true));
         if (pot) k_force_f32<float, 1, true, FAR_MFMA_BF16, false><<<(unsigned)nb, BS, 0, s>>>(b);
-        else k_force_f32<float, 1, false, FAR_MFMA_BF16><<<(unsigned)nb, BS, 0, s>>>(b);
+        else launch_bf16_forces(b, nb, s);
         return check_launch("k_force_f32<float, bf16 mfma>");
     }
     return launch_f32_impl<float, FAR_VALU>(a, tpt, pot, s, acc);
@@ -334,7 +372,7 @@ int launch_force_split(const ForceArgs<double> &a_in, int tpt, bool pot, hipStre
         const int64_t nb = (int64_t)a.nblk_i * a.nseg;
         if (nb <= 0) return NB_OK;
         NB_TRY(prepare_target_geom(a, tpt, s, true));
-        k_force_f32<double, 1, false, FAR_MFMA_BF16><<<(unsigned)nb, BS, 0, s>>>(a);
+        launch_bf16_forces(a, nb, s);
         return check_launch("k_force_f32<bf16 mfma>");
     }
     if (far_path == FAR_WEIGHTED) return launch_f32_impl<double, FAR_WEIGHTED>(a, tpt, pot, s, acc);

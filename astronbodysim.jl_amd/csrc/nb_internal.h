@@ -52,6 +52,7 @@ struct ForceArgs {
     const double *sub_box;   // [ntiles][NQ][8]: the same for each tile's quarters, or nullptr
     double *tgeo;            // [nblk_i][TG_STRIDE] workspace for the target blocks' geometry, or nullptr
     unsigned *tbg;           // [nblk_i][BS][16] workspace for their bf16 B operands, or nullptr
+    uint8_t *nflag;          // [nblk_i * nseg]: blocks with near tiles (split passes), or nullptr
     int use_origin;          // 1: per-block FP64 origin (SPLIT), 0: absolute FP32 coordinates
     int nseg;                // source segments (grid = nblk_i * nseg)
     int64_t seg_len;         // sources per segment (multiple of BS)
@@ -70,6 +71,12 @@ constexpr int FAR_MFMA_SUM = 1;   // sum on the matrix cores
 constexpr int FAR_MFMA_R2 = 2;    // r^2 and sum on the matrix cores (farthest tiles)
 constexpr int FAR_WEIGHTED = 3;   // weighted far tiles: 10 VALU + rsq per pair
 constexpr int FAR_MFMA_BF16 = 4;  // bf16-split r^2 on the matrix cores, sum on the VALU
+
+// Passes of the force kernel (force_block): every tile, the far tiles only,
+// or the near tiles only (the last two split the bf16 matrix-core launch).
+constexpr int PASS_ALL = 0;
+constexpr int PASS_FAR = 1;
+constexpr int PASS_NEAR = 2;
 
 // Launchers (nb_force.hip)
 int launch_tile_boxes(const void *x, const void *y, const void *z, int is_double, int64_t n,
