@@ -333,9 +333,7 @@ constexpr float FARW_MASSLESS = 1.329228e36f;   // 2^120
 // positions relative to the origin (sjd) and form each separation with one
 // FP64 subtraction and a conversion per axis (pair_f32_d): 15 VALU per pair
 // instead of the hi/lo loop's 18, at the same accuracy; sjl is then unused.
-// NOROWS: the near pass of a split bf16 launch, which stages no bf16 rows
-// (half the LDS, so more of its blocks fit on a CU).
-template <typename T, int MF = 0, bool NOROWS = false>
+template <typename T, int MF = 0>
 struct BlockShared {
     static constexpr bool HILO = sizeof(T) == 8 || MF == FAR_MFMA_BF16;
     static constexpr bool ABSNEAR = sizeof(T) == 4 && MF == FAR_MFMA_BF16;
@@ -350,9 +348,8 @@ struct BlockShared {
     float4 sW[WFAR ? 2 : 1][WFAR ? BS : 1];          // (w x', w y', w z', w |x'|^2)
     float sw[WFAR ? 2 : 1][WFAR ? BS : 1];           // w = m^(-2/3)
     static constexpr bool ROWS = HILO && MF == 4;
-    nb_bf8 sA[1][ROWS && !NOROWS ? BS : 1][4];        // bf16 A rows (K = 32), single-buffered
+    nb_bf8 sA[1][ROWS ? BS : 1][4];                   // bf16 A rows (K = 32), single-buffered
     double red[11][NW];
-    unsigned need[2][NW];   // near pass: tiles of the next 16 that a wave needs
 };
 
 typedef float nb_f4v __attribute__((ext_vector_type(4)));
@@ -847,14 +844,13 @@ __device__ __forceinline__ int64_t target_of(int64_t i0, int k, int tid) {
 // Returns FP64 sums (without G) in Ax..Aq and the target validity.  With
 // ACC = false (and POT) only Aq is meaningful: the kernel writes the
 // potential alone, so the acceleration arithmetic is dead and removed.
-// PASS splits the bf16 matrix-core launch (forces only) in two: PASS_FAR sums
-// the far tiles (quarters) only and returns whether the wave left near ones
-// out; PASS_NEAR sums the near tiles only.  Each pass's kernel then holds
-// only its own loops: the far loop runs faster without the near loops
-// beside it.
+// PASS_FAR (the bf16 matrix-core path, forces only) sums the far tiles
+// (quarters) only and returns whether the wave left near ones out, which a
+// separate pass (near_wave) adds: without the near loops beside it the far
+// loop runs faster.
 template <typename T, int TPT, bool POT, int MF = 0, bool ACC = true, int PASS = PASS_ALL>
 __device__ __forceinline__ bool force_block(
-    BlockShared<T, MF, PASS == PASS_NEAR> &sh, const T *__restrict__ tx, const T *__restrict__ ty,
+    BlockShared<T, MF> &sh, const T *__restrict__ tx, const T *__restrict__ ty,
     const T *__restrict__ tz, const double *__restrict__ th, double th_const, int64_t i0,
     int64_t i_end, const T *__restrict__ sx, const T *__restrict__ sy, const T *__restrict__ sz,
     const T *__restrict__ sm, const float *__restrict__ sw, int64_t j0, int64_t j1, const double *__restrict__ tile_box,
@@ -870,9 +866,10 @@ __device__ __forceinline__ bool force_block(
     static_assert(MF != FAR_MFMA_BF16 || (HILO && TPT == 1 && (!POT || !ACC)),
                   "bf16 matrix-core far tiles: SPLIT, 1 target/lane, acc or potential alone");
     static_assert(ACC || POT, "nothing to compute");
-    static_assert(PASS == PASS_ALL || (MF == FAR_MFMA_BF16 && ACC && !POT), "split passes: bf16 far path, forces");
+    static_assert(PASS == PASS_ALL || (PASS == PASS_FAR && MF == FAR_MFMA_BF16 && ACC && !POT),
+                  "far pass: bf16 far path, forces");
     const int tid = threadIdx.x;
-    bool had_near = false;   // PASS_FAR: near tiles of this wave left to PASS_NEAR
+    bool had_near = false;   // PASS_FAR: near tiles of this wave left to the near pass
     __syncthreads();  // the previous call's readers of sh are done
     double S[4] = {0.0, 0.0, 0.0, 0.0};
 
@@ -944,7 +941,7 @@ __device__ __forceinline__ bool force_block(
     constexpr bool QUARTERS = MF == FAR_VALU || (MF == FAR_MFMA_BF16 && ABSNEAR);
     const bool by_quarter = QUARTERS && sub_box != nullptr;
     uint64_t m_soft = 0, m_close = 0, m_far = 0;
-    if constexpr (ROWS && PASS != PASS_NEAR) {
+    if constexpr (ROWS) {
         if (tbg) {   // stored for this target block with its geometry
             const nb_u4v *b = reinterpret_cast<const nb_u4v *>(tbg) + (int64_t)tid * 4;
 #pragma unroll
@@ -983,7 +980,7 @@ __device__ __forceinline__ bool force_block(
     }
     // Register prefetch of the first tile (with the source weight when the
     // weighted far paths are compiled in).
-    constexpr bool WEIGHTS = WFAR || (ROWS && PASS != PASS_NEAR);
+    constexpr bool WEIGHTS = WFAR || ROWS;
     T rx = 0, ry = 0, rz = 0, rm = 0;
     float rw = 0.0f;
     bool rv = false;
@@ -996,56 +993,10 @@ __device__ __forceinline__ bool force_block(
         }
     }
     int it = 0;
-    unsigned blk_need = 0;   // near pass: tiles of the current 16 that some wave needs
-    int nstaged = 0;         // near pass: tiles staged so far (their parity picks the buffer)
     for (int64_t base = j0; base < j1; base += BS, ++it) {
-        int buf = it & 1;
-        if constexpr (PASS == PASS_NEAR) {
-            // classes first: a tile that is far for every wave is not staged
-            if ((it & 15) == 0) {
-                const int lane = tid & 63;
-                const int64_t t = it + (lane >> 2);
-                const int q = lane & 3;
-                bool c_soft = false, c_close = false, c_far = true;
-                if (j0 + t * BS + (by_quarter ? q * SB : 0) < j1) {
-                    const double g2 = tile_gap2(by_quarter ? sub_box + (t * NQ + q) * 8 : tile_box + t * 8, wg);
-                    c_soft = g2 <= soft_d2;
-                    c_close = !c_soft && g2 <= close_d2;
-                    c_far = g2 > wg.farw_d2;
-                }
-                m_soft = __builtin_amdgcn_ballot_w64(c_soft);
-                m_close = __builtin_amdgcn_ballot_w64(c_close);
-                m_far = __builtin_amdgcn_ballot_w64(c_far);
-            }
-            // one exchange per 16 tiles: bit t of a wave's word = tile it + t
-            // has a quarter that is not far for it (two slots, so a wave that
-            // runs ahead over skipped tiles never overwrites a word being read)
-            if ((it & 15) == 0) {
-                unsigned mine = 0;
-#pragma unroll
-                for (int t = 0; t < 16; ++t)
-                    mine |= (((unsigned)(m_far >> (4 * t)) & 15u) != 15u ? 1u : 0u) << t;
-                if ((tid & 63) == 0) sh.need[(it >> 4) & 1][tid >> 6] = mine;
-                __syncthreads();
-                unsigned all = 0;
-#pragma unroll
-                for (int w = 0; w < NW; ++w) all |= sh.need[(it >> 4) & 1][w];
-                blk_need = __builtin_amdgcn_readfirstlane(all);
-            }
-            if (!((blk_need >> (it & 15)) & 1u)) {
-                const int64_t j = base + BS + tid;   // keep the register prefetch one tile ahead
-                rv = j < j1 && !(skip_src && skip_src[j]);
-                if (rv) { rx = sx[j]; ry = sy[j]; rz = sz[j]; rm = sm[j]; }
-                continue;
-            }
-            // Staged tiles alternate buffers: a buffer is written again only
-            // after the next staged tile's barrier, which every wave reaches
-            // once done with the tile that used it (skipped tiles have none).
-            buf = nstaged & 1;
-            ++nstaged;
-        }
+        const int buf = it & 1;
         // the single-buffered bf16 rows: the last tile's readers finish first
-        if constexpr (ROWS && PASS != PASS_NEAR) if (it > 0) __syncthreads();
+        if constexpr (ROWS) if (it > 0) __syncthreads();
         if (rv) {
             const double dx = (double)rx - ox, dy = (double)ry - oy, dz = (double)rz - oz;
             const float hx = (float)dx, hy = (float)dy, hz = (float)dz;
@@ -1071,7 +1022,7 @@ __device__ __forceinline__ bool force_block(
                 sh.sB[buf][3][tid] = (float)md;
             }
             if constexpr (MF == 2) sh.sQ[MF == 2 ? buf : 0][MF == 2 ? tid : 0] = (float)(dx * dx + dy * dy + dz * dz);
-            if constexpr (ROWS && PASS != PASS_NEAR) {
+            if constexpr (ROWS) {
                 if (rw > 0.0f) {
                     const double q2 = (double)hx * hx + (double)hy * hy + (double)hz * hz;
                     bf16_source_row(rw * hx, rw * hy, rw * hz, (float)((double)rw * q2), rw,
@@ -1113,7 +1064,7 @@ __device__ __forceinline__ bool force_block(
             }
             // |PAD|^2: r^2 ~ 3e36 keeps r^-3 at zero (and finite) for padding
             if constexpr (MF == 2) sh.sQ[MF == 2 ? buf : 0][MF == 2 ? tid : 0] = 3.0f * PAD_COORD * PAD_COORD;
-            if constexpr (ROWS && PASS != PASS_NEAR) bf16_massless_row(sh.sA[0][ROWS ? tid : 0]);
+            if constexpr (ROWS) bf16_massless_row(sh.sA[0][ROWS ? tid : 0]);
             if constexpr (WFAR) {
                 sh.sW[WFAR ? buf : 0][WFAR ? tid : 0] = make_float4(0.0f, 0.0f, 0.0f, FARW_MASSLESS);
                 sh.sw[WFAR ? buf : 0][WFAR ? tid : 0] = 0.0f;
@@ -1131,7 +1082,7 @@ __device__ __forceinline__ bool force_block(
         // Quarter classes of the next 16 tiles, one per lane, as wave-wide bit
         // masks (the per-tile test is then scalar).  A quarter past the end of
         // the sources holds padding only, which every path sums to zero.
-        if (PASS != PASS_NEAR && (it & 15) == 0) {
+        if ((it & 15) == 0) {
             const int lane = tid & 63;
             const int64_t t = it + (lane >> 2);
             const int q = lane & 3;
@@ -1171,14 +1122,11 @@ __device__ __forceinline__ bool force_block(
             }
             continue;
         }
-        if constexpr (PASS == PASS_NEAR) {
-            if (q_far == 15u) continue;   // summed by PASS_FAR
-        }
         if (by_quarter && q_soft != 15u && q_close != 15u && q_fown != 15u && (q_soft | q_close | q_fown) != 0u) {
             // a tile of mixed quarters: its far quarters first (their sums
             // folded before the near quarters' FP32 sums start)
             if constexpr (QUARTERS) {
-                if constexpr (ROWS && PASS != PASS_NEAR) {
+                if constexpr (ROWS) {
                     if (q_fown != 0u) {
                         float S3[4][4], Sq[4];
 #pragma unroll
@@ -1315,6 +1263,143 @@ __device__ __forceinline__ bool force_block(
         Az[0] += col[2] - zd * col[3];
     }
     return had_near;
+}
+
+// ---------------------------------------------------------------------------
+// Near pass of a split bf16 launch, one wavefront at a time.  Each wave walks
+// its block's source segment on its own: it classifies tile quarters as
+// force_block does and, for each quarter that is not far for it, stages the
+// quarter's 64 sources into its own LDS slice (one per lane) and runs the
+// near arithmetic of its class over them.  No block barriers: a wave with no
+// near quarters finishes at once, and no wave waits for another's tiles.
+// A wavefront's LDS instructions complete in issue order, so wavefront-scope
+// fences (code-motion barriers) order the staging and the reads.
+// ---------------------------------------------------------------------------
+template <typename T, int MF>
+struct NearShared {
+    static constexpr bool LO = BlockShared<T, MF>::HILO && !BlockShared<T, MF>::ABSNEAR;
+    float4 wj[NW][SB];              // hi parts (or the FP32 positions themselves)
+    float4 wl[LO ? NW : 1][SB];     // lo parts
+    double red[11][NW];
+};
+
+template <typename T, int MF>
+__device__ __forceinline__ void near_wave(NearShared<T, MF> &sh, const T *__restrict__ tx,
+                                          const T *__restrict__ ty, const T *__restrict__ tz,
+                                          const double *__restrict__ th, double th_const, int64_t i0,
+                                          int64_t i_end, const T *__restrict__ sx, const T *__restrict__ sy,
+                                          const T *__restrict__ sz, const T *__restrict__ sm, int64_t j0,
+                                          int64_t j1, const double *__restrict__ tile_box,
+                                          const double *__restrict__ sub_box, const double *__restrict__ tgeo,
+                                          const uint8_t *__restrict__ skip_src, bool use_origin, double &Ax,
+                                          double &Ay, double &Az, bool &valid) {
+    constexpr bool HILO = BlockShared<T, MF>::HILO;
+    constexpr bool ABSNEAR = BlockShared<T, MF>::ABSNEAR;
+    constexpr bool LO = NearShared<T, MF>::LO;
+    static_assert(MF == FAR_MFMA_BF16, "near pass of the bf16 far path");
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    double px[1], py[1], pz[1], ph[1];
+    bool v1[1];
+    {
+        const int64_t i = target_of<1>(i0, 0, tid);
+        v1[0] = i < i_end;
+        px[0] = v1[0] ? (double)tx[i] : 0.0;
+        py[0] = v1[0] ? (double)ty[i] : 0.0;
+        pz[0] = v1[0] ? (double)tz[i] : 0.0;
+        ph[0] = v1[0] ? (th ? th[i] : th_const) : 0.0;
+    }
+    valid = v1[0];
+    double ox, oy, oz;
+    if (tgeo) {
+        ox = uniform_f64(tgeo[0]); oy = uniform_f64(tgeo[1]); oz = uniform_f64(tgeo[2]);
+    } else {
+        BlockStats st;
+        block_stats<1>(px, py, pz, ph, v1, use_origin, sh.red, st);
+        ox = uniform_f64(st.ox); oy = uniform_f64(st.oy); oz = uniform_f64(st.oz);
+    }
+    const WaveGeom wg = tgeo ? load_target_geom(tgeo, wave)
+                             : wave_geometry<1>(px, py, pz, ph, v1, ox, oy, oz, HILO && !ABSNEAR);
+    const double dx = px[0] - ox, dy = py[0] - oy, dz = pz[0] - oz;
+    float xi[1] = {(float)dx}, yi[1] = {(float)dy}, zi[1] = {(float)dz};
+    float xil[1] = {(float)(dx - (double)xi[0])}, yil[1] = {(float)(dy - (double)yi[0])},
+          zil[1] = {(float)(dz - (double)zi[0])};
+    float xa[1] = {(float)px[0]}, ya[1] = {(float)py[0]}, za[1] = {(float)pz[0]};
+    const float hf = (float)ph[0];
+    float h2[1] = {hf * hf}, h2c[1] = {fmaxf(hf * hf, FLT_MIN)}, hinv[1] = {1.0f / hf};
+    float hinv3[1] = {hinv[0] * hinv[0] * hinv[0]};
+    float ax[1] = {0.0f}, ay[1] = {0.0f}, az[1] = {0.0f}, aq[1] = {0.0f};
+    Ax = Ay = Az = 0.0;
+    constexpr bool QUARTERS = ABSNEAR;
+    const bool by_quarter = QUARTERS && sub_box != nullptr;
+    float4 *wj = sh.wj[wave];
+    float4 *wl = sh.wl[LO ? wave : 0];
+    uint64_t m_soft = 0, m_close = 0, m_far = 0;
+    int it = 0;
+    for (int64_t base = j0; base < j1; base += BS, ++it) {
+        if ((it & 15) == 0) {
+            const int64_t t = it + (lane >> 2);
+            const int q = lane & 3;
+            bool c_soft = false, c_close = false, c_far = true;
+            if (j0 + t * BS + (by_quarter ? q * SB : 0) < j1) {
+                const double g2 = tile_gap2(by_quarter ? sub_box + (t * NQ + q) * 8 : tile_box + t * 8, wg);
+                c_soft = g2 <= wg.soft_d2;
+                c_close = !c_soft && g2 <= wg.close_d2;
+                c_far = g2 > wg.farw_d2;
+            }
+            m_soft = __builtin_amdgcn_ballot_w64(c_soft);
+            m_close = __builtin_amdgcn_ballot_w64(c_close);
+            m_far = __builtin_amdgcn_ballot_w64(c_far);
+        }
+        const int tsh = 4 * (it & 15);
+        const unsigned q_far = (unsigned)(m_far >> tsh) & 15u;
+        if (q_far == 15u) continue;
+        const unsigned q_soft = (unsigned)(m_soft >> tsh) & 15u;
+        const unsigned q_close = (unsigned)(m_close >> tsh) & 15u;
+#pragma unroll 1
+        for (int q = 0; q < NQ; ++q) {
+            const unsigned qb = 1u << q;
+            if (q_far & qb) continue;
+            {   // stage the quarter: source base + 64 q + lane
+                const int64_t j = base + q * SB + lane;
+                const bool rv = j < j1 && !(skip_src && skip_src[j]);
+                if (ABSNEAR) {
+                    wj[lane] = rv ? make_float4((float)sx[j], (float)sy[j], (float)sz[j], (float)sm[j])
+                                  : make_float4(PAD_COORD, PAD_COORD, PAD_COORD, 0.0f);
+                } else {
+                    if (rv) {
+                        const double sdx = (double)sx[j] - ox, sdy = (double)sy[j] - oy, sdz = (double)sz[j] - oz;
+                        const float hx = (float)sdx, hy = (float)sdy, hz = (float)sdz;
+                        wj[lane] = make_float4(hx, hy, hz, (float)sm[j]);
+                        if (LO)
+                            wl[lane] = make_float4((float)(sdx - (double)hx), (float)(sdy - (double)hy),
+                                                   (float)(sdz - (double)hz), 0.0f);
+                    } else {
+                        wj[lane] = make_float4(PAD_COORD, PAD_COORD, PAD_COORD, 0.0f);
+                        if (LO) wl[lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    }
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            if (q_soft & qb) {
+                if (ABSNEAR)
+                    tile_f32<1, true, false, SB>(wj, xa, ya, za, h2c, h2, hinv, hinv3, ax, ay, az, aq);
+                else
+                    tile_f32_hilo<1, true, false, SB>(wj, wl, xi, yi, zi, xil, yil, zil, h2c, h2, hinv, hinv3, ax,
+                                                      ay, az, aq);
+            } else if (LO && (q_close & qb)) {
+                tile_f32_hilo<1, false, false, SB>(wj, wl, xi, yi, zi, xil, yil, zil, h2c, h2, hinv, hinv3, ax, ay,
+                                                   az, aq);
+            } else if (ABSNEAR) {
+                tile_f32<1, false, false, SB>(wj, xa, ya, za, h2c, h2, hinv, hinv3, ax, ay, az, aq);
+            } else {
+                tile_f32<1, false, false, SB>(wj, xi, yi, zi, h2c, h2, hinv, hinv3, ax, ay, az, aq);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");   // reads done before the next staging
+        }
+        Ax += (double)ax[0]; Ay += (double)ay[0]; Az += (double)az[0];
+        ax[0] = ay[0] = az[0] = 0.0f;
+    }
 }
 
 }  // namespace nb

@@ -24,17 +24,14 @@ namespace nb {
 // with the bf16 far tiles, the SPLIT layout of the FP32 positions).  MF
 // selects the matrix-core far tiles (TPT 1, forces or potentials alone).
 // ACC = false writes the potential only.
-// PASS: every tile (PASS_ALL), or the two halves of a split bf16 launch:
-// PASS_FAR writes the far tiles' sums and flags the blocks that left near
-// tiles out (a.nflag, which holds zeros between launches), then PASS_NEAR
-// adds the near tiles' sums of the flagged blocks to the same outputs and
-// clears their flags.  (A stale flag would only add a zero sum.)
+// PASS: every tile (PASS_ALL), or the far half of a split bf16 launch
+// (PASS_FAR): the far tiles' sums, with the blocks that left near tiles out
+// flagged in a.nflag (which holds zeros between launches) for k_force_near.
 template <typename T, int TPT, bool POT, int MF = 0, bool ACC = true, int PASS = PASS_ALL>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MF == 4 ? 4 : 3))) void k_force_f32(ForceArgs<T> a) {
-    __shared__ BlockShared<T, MF, PASS == PASS_NEAR> sh;
+    __shared__ BlockShared<T, MF> sh;
     const int tid = threadIdx.x;
     const int logical = xcd_remap(blockIdx.x, gridDim.x);
-    if (PASS == PASS_NEAR && !a.nflag[logical]) return;   // block-uniform
     const int seg = logical / a.nblk_i;
     const int ib = logical - seg * a.nblk_i;
     const int64_t i0 = (int64_t)ib * (BS * TPT);
@@ -51,24 +48,11 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MF == 4 ? 4 
     // (plain byte stores: any wave that left near tiles out sets the flag;
     // a block-wide reduction here cost the far pass a few percent)
     if (PASS == PASS_FAR && had_near && (tid & 63) == 0) a.nflag[logical] = 1;
-    // every wave has read the flag: force_block passed barriers since
-    if (PASS == PASS_NEAR && tid == 0) a.nflag[logical] = 0;
 #pragma unroll
     for (int k = 0; k < TPT; ++k) {
         if (!valid[k]) continue;
         const int64_t i = target_of<TPT>(i0, k, tid);
-        if (PASS == PASS_NEAR) {   // forces only: add to the far pass's outputs
-            if (a.nseg == 1) {
-                a.ax[i] += a.G * Ax[k];
-                a.ay[i] += a.G * Ay[k];
-                a.az[i] += a.G * Az[k];
-            } else {
-                double *p = a.part + (int64_t)seg * 4 * a.nt;
-                p[i] += Ax[k];
-                p[a.nt + i] += Ay[k];
-                p[2 * a.nt + i] += Az[k];
-            }
-        } else if (a.nseg == 1) {
+        if (a.nseg == 1) {
             if (ACC) {
                 a.ax[i] = a.G * Ax[k];
                 a.ay[i] = a.G * Ay[k];
@@ -87,6 +71,42 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MF == 4 ? 4 
     }
 }
 
+// Near pass of a split bf16 launch (forces only): the blocks the far pass
+// flagged add their waves' near quarters (near_wave) to the same outputs
+// and clear their flags.  (A stale flag would only add a zero sum.)
+template <typename T>
+__global__ __launch_bounds__(BS) void k_force_near(ForceArgs<T> a) {
+    __shared__ NearShared<T, FAR_MFMA_BF16> sh;
+    const int tid = threadIdx.x;
+    const int logical = xcd_remap(blockIdx.x, gridDim.x);
+    if (!a.nflag[logical]) return;   // block-uniform
+    const int seg = logical / a.nblk_i;
+    const int ib = logical - seg * a.nblk_i;
+    const int64_t i0 = (int64_t)ib * BS;
+    const int64_t j0 = (int64_t)seg * a.seg_len;
+    const int64_t j1 = (j0 + a.seg_len < a.ns) ? j0 + a.seg_len : a.ns;
+    double Ax, Ay, Az;
+    bool valid;
+    near_wave<T, FAR_MFMA_BF16>(sh, a.tx, a.ty, a.tz, a.th, a.th_const, i0, a.nt, a.sx, a.sy, a.sz, a.sm, j0, j1,
+                                a.tile_box + (j0 / BS) * 8, a.sub_box ? a.sub_box + (j0 / BS) * (NQ * 8) : nullptr,
+                                a.tgeo ? a.tgeo + (int64_t)ib * TG_STRIDE : nullptr, a.skip_src,
+                                a.use_origin != 0, Ax, Ay, Az, valid);
+    __syncthreads();   // every wave has read the flag
+    if (tid == 0) a.nflag[logical] = 0;
+    if (!valid) return;
+    const int64_t i = target_of<1>(i0, 0, tid);
+    if (a.nseg == 1) {
+        a.ax[i] += a.G * Ax;
+        a.ay[i] += a.G * Ay;
+        a.az[i] += a.G * Az;
+    } else {
+        double *p = a.part + (int64_t)seg * 4 * a.nt;
+        p[i] += Ax;
+        p[a.nt + i] += Ay;
+        p[2 * a.nt + i] += Az;
+    }
+}
+
 // The bf16 matrix-core force launch (forces only, 1 target per lane): one
 // kernel, or the far and near passes when the caller provides the flags.
 // The split pays where near tiles are few (FP64 state: the collision at 2^20
@@ -97,7 +117,7 @@ template <typename T>
 static void launch_bf16_forces(const ForceArgs<T> &a, int64_t nb, hipStream_t s) {
     if (a.nflag && sizeof(T) == 8) {
         k_force_f32<T, 1, false, FAR_MFMA_BF16, true, PASS_FAR><<<(unsigned)nb, BS, 0, s>>>(a);
-        k_force_f32<T, 1, false, FAR_MFMA_BF16, true, PASS_NEAR><<<(unsigned)nb, BS, 0, s>>>(a);
+        k_force_near<T><<<(unsigned)nb, BS, 0, s>>>(a);
     } else {
         k_force_f32<T, 1, false, FAR_MFMA_BF16><<<(unsigned)nb, BS, 0, s>>>(a);
     }

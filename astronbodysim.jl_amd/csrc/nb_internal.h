@@ -72,11 +72,10 @@ constexpr int FAR_MFMA_R2 = 2;    // r^2 and sum on the matrix cores (farthest t
 constexpr int FAR_WEIGHTED = 3;   // weighted far tiles: 10 VALU + rsq per pair
 constexpr int FAR_MFMA_BF16 = 4;  // bf16-split r^2 on the matrix cores, sum on the VALU
 
-// Passes of the force kernel (force_block): every tile, the far tiles only,
-// or the near tiles only (the last two split the bf16 matrix-core launch).
+// Passes of the force kernel (force_block): every tile, or the far tiles
+// only (the split bf16 matrix-core launch, whose near tiles k_force_near adds).
 constexpr int PASS_ALL = 0;
 constexpr int PASS_FAR = 1;
-constexpr int PASS_NEAR = 2;
 
 // Launchers (nb_force.hip)
 int launch_tile_boxes(const void *x, const void *y, const void *z, int is_double, int64_t n,
