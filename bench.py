@@ -512,8 +512,12 @@ def main():
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": traffic_from_profile(tuning),
-                     "kernel": "k_force_f32<double,%d,false,%s>" % (
-                         tuning["targets_per_thread"], tuning["far_path"]),
+                     # far path 4 with 1 target per lane runs as a far pass and a
+                     # near pass (one force evaluation = both, timed together)
+                     "kernel": ("k_force_f32<double,1,false,4> far pass + k_force_near<double>"
+                                if tuning["far_path"] == 4 and tuning["targets_per_thread"] == 1 else
+                                "k_force_f32<double,%d,false,%s>" % (
+                                    tuning["targets_per_thread"], tuning["far_path"])),
                      "flops_per_pair": FLOPS_PER_PAIR,
                      "avg_kernel_ms": avg_force_s * 1e3, "launches": nforce,
                      "median_single_force_ms": float(np.median(singles)), "single_forces": len(singles)},
