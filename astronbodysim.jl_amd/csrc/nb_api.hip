@@ -110,11 +110,11 @@ struct Geometry {
 };
 
 // Workspaces of the matrix-core far path, allocated when a launch first needs
-// them: the flags of the split far/near passes (one byte per block) and, with
+// them: the flags of the split far/near passes (one byte per wave) and, with
 // several source segments, the bf16 B operands (16 dwords per target).
 int ensure_tbg(nb_ctx *c, const Geometry &g) {
     if (g.tpt != 1 || c->tun.far_path != FAR_MFMA_BF16 || c->mode == NB_MODE_FP64) return NB_OK;
-    const size_t nf = (size_t)g.nblk_i * (size_t)g.nseg;
+    const size_t nf = (size_t)g.nblk_i * (size_t)g.nseg * NW;
     if (nf > c->nflag_cap) {
         drop_graph(c);
         dfree(c->nflag);
@@ -261,7 +261,7 @@ int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = tru
         a.sub_box = c->sub_box;
         a.tgeo = c->tgeo;
         a.tbg = c->tbg;
-        a.nflag = c->nflag_cap >= (size_t)g.nblk_i * (size_t)g.nseg ? c->nflag : nullptr;
+        a.nflag = c->nflag_cap >= (size_t)g.nblk_i * (size_t)g.nseg * NW ? c->nflag : nullptr;
         a.use_origin = 1;
         a.nseg = g.nseg; a.seg_len = g.seg_len; a.nblk_i = g.nblk_i;
         a.G = c->G;
@@ -285,7 +285,7 @@ int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = tru
         a.sub_box = c->sub_box;
         a.tgeo = c->tgeo;
         a.tbg = c->tbg;
-        a.nflag = c->nflag_cap >= (size_t)g.nblk_i * (size_t)g.nseg ? c->nflag : nullptr;
+        a.nflag = c->nflag_cap >= (size_t)g.nblk_i * (size_t)g.nseg * NW ? c->nflag : nullptr;
         a.use_origin = 0;
         a.nseg = g.nseg; a.seg_len = g.seg_len; a.nblk_i = g.nblk_i;
         a.G = c->G;

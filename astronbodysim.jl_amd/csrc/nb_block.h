@@ -46,8 +46,6 @@ struct BlockStats {
     double hmax;
 };
 
-constexpr int NW = BS / 64;
-
 template <int TPT>
 __device__ void block_stats(const double (&px)[TPT], const double (&py)[TPT], const double (&pz)[TPT],
                             const double (&ph)[TPT], const bool (&valid)[TPT], bool centroid,
@@ -1266,139 +1264,150 @@ __device__ __forceinline__ bool force_block(
 }
 
 // ---------------------------------------------------------------------------
-// Near pass of a split bf16 launch, one wavefront at a time.  Each wave walks
-// its block's source segment on its own: it classifies tile quarters as
-// force_block does and, for each quarter that is not far for it, stages the
-// quarter's 64 sources into its own LDS slice (one per lane) and runs the
-// near arithmetic of its class over them.  No block barriers: a wave with no
-// near quarters finishes at once, and no wave waits for another's tiles.
-// A wavefront's LDS instructions complete in issue order, so wavefront-scope
-// fences (code-motion barriers) order the staging and the reads.
+// Near pass of a split bf16 launch, one wavefront at a time.  Wave `wave` of
+// a force block walks the block's source segment on its own: it classifies
+// tile quarters as force_block does, from the block's stored geometry (tgeo,
+// as the far pass read it, so both passes draw the same line between far
+// and near), and, for each quarter that is not far for it, stages the
+// quarter's 64 sources into its LDS slice (one per lane) and runs the near
+// arithmetic of its class over them.  A wavefront's LDS instructions
+// complete in issue order, so wavefront-scope fences (code-motion barriers)
+// order the staging and the reads.
 // ---------------------------------------------------------------------------
 template <typename T, int MF>
 struct NearShared {
     static constexpr bool LO = BlockShared<T, MF>::HILO && !BlockShared<T, MF>::ABSNEAR;
-    float4 wj[NW][SB];              // hi parts (or the FP32 positions themselves)
-    float4 wl[LO ? NW : 1][SB];     // lo parts
-    double red[11][NW];
+    float4 wj[SB];              // hi parts (or the FP32 positions themselves)
+    float4 wl[LO ? SB : 1];     // lo parts
 };
 
 template <typename T, int MF>
-__device__ __forceinline__ void near_wave(NearShared<T, MF> &sh, const T *__restrict__ tx,
+__device__ __forceinline__ void near_wave(NearShared<T, MF> &sh, int wave, const T *__restrict__ tx,
                                           const T *__restrict__ ty, const T *__restrict__ tz,
                                           const double *__restrict__ th, double th_const, int64_t i0,
                                           int64_t i_end, const T *__restrict__ sx, const T *__restrict__ sy,
                                           const T *__restrict__ sz, const T *__restrict__ sm, int64_t j0,
                                           int64_t j1, const double *__restrict__ tile_box,
                                           const double *__restrict__ sub_box, const double *__restrict__ tgeo,
-                                          const uint8_t *__restrict__ skip_src, bool use_origin, double &Ax,
-                                          double &Ay, double &Az, bool &valid) {
-    constexpr bool HILO = BlockShared<T, MF>::HILO;
+                                          const uint8_t *__restrict__ skip_src, double &Ax, double &Ay,
+                                          double &Az, bool &valid) {
     constexpr bool ABSNEAR = BlockShared<T, MF>::ABSNEAR;
     constexpr bool LO = NearShared<T, MF>::LO;
     static_assert(MF == FAR_MFMA_BF16, "near pass of the bf16 far path");
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    double px[1], py[1], pz[1], ph[1];
-    bool v1[1];
+    const int lane = threadIdx.x & 63;
+    double px, py, pz, ph;
     {
-        const int64_t i = target_of<1>(i0, 0, tid);
-        v1[0] = i < i_end;
-        px[0] = v1[0] ? (double)tx[i] : 0.0;
-        py[0] = v1[0] ? (double)ty[i] : 0.0;
-        pz[0] = v1[0] ? (double)tz[i] : 0.0;
-        ph[0] = v1[0] ? (th ? th[i] : th_const) : 0.0;
+        const int64_t i = i0 + wave * 64 + lane;
+        valid = i < i_end;
+        px = valid ? (double)tx[i] : 0.0;
+        py = valid ? (double)ty[i] : 0.0;
+        pz = valid ? (double)tz[i] : 0.0;
+        ph = valid ? (th ? th[i] : th_const) : 0.0;
     }
-    valid = v1[0];
-    double ox, oy, oz;
-    if (tgeo) {
-        ox = uniform_f64(tgeo[0]); oy = uniform_f64(tgeo[1]); oz = uniform_f64(tgeo[2]);
-    } else {
-        BlockStats st;
-        block_stats<1>(px, py, pz, ph, v1, use_origin, sh.red, st);
-        ox = uniform_f64(st.ox); oy = uniform_f64(st.oy); oz = uniform_f64(st.oz);
-    }
-    const WaveGeom wg = tgeo ? load_target_geom(tgeo, wave)
-                             : wave_geometry<1>(px, py, pz, ph, v1, ox, oy, oz, HILO && !ABSNEAR);
-    const double dx = px[0] - ox, dy = py[0] - oy, dz = pz[0] - oz;
+    const double ox = uniform_f64(tgeo[0]), oy = uniform_f64(tgeo[1]), oz = uniform_f64(tgeo[2]);
+    const WaveGeom wg = load_target_geom(tgeo, wave);
+    const double dx = px - ox, dy = py - oy, dz = pz - oz;
     float xi[1] = {(float)dx}, yi[1] = {(float)dy}, zi[1] = {(float)dz};
     float xil[1] = {(float)(dx - (double)xi[0])}, yil[1] = {(float)(dy - (double)yi[0])},
           zil[1] = {(float)(dz - (double)zi[0])};
-    float xa[1] = {(float)px[0]}, ya[1] = {(float)py[0]}, za[1] = {(float)pz[0]};
-    const float hf = (float)ph[0];
+    float xa[1] = {(float)px}, ya[1] = {(float)py}, za[1] = {(float)pz};
+    const float hf = (float)ph;
     float h2[1] = {hf * hf}, h2c[1] = {fmaxf(hf * hf, FLT_MIN)}, hinv[1] = {1.0f / hf};
     float hinv3[1] = {hinv[0] * hinv[0] * hinv[0]};
     float ax[1] = {0.0f}, ay[1] = {0.0f}, az[1] = {0.0f}, aq[1] = {0.0f};
     Ax = Ay = Az = 0.0;
     constexpr bool QUARTERS = ABSNEAR;
     const bool by_quarter = QUARTERS && sub_box != nullptr;
-    float4 *wj = sh.wj[wave];
-    float4 *wl = sh.wl[LO ? wave : 0];
-    uint64_t m_soft = 0, m_close = 0, m_far = 0;
-    int it = 0;
-    for (int64_t base = j0; base < j1; base += BS, ++it) {
-        if ((it & 15) == 0) {
-            const int64_t t = it + (lane >> 2);
-            const int q = lane & 3;
-            bool c_soft = false, c_close = false, c_far = true;
-            if (j0 + t * BS + (by_quarter ? q * SB : 0) < j1) {
-                const double g2 = tile_gap2(by_quarter ? sub_box + (t * NQ + q) * 8 : tile_box + t * 8, wg);
-                c_soft = g2 <= wg.soft_d2;
-                c_close = !c_soft && g2 <= wg.close_d2;
-                c_far = g2 > wg.farw_d2;
+    float4 *wj = sh.wj;
+    float4 *wl = sh.wl;
+    // The classes of the 16 tiles of window `win` (4 quarter bits per tile).
+    uint64_t m_soft = 0, m_close = 0, m_far = ~0ull;
+    int win = -1;
+    // Steps (t, q) on to the wave's next near quarter; false past the segment.
+    auto advance = [&](int &t, int &q) -> bool {
+        for (;;) {
+            if (++q == NQ) { q = 0; ++t; }
+            if (j0 + (int64_t)t * BS >= j1) return false;
+            if ((t >> 4) != win) {
+                win = t >> 4;
+                const int64_t u = 16 * (int64_t)win + (lane >> 2);
+                const int lq = lane & 3;
+                bool c_soft = false, c_close = false, c_far = true;
+                if (j0 + u * BS + (by_quarter ? lq * SB : 0) < j1) {
+                    const double g2 = tile_gap2(by_quarter ? sub_box + (u * NQ + lq) * 8 : tile_box + u * 8, wg);
+                    c_soft = g2 <= wg.soft_d2;
+                    c_close = !c_soft && g2 <= wg.close_d2;
+                    c_far = g2 > wg.farw_d2;
+                }
+                m_soft = __builtin_amdgcn_ballot_w64(c_soft);
+                m_close = __builtin_amdgcn_ballot_w64(c_close);
+                m_far = __builtin_amdgcn_ballot_w64(c_far);
             }
-            m_soft = __builtin_amdgcn_ballot_w64(c_soft);
-            m_close = __builtin_amdgcn_ballot_w64(c_close);
-            m_far = __builtin_amdgcn_ballot_w64(c_far);
+            const unsigned q_far = (unsigned)(m_far >> (4 * (t & 15))) & 15u;
+            if (q_far == 15u) { q = NQ - 1; continue; }   // the whole tile is far
+            if (!((q_far >> q) & 1u)) return true;
         }
-        const int tsh = 4 * (it & 15);
-        const unsigned q_far = (unsigned)(m_far >> tsh) & 15u;
-        if (q_far == 15u) continue;
-        const unsigned q_soft = (unsigned)(m_soft >> tsh) & 15u;
-        const unsigned q_close = (unsigned)(m_close >> tsh) & 15u;
-#pragma unroll 1
-        for (int q = 0; q < NQ; ++q) {
-            const unsigned qb = 1u << q;
-            if (q_far & qb) continue;
-            {   // stage the quarter: source base + 64 q + lane
-                const int64_t j = base + q * SB + lane;
-                const bool rv = j < j1 && !(skip_src && skip_src[j]);
-                if (ABSNEAR) {
-                    wj[lane] = rv ? make_float4((float)sx[j], (float)sy[j], (float)sz[j], (float)sm[j])
-                                  : make_float4(PAD_COORD, PAD_COORD, PAD_COORD, 0.0f);
+    };
+    // The next quarter's sources (this lane's one) are loaded while the
+    // current quarter is computed: index clamped into the segment, validity
+    // decided at staging.
+    T rx, ry, rz, rm;
+    int64_t rj = 0;
+    unsigned rskip = 0;   // the byte itself: tested at staging, so no wait here
+    auto fetch = [&](int t, int q) {
+        rj = j0 + (int64_t)t * BS + q * SB + lane;
+        const int64_t jc = rj < j1 ? rj : j1 - 1;
+        rx = sx[jc]; ry = sy[jc]; rz = sz[jc]; rm = sm[jc];
+        if (skip_src) rskip = skip_src[jc];
+    };
+    int t = 0, q = -1;
+    bool have = advance(t, q);
+    if (have) fetch(t, q);
+    while (have) {
+        {   // stage the quarter fetched last (the previous quarter's reads are done)
+            const bool rv = rj < j1 && rskip == 0;
+            if (ABSNEAR) {
+                wj[lane] = rv ? make_float4((float)rx, (float)ry, (float)rz, (float)rm)
+                              : make_float4(PAD_COORD, PAD_COORD, PAD_COORD, 0.0f);
+            } else {
+                if (rv) {
+                    const double sdx = (double)rx - ox, sdy = (double)ry - oy, sdz = (double)rz - oz;
+                    const float hx = (float)sdx, hy = (float)sdy, hz = (float)sdz;
+                    wj[lane] = make_float4(hx, hy, hz, (float)rm);
+                    if (LO)
+                        wl[lane] = make_float4((float)(sdx - (double)hx), (float)(sdy - (double)hy),
+                                               (float)(sdz - (double)hz), 0.0f);
                 } else {
-                    if (rv) {
-                        const double sdx = (double)sx[j] - ox, sdy = (double)sy[j] - oy, sdz = (double)sz[j] - oz;
-                        const float hx = (float)sdx, hy = (float)sdy, hz = (float)sdz;
-                        wj[lane] = make_float4(hx, hy, hz, (float)sm[j]);
-                        if (LO)
-                            wl[lane] = make_float4((float)(sdx - (double)hx), (float)(sdy - (double)hy),
-                                                   (float)(sdz - (double)hz), 0.0f);
-                    } else {
-                        wj[lane] = make_float4(PAD_COORD, PAD_COORD, PAD_COORD, 0.0f);
-                        if (LO) wl[lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                    }
+                    wj[lane] = make_float4(PAD_COORD, PAD_COORD, PAD_COORD, 0.0f);
+                    if (LO) wl[lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 }
             }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            if (q_soft & qb) {
-                if (ABSNEAR)
-                    tile_f32<1, true, false, SB>(wj, xa, ya, za, h2c, h2, hinv, hinv3, ax, ay, az, aq);
-                else
-                    tile_f32_hilo<1, true, false, SB>(wj, wl, xi, yi, zi, xil, yil, zil, h2c, h2, hinv, hinv3, ax,
-                                                      ay, az, aq);
-            } else if (LO && (q_close & qb)) {
-                tile_f32_hilo<1, false, false, SB>(wj, wl, xi, yi, zi, xil, yil, zil, h2c, h2, hinv, hinv3, ax, ay,
-                                                   az, aq);
-            } else if (ABSNEAR) {
-                tile_f32<1, false, false, SB>(wj, xa, ya, za, h2c, h2, hinv, hinv3, ax, ay, az, aq);
-            } else {
-                tile_f32<1, false, false, SB>(wj, xi, yi, zi, h2c, h2, hinv, hinv3, ax, ay, az, aq);
-            }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");   // reads done before the next staging
         }
-        Ax += (double)ax[0]; Ay += (double)ay[0]; Az += (double)az[0];
-        ax[0] = ay[0] = az[0] = 0.0f;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const int ct = t;
+        const int bit = 4 * (t & 15) + q;
+        const bool is_soft = (m_soft >> bit) & 1u, is_close = (m_close >> bit) & 1u;
+        have = advance(t, q);
+        if (have) fetch(t, q);
+        if (is_soft) {
+            if (ABSNEAR)
+                tile_f32<1, true, false, SB>(wj, xa, ya, za, h2c, h2, hinv, hinv3, ax, ay, az, aq);
+            else
+                tile_f32_hilo<1, true, false, SB>(wj, wl, xi, yi, zi, xil, yil, zil, h2c, h2, hinv, hinv3, ax, ay,
+                                                  az, aq);
+        } else if (LO && is_close) {
+            tile_f32_hilo<1, false, false, SB>(wj, wl, xi, yi, zi, xil, yil, zil, h2c, h2, hinv, hinv3, ax, ay, az,
+                                               aq);
+        } else if (ABSNEAR) {
+            tile_f32<1, false, false, SB>(wj, xa, ya, za, h2c, h2, hinv, hinv3, ax, ay, az, aq);
+        } else {
+            tile_f32<1, false, false, SB>(wj, xi, yi, zi, h2c, h2, hinv, hinv3, ax, ay, az, aq);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");   // reads done before the next staging
+        if (!have || t != ct) {   // a tile's quarters are summed in FP32, tiles in FP64
+            Ax += (double)ax[0]; Ay += (double)ay[0]; Az += (double)az[0];
+            ax[0] = ay[0] = az[0] = 0.0f;
+        }
     }
 }
 

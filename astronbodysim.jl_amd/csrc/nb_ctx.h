@@ -73,7 +73,7 @@ struct nb_ctx {
     double *sub_box = nullptr;    // [ntiles][NQ][8] boxes of the tile quarters
     double *tgeo = nullptr;       // [target blocks][TG_STRIDE] geometry workspace of the force kernel
     unsigned *tbg = nullptr;      // [target blocks][BS][16] bf16 B operands (far path 4), on first need
-    uint8_t *nflag = nullptr;     // one byte per force block: near tiles left to the near pass
+    uint8_t *nflag = nullptr;     // one byte per force wave: near tiles left to the near pass
     size_t nflag_cap = 0;
     double *part = nullptr;
     int part_cap = 0;             // segments the partial buffer holds

@@ -25,7 +25,7 @@ namespace nb {
 // selects the matrix-core far tiles (TPT 1, forces or potentials alone).
 // ACC = false writes the potential only.
 // PASS: every tile (PASS_ALL), or the far half of a split bf16 launch
-// (PASS_FAR): the far tiles' sums, with the blocks that left near tiles out
+// (PASS_FAR): the far tiles' sums, with the waves that left near tiles out
 // flagged in a.nflag (which holds zeros between launches) for k_force_near.
 template <typename T, int TPT, bool POT, int MF = 0, bool ACC = true, int PASS = PASS_ALL>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MF == 4 ? 4 : 3))) void k_force_f32(ForceArgs<T> a) {
@@ -45,9 +45,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MF == 4 ? 4 
                              a.tgeo ? a.tgeo + (int64_t)ib * TG_STRIDE : nullptr,
                              a.tbg ? a.tbg + (int64_t)ib * BS * 16 : nullptr, a.skip_src,
                              a.use_origin != 0, Ax, Ay, Az, Aq, valid);
-    // (plain byte stores: any wave that left near tiles out sets the flag;
-    // a block-wide reduction here cost the far pass a few percent)
-    if (PASS == PASS_FAR && had_near && (tid & 63) == 0) a.nflag[logical] = 1;
+    // (one plain byte store per wave that left near tiles out)
+    if (PASS == PASS_FAR && had_near && (tid & 63) == 0) a.nflag[(int64_t)logical * NW + (tid >> 6)] = 1;
 #pragma unroll
     for (int k = 0; k < TPT; ++k) {
         if (!valid[k]) continue;
@@ -71,15 +70,18 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MF == 4 ? 4 
     }
 }
 
-// Near pass of a split bf16 launch (forces only): the blocks the far pass
-// flagged add their waves' near quarters (near_wave) to the same outputs
-// and clear their flags.  (A stale flag would only add a zero sum.)
+// Near pass of a split bf16 launch (forces only): one wavefront per
+// workgroup, one workgroup per (force block, wave) of the far pass.  The
+// waves the far pass flagged add their near quarters (near_wave) to the same
+// outputs and clear their flags; the rest return at once.  Each wave runs on
+// its own, so a wave with many near tiles holds no other wave's resources.
 template <typename T>
-__global__ __launch_bounds__(BS) void k_force_near(ForceArgs<T> a) {
+__global__ __launch_bounds__(64) void k_force_near(ForceArgs<T> a) {
     __shared__ NearShared<T, FAR_MFMA_BF16> sh;
-    const int tid = threadIdx.x;
-    const int logical = xcd_remap(blockIdx.x, gridDim.x);
-    if (!a.nflag[logical]) return;   // block-uniform
+    const int lw = xcd_remap(blockIdx.x, gridDim.x);
+    if (!a.nflag[lw]) return;   // wave-uniform
+    if (threadIdx.x == 0) a.nflag[lw] = 0;
+    const int logical = lw / NW, wave = lw - logical * NW;
     const int seg = logical / a.nblk_i;
     const int ib = logical - seg * a.nblk_i;
     const int64_t i0 = (int64_t)ib * BS;
@@ -87,14 +89,12 @@ __global__ __launch_bounds__(BS) void k_force_near(ForceArgs<T> a) {
     const int64_t j1 = (j0 + a.seg_len < a.ns) ? j0 + a.seg_len : a.ns;
     double Ax, Ay, Az;
     bool valid;
-    near_wave<T, FAR_MFMA_BF16>(sh, a.tx, a.ty, a.tz, a.th, a.th_const, i0, a.nt, a.sx, a.sy, a.sz, a.sm, j0, j1,
-                                a.tile_box + (j0 / BS) * 8, a.sub_box ? a.sub_box + (j0 / BS) * (NQ * 8) : nullptr,
-                                a.tgeo ? a.tgeo + (int64_t)ib * TG_STRIDE : nullptr, a.skip_src,
-                                a.use_origin != 0, Ax, Ay, Az, valid);
-    __syncthreads();   // every wave has read the flag
-    if (tid == 0) a.nflag[logical] = 0;
+    near_wave<T, FAR_MFMA_BF16>(sh, wave, a.tx, a.ty, a.tz, a.th, a.th_const, i0, a.nt, a.sx, a.sy, a.sz, a.sm,
+                                j0, j1, a.tile_box + (j0 / BS) * 8,
+                                a.sub_box ? a.sub_box + (j0 / BS) * (NQ * 8) : nullptr,
+                                a.tgeo + (int64_t)ib * TG_STRIDE, a.skip_src, Ax, Ay, Az, valid);
     if (!valid) return;
-    const int64_t i = target_of<1>(i0, 0, tid);
+    const int64_t i = i0 + wave * 64 + (threadIdx.x & 63);
     if (a.nseg == 1) {
         a.ax[i] += a.G * Ax;
         a.ay[i] += a.G * Ay;
@@ -108,16 +108,17 @@ __global__ __launch_bounds__(BS) void k_force_near(ForceArgs<T> a) {
 }
 
 // The bf16 matrix-core force launch (forces only, 1 target per lane): one
-// kernel, or the far and near passes when the caller provides the flags.
+// kernel, or the far and near passes when the caller provides the flags
+// and the target blocks' stored geometry (several source segments).
 // The split pays where near tiles are few (FP64 state: the collision at 2^20
 // has 1.8 % of its pairs in them, and its far loop alone runs about 10 %
 // faster); with FP32 state (the FP32 Plummer sphere at 2^16: 18 %) the near
 // pass cost more than it saved, so that launch stays in one kernel.
 template <typename T>
 static void launch_bf16_forces(const ForceArgs<T> &a, int64_t nb, hipStream_t s) {
-    if (a.nflag && sizeof(T) == 8) {
+    if (a.nflag && a.tgeo && sizeof(T) == 8) {
         k_force_f32<T, 1, false, FAR_MFMA_BF16, true, PASS_FAR><<<(unsigned)nb, BS, 0, s>>>(a);
-        k_force_near<T><<<(unsigned)nb, BS, 0, s>>>(a);
+        k_force_near<T><<<(unsigned)(nb * NW), 64, 0, s>>>(a);
     } else {
         k_force_f32<T, 1, false, FAR_MFMA_BF16><<<(unsigned)nb, BS, 0, s>>>(a);
     }

@@ -9,8 +9,9 @@
 
 namespace nb {
 
-// Source tile = one LDS stage = one block of BS threads.
+// Source tile = one LDS stage = one block of BS threads (NW wavefronts).
 constexpr int BS = 256;
+constexpr int NW = BS / 64;
 // Tile quarters: the SB sources one wave stages.  Their boxes let a wave
 // classify a tile quarter by quarter.
 constexpr int SB = 64;
@@ -52,7 +53,7 @@ struct ForceArgs {
     const double *sub_box;   // [ntiles][NQ][8]: the same for each tile's quarters, or nullptr
     double *tgeo;            // [nblk_i][TG_STRIDE] workspace for the target blocks' geometry, or nullptr
     unsigned *tbg;           // [nblk_i][BS][16] workspace for their bf16 B operands, or nullptr
-    uint8_t *nflag;          // [nblk_i * nseg]: blocks with near tiles (split passes), or nullptr
+    uint8_t *nflag;          // [nblk_i * nseg * NW]: waves with near tiles (split passes), or nullptr
     int use_origin;          // 1: per-block FP64 origin (SPLIT), 0: absolute FP32 coordinates
     int nseg;                // source segments (grid = nblk_i * nseg)
     int64_t seg_len;         // sources per segment (multiple of BS)
