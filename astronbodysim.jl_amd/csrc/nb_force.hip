@@ -116,12 +116,14 @@ __global__ __launch_bounds__(64) void k_force_near(ForceArgs<T> a) {
 // pass cost more than it saved, so that launch stays in one kernel.
 template <typename T>
 static void launch_bf16_forces(const ForceArgs<T> &a, int64_t nb, hipStream_t s) {
-    if (a.nflag && a.tgeo && sizeof(T) == 8) {
-        k_force_f32<T, 1, false, FAR_MFMA_BF16, true, PASS_FAR><<<(unsigned)nb, BS, 0, s>>>(a);
-        k_force_near<T><<<(unsigned)(nb * NW), 64, 0, s>>>(a);
-    } else {
-        k_force_f32<T, 1, false, FAR_MFMA_BF16><<<(unsigned)nb, BS, 0, s>>>(a);
+    if constexpr (sizeof(T) == 8) {
+        if (a.nflag && a.tgeo) {
+            k_force_f32<T, 1, false, FAR_MFMA_BF16, true, PASS_FAR><<<(unsigned)nb, BS, 0, s>>>(a);
+            k_force_near<T><<<(unsigned)(nb * NW), 64, 0, s>>>(a);
+            return;
+        }
     }
+    k_force_f32<T, 1, false, FAR_MFMA_BF16><<<(unsigned)nb, BS, 0, s>>>(a);
 }
 
 // The origin and wave geometry of each target block (force_block's first

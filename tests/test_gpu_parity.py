@@ -202,12 +202,17 @@ def test_kdk_fp64_matches_oracle_run(pkg, orc):
     assert np.max(np.abs(d["vel"] - vo)) < 1e-10
 
 
-def test_graph_replay_matches_eager(pkg):
+@pytest.mark.parametrize("far", [0, 4])
+def test_graph_replay_matches_eager(pkg, far):
+    """KDK steps replayed from a captured graph, with host work between the
+    replays, give the eager run's bits.  With far path 4 and several source
+    segments each force is the far pass and the near pass, whose per-wave
+    flags the far pass sets and the near pass clears on every replay."""
     pos, vel, mass = pkg.ics.plummer(3000, seed=4)
     soft, dt = [0.01] * 6, 1e-3
     out = []
     for use_graph in (False, True, True):
-        with make_ctx(pkg, pos, vel, mass, soft, "split", tuning=(1, 4, 0)) as ctx:
+        with make_ctx(pkg, pos, vel, mass, soft, "split", tuning=(1, 4, 0, far)) as ctx:
             ctx.compute_force()
             ctx.half_kick(+1, dt)
             for rep in range(3):    # several replays with host work between them
@@ -218,6 +223,32 @@ def test_graph_replay_matches_eager(pkg):
     for k in (1, 2):
         assert np.array_equal(out[0]["pos"], out[k]["pos"])
         assert np.array_equal(out[0]["vel"], out[k]["vel"])
+
+
+def test_split_far_near_passes_repeat_and_match_one_kernel(pkg, orc):
+    """The FP64-state bf16 launch with several source segments runs as a far
+    pass and a near pass (per-wave flags set by the first, cleared by the
+    second): three force calls in a row give the same bits, and the result
+    matches the one-kernel launch of one segment (same tiles, same classes:
+    only the FP64 segment sums differ) and the oracle."""
+    pos, vel, mass = pkg.ics.collision(8192, seeds=(3, 4))
+    soft = [0.002] * 6
+    ref = orc.compute_force(pos, mass, soft)
+    accs = []
+    with make_ctx(pkg, pos, vel, mass, soft, "split", tuning=(1, 4, 16, 4)) as ctx:
+        for _ in range(3):
+            ctx.compute_force()
+            accs.append(ctx.download(pos=False, vel=False)["acc"])
+    with make_ctx(pkg, pos, vel, mass, soft, "split", tuning=(1, 1, 16, 4)) as ctx:
+        ctx.compute_force()
+        one = ctx.download(pos=False, vel=False)["acc"]
+    assert np.array_equal(accs[0], accs[1]) and np.array_equal(accs[0], accs[2])
+    d = orc.rel_err(accs[0], one)
+    e = orc.rel_err(accs[0], ref)
+    print(f"split passes vs one kernel: median {np.median(d):.3e} max {d.max():.3e}; "
+          f"vs oracle: median {np.median(e):.3e} p99 {np.percentile(e, 99):.3e}")
+    assert np.median(d) <= 1e-13 and d.max() <= 1e-12   # measured 1.6e-16 and 1.2e-15
+    assert np.median(e) <= 2e-7 and np.percentile(e, 99) <= 2e-6
 
 
 @pytest.mark.parametrize("far", [0, 1, 2, 3, 4])
