@@ -224,26 +224,41 @@ int gather_central(nb_ctx *c) {
 
 // The all-pairs evaluation for every particle (acc, and potential when asked).
 // Re-sort into Morton order when the order is stale.
+static bool sort_due(const nb_ctx *c, int64_t upcoming) {
+    return c->sort.interval > 0 && c->n >= 2 &&
+           (!c->sort.valid || c->calls_since_sort + upcoming > c->sort.interval);
+}
+
 int maybe_sort(nb_ctx *c, int64_t upcoming) {
-    if (c->sort.interval <= 0 || c->n < 2) return NB_OK;
-    if (!c->sort.valid || c->calls_since_sort + upcoming > c->sort.interval) {
+    if (sort_due(c, upcoming)) {
         NB_TRY(morton_reorder(c));
         c->calls_since_sort = 0;
     }
     return NB_OK;
 }
 
+// Work of a KDK step that rides on run_force's kernels: the drift has written
+// the tile boxes already (boxes_ready), and the kick of `kick` (x dt) may be
+// done by the finalize (kicked reports whether it was).
+struct StepFusion {
+    bool boxes_ready = false;
+    double kick = 0.0;
+    bool kicked = false;
+};
+
 // pot_only: the FP32-pair kernels write the potential alone (the caller keeps
 // the accelerations; the FP64 kernels and passes still write both).
 int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = true,
-              bool pot_only = false) {
+              bool pot_only = false, StepFusion *fuse = nullptr) {
     hipStream_t s = c->stream;
     const int64_t n = c->n;
     if (n == 0) return NB_OK;
+    const bool resorted = allow_sort && sort_due(c, 1);   // the boxes of a drift no longer hold
     if (allow_sort) NB_TRY(maybe_sort(c, 1));
     c->force_calls++;
     c->calls_since_sort++;
-    NB_TRY(launch_tile_boxes(c->x, c->y, c->z, c->is_double, n, c->tile_box, c->sub_box, s));
+    if (!(fuse && fuse->boxes_ready && !resorted))
+        NB_TRY(launch_tile_boxes(c->x, c->y, c->z, c->is_double, n, c->tile_box, c->sub_box, s));
     const bool f64 = c->mode == NB_MODE_FP64;
     const int tpt = f64 ? 1 : c->tun.targets_per_thread;
     Geometry g = geometry(n, n, tpt, c->tun.j_segments);
@@ -301,9 +316,17 @@ int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = tru
         NB_TRY(launch_force_f32(a, tpt, with_pot, s, acc_out, c->tun.far_path));
     }
     if (allow_timing) NB_TRY(record_stop(c, e0));
-    if (g.nseg > 1)
+    // the kick rides on the finalize when nothing else adds to the accelerations
+    const bool kick_here = fuse && !with_pot && g.nseg > 1 && c->bg.empty() && !(c->mond_acc0 > 0.0) &&
+                           !(c->mode == NB_MODE_SPLIT_CENTRAL && c->ncentral > 0);
+    if (kick_here) {
+        NB_TRY(launch_finalize_kick(c->part, g.nseg, n, c->G, c->ax, c->ay, c->az, c->vx, c->vy, c->vz,
+                                    fuse->kick, c->is_double, s));
+        fuse->kicked = true;
+    } else if (g.nseg > 1) {
         NB_TRY(launch_finalize(c->part, g.nseg, n, c->G, c->ax, c->ay, c->az, c->pot, with_pot, s,
                                !(with_pot && pot_only) || f64));
+    }
     // FP64 central-body terms: every target, sources = the central particles.
     if (c->mode == NB_MODE_SPLIT_CENTRAL && c->ncentral > 0) {
         const int64_t k = c->ncentral;
@@ -786,12 +809,23 @@ int nb_kick(nb_ctx *c, double dt) {
     return NB_OK;
 }
 
+// Drift (with the tile boxes of the new positions), force, kick (with the
+// finalize of the accelerations where it can).
 static int enqueue_step(nb_ctx *c, int64_t k, double dt, bool allow_timing, bool allow_sort) {
     const int rd = (int)(k & 1), wr = 1 - rd;
-    NB_TRY(drift(c->x, c->y, c->z, c->vx, c->vy, c->vz, c->n, dt, c->time, rd, wr, c->is_double,
-                 c->stream));
-    NB_TRY(run_force(c, false, allow_timing, allow_sort));
-    NB_TRY(kick(c->vx, c->vy, c->vz, c->ax, c->ay, c->az, c->n, dt, c->is_double, c->stream));
+    StepFusion fuse;
+    if (c->n > 0) {
+        NB_TRY(launch_drift_boxes(c->x, c->y, c->z, c->vx, c->vy, c->vz, c->n, dt, c->time, rd, wr, c->is_double,
+                                  c->tile_box, c->sub_box, c->stream));
+        fuse.boxes_ready = true;
+    } else {
+        NB_TRY(drift(c->x, c->y, c->z, c->vx, c->vy, c->vz, c->n, dt, c->time, rd, wr, c->is_double,
+                     c->stream));
+    }
+    fuse.kick = dt;
+    NB_TRY(run_force(c, false, allow_timing, allow_sort, false, &fuse));
+    if (!fuse.kicked)
+        NB_TRY(kick(c->vx, c->vy, c->vz, c->ax, c->ay, c->az, c->n, dt, c->is_double, c->stream));
     return NB_OK;
 }
 

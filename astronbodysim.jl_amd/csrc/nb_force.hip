@@ -239,19 +239,16 @@ __global__ void k_finalize(const double *__restrict__ part, int nseg, int64_t n,
 
 // Bounding box of each BS-source tile (lo xyz, hi xyz) and, when sub is not
 // null, of each of its NQ quarters (the sources of one wave; an empty quarter
-// gets lo = DBL_MAX, hi = -DBL_MAX).
-template <typename T>
-__global__ __launch_bounds__(BS) void k_tile_boxes(const T *__restrict__ x, const T *__restrict__ y,
-                                                   const T *__restrict__ z, int64_t n,
-                                                   double *__restrict__ box, double *__restrict__ sub) {
-    __shared__ double red[6][NW];
-    const int64_t j = (int64_t)blockIdx.x * BS + threadIdx.x;
-    const bool v = j < n;
+// gets lo = DBL_MAX, hi = -DBL_MAX).  One block per tile; thread t holds the
+// position of source BS * blockIdx.x + t (v: the source exists).
+__device__ __forceinline__ void tile_box_block(double px, double py, double pz, bool v,
+                                               double *__restrict__ box, double *__restrict__ sub,
+                                               double (&red)[6][NW]) {
     double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
     if (v) {
-        lo[0] = hi[0] = (double)x[j];
-        lo[1] = hi[1] = (double)y[j];
-        lo[2] = hi[2] = (double)z[j];
+        lo[0] = hi[0] = px;
+        lo[1] = hi[1] = py;
+        lo[2] = hi[2] = pz;
     }
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -275,6 +272,68 @@ __global__ __launch_bounds__(BS) void k_tile_boxes(const T *__restrict__ x, cons
         for (int ww = 1; ww < NW; ++ww) r = q < 3 ? fmin(r, red[q][ww]) : fmax(r, red[q][ww]);
         box[(int64_t)blockIdx.x * 8 + q] = r;
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(BS) void k_tile_boxes(const T *__restrict__ x, const T *__restrict__ y,
+                                                   const T *__restrict__ z, int64_t n,
+                                                   double *__restrict__ box, double *__restrict__ sub) {
+    __shared__ double red[6][NW];
+    const int64_t j = (int64_t)blockIdx.x * BS + threadIdx.x;
+    const bool v = j < n;
+    tile_box_block(v ? (double)x[j] : 0.0, v ? (double)y[j] : 0.0, v ? (double)z[j] : 0.0, v, box, sub, red);
+}
+
+// A step's drift (Pos += Vel * (t_new - t_old), the operations of k_drift,
+// with block 0 publishing t_new = time[rd] + dt to time[wr]) fused with the
+// tile boxes of the drifted positions, which the next force needs.
+template <typename T>
+__global__ __launch_bounds__(BS) void k_drift_boxes(T *__restrict__ x, T *__restrict__ y, T *__restrict__ z,
+                                                    const T *__restrict__ vx, const T *__restrict__ vy,
+                                                    const T *__restrict__ vz, int64_t n, double dt,
+                                                    double *time, int rd, int wr, double *__restrict__ box,
+                                                    double *__restrict__ sub) {
+    __shared__ double red[6][NW];
+    const int64_t j = (int64_t)blockIdx.x * BS + threadIdx.x;
+    const double t_old = time[rd];
+    const double t_new = t_old + dt;
+    if (j == 0) time[wr] = t_new;
+    const bool v = j < n;
+    T px = 0, py = 0, pz = 0;
+    if (v) {
+        const T d = (T)(t_new - t_old);
+        px = x[j]; py = y[j]; pz = z[j];
+        px += vx[j] * d;
+        py += vy[j] * d;
+        pz += vz[j] * d;
+        x[j] = px; y[j] = py; z[j] = pz;
+    }
+    tile_box_block((double)px, (double)py, (double)pz, v, box, sub, red);
+}
+
+// k_finalize followed by the step's kick (Vel += Acc * coef, the operations
+// of k_kick) on the accelerations it has just written.
+template <typename T>
+__global__ void k_finalize_kick(const double *__restrict__ part, int nseg, int64_t n, double G,
+                                double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
+                                T *__restrict__ vx, T *__restrict__ vy, T *__restrict__ vz, double coef) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int s = 0; s < nseg; ++s) {
+        const double *p = part + (int64_t)s * 4 * n;
+        sx += p[i];
+        sy += p[n + i];
+        sz += p[2 * n + i];
+    }
+    const double fx = G * sx, fy = G * sy, fz = G * sz;
+    ax[i] = fx;
+    ay[i] = fy;
+    az[i] = fz;
+    const T c = (T)coef;
+    vx[i] += (T)fx * c;
+    vy[i] += (T)fy * c;
+    vz[i] += (T)fz * c;
 }
 
 // ---------------------------------------------------------------------------
@@ -424,6 +483,35 @@ int launch_force_f64(const ForceArgs<double> &a, bool pot, hipStream_t s) {
     if (pot) k_force_f64<true><<<(unsigned)nb, BS, 0, s>>>(a);
     else k_force_f64<false><<<(unsigned)nb, BS, 0, s>>>(a);
     return check_launch("k_force_f64");
+}
+
+int launch_drift_boxes(void *x, void *y, void *z, const void *vx, const void *vy, const void *vz, int64_t n,
+                       double dt, double *time, int rd, int wr, int is_double, double *box, double *sub_box,
+                       hipStream_t s) {
+    if (n <= 0) return NB_OK;
+    const int64_t ntiles = (n + BS - 1) / BS;
+    if (is_double)
+        k_drift_boxes<double><<<(unsigned)ntiles, BS, 0, s>>>(
+            (double *)x, (double *)y, (double *)z, (const double *)vx, (const double *)vy, (const double *)vz, n,
+            dt, time, rd, wr, box, sub_box);
+    else
+        k_drift_boxes<float><<<(unsigned)ntiles, BS, 0, s>>>(
+            (float *)x, (float *)y, (float *)z, (const float *)vx, (const float *)vy, (const float *)vz, n, dt,
+            time, rd, wr, box, sub_box);
+    return check_launch("k_drift_boxes");
+}
+
+int launch_finalize_kick(const double *part, int nseg, int64_t n, double G, double *ax, double *ay, double *az,
+                         void *vx, void *vy, void *vz, double coef, int is_double, hipStream_t s) {
+    if (n <= 0) return NB_OK;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    if (is_double)
+        k_finalize_kick<double><<<nb, 256, 0, s>>>(part, nseg, n, G, ax, ay, az, (double *)vx, (double *)vy,
+                                                   (double *)vz, coef);
+    else
+        k_finalize_kick<float><<<nb, 256, 0, s>>>(part, nseg, n, G, ax, ay, az, (float *)vx, (float *)vy,
+                                                  (float *)vz, coef);
+    return check_launch("k_finalize_kick");
 }
 
 int launch_finalize(const double *part, int nseg, int64_t n, double G, double *ax, double *ay,

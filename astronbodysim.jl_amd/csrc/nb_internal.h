@@ -95,5 +95,13 @@ inline bool far_path_weighted(int far_path) { return far_path == FAR_WEIGHTED ||
 int launch_finalize(const double *part, int nseg, int64_t n, double G, double *ax, double *ay,
                     double *az, double *pot, bool with_pot, hipStream_t s,
                     bool with_acc = true);
+// The KDK step's drift fused with the tile boxes of the drifted positions, and
+// its kick fused with the finalize of the accelerations (same arithmetic as
+// the separate kernels).
+int launch_drift_boxes(void *x, void *y, void *z, const void *vx, const void *vy, const void *vz, int64_t n,
+                       double dt, double *time, int rd, int wr, int is_double, double *box, double *sub_box,
+                       hipStream_t s);
+int launch_finalize_kick(const double *part, int nseg, int64_t n, double G, double *ax, double *ay, double *az,
+                         void *vx, void *vy, void *vz, double coef, int is_double, hipStream_t s);
 
 }  // namespace nb
