@@ -225,6 +225,27 @@ def test_graph_replay_matches_eager(pkg, far):
         assert np.array_equal(out[0]["vel"], out[k]["vel"])
 
 
+@pytest.mark.parametrize("far,sort,use_graph", [(4, 1, False), (4, 2, True), (0, 1, False)])
+def test_kdk_step_forces_after_resorts(pkg, orc, far, sort, use_graph):
+    """Inside nb_kdk_steps the drift writes the tile boxes of the drifted
+    positions and the kick rides on the finalize; a step that re-sorts
+    first recomputes the boxes.  With a re-sort every step (eager) or between
+    graph launches, the accelerations left by the last step must be the
+    forces at the final positions: against the oracle on the downloaded
+    state."""
+    pos, vel, mass = pkg.ics.collision(8192, seeds=(3, 4))
+    soft, dt = [0.002] * 6, 1e-3
+    with make_ctx(pkg, pos, vel, mass, soft, "split", tuning=(1, 4, sort, far)) as ctx:
+        ctx.compute_force()
+        ctx.half_kick(+1, dt)
+        ctx.kdk_steps(8, dt, 0.0, use_graph=use_graph)
+        d = ctx.download(pos=True, vel=False, acc=True)
+    ref = orc.compute_force(d["pos"], mass, soft)
+    e = orc.rel_err(d["acc"], ref)
+    print(f"far {far} sort {sort} graph {use_graph}: median {np.median(e):.3e} p99 {np.percentile(e, 99):.3e}")
+    assert np.median(e) <= 2e-7 and np.percentile(e, 99) <= 2e-6
+
+
 def test_split_far_near_passes_repeat_and_match_one_kernel(pkg, orc):
     """The FP64-state bf16 launch with several source segments runs as a far
     pass and a near pass (per-wave flags set by the first, cleared by the
