@@ -80,14 +80,6 @@ __global__ void k_morton(const T *__restrict__ x, const T *__restrict__ y, const
     idx[i] = i;
 }
 
-template <typename V>
-__global__ void k_gather(const V *__restrict__ src, const int64_t *__restrict__ idx, int64_t n,
-                         V *__restrict__ dst) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    dst[i] = src[idx[i]];
-}
-
 __global__ void k_compose_perm(const int64_t *__restrict__ old_perm, const int64_t *__restrict__ idx,
                                int64_t n, int64_t *__restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -95,13 +87,50 @@ __global__ void k_compose_perm(const int64_t *__restrict__ old_perm, const int64
     out[i] = old_perm ? old_perm[idx[i]] : idx[i];
 }
 
-template <typename V>
-static int permute(V *arr, const int64_t *idx, int64_t n, void *spare, hipStream_t s) {
-    const unsigned g = (unsigned)((n + 255) / 256);
-    k_gather<V><<<g, 256, 0, s>>>(arr, idx, n, (V *)spare);
-    NB_TRY(check_launch("k_gather"));
-    NB_HIP(hipMemcpyAsync(arr, spare, sizeof(V) * (size_t)n, hipMemcpyDeviceToDevice, s));
-    return NB_OK;
+// The particle arrays that follow a re-sort, gathered by one permutation in
+// one pass (elements of 1, 4 or 8 bytes) into their own areas of the spare
+// buffer, then copied back in a second pass: two launches for all of them.
+constexpr int MAX_SORTED_ARRAYS = 16;
+struct GatherSet {
+    void *arr[MAX_SORTED_ARRAYS];
+    int64_t off[MAX_SORTED_ARRAYS];   // byte offset of the array's area in the spare buffer
+    int size[MAX_SORTED_ARRAYS];      // bytes per element
+    int count;
+};
+
+__global__ void k_gather_set(GatherSet g, const int64_t *__restrict__ idx, int64_t n,
+                             char *__restrict__ spare) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t k = idx[i];
+    for (int a = 0; a < g.count; ++a) {
+        char *dst = spare + g.off[a];
+        if (g.size[a] == 8) ((uint64_t *)dst)[i] = ((const uint64_t *)g.arr[a])[k];
+        else if (g.size[a] == 4) ((uint32_t *)dst)[i] = ((const uint32_t *)g.arr[a])[k];
+        else ((uint8_t *)dst)[i] = ((const uint8_t *)g.arr[a])[k];
+    }
+}
+
+__global__ void k_copy_back_set(GatherSet g, int64_t n, const char *__restrict__ spare) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int a = 0; a < g.count; ++a) {
+        const char *src = spare + g.off[a];
+        if (g.size[a] == 8) ((uint64_t *)g.arr[a])[i] = ((const uint64_t *)src)[i];
+        else if (g.size[a] == 4) ((uint32_t *)g.arr[a])[i] = ((const uint32_t *)src)[i];
+        else ((uint8_t *)g.arr[a])[i] = ((const uint8_t *)src)[i];
+    }
+}
+
+// Bytes of spare buffer for the largest set: 16 arrays of 8-byte elements.
+static size_t spare_bytes(int64_t n) { return (size_t)MAX_SORTED_ARRAYS * 8 * (size_t)n; }
+
+static void add_array(GatherSet &g, void *arr, int size, int64_t n) {
+    const int64_t off = g.count ? g.off[g.count - 1] + ((g.size[g.count - 1] * n + 7) / 8) * 8 : 0;
+    g.arr[g.count] = arr;
+    g.off[g.count] = off;
+    g.size[g.count] = size;
+    ++g.count;
 }
 
 int morton_reorder(nb_ctx *c) {
@@ -113,7 +142,7 @@ int morton_reorder(nb_ctx *c) {
         NB_TRY(dalloc(&S.keys, sizeof(uint64_t) * 2 * (size_t)n));
         NB_TRY(dalloc(&S.idx, sizeof(int64_t) * 2 * (size_t)n));
         NB_TRY(dalloc(&S.box, sizeof(double) * 8));
-        NB_TRY(dalloc(&S.spare, sizeof(double) * (size_t)n));
+        NB_TRY(dalloc(&S.spare, spare_bytes(n)));
         size_t bytes = 0;
         NB_HIP(rocprim::radix_sort_pairs(nullptr, bytes, S.keys, S.keys + n, S.idx, S.idx + n,
                                          (size_t)n, 0, 63, s));
@@ -136,34 +165,20 @@ int morton_reorder(nb_ctx *c) {
     NB_HIP(rocprim::radix_sort_pairs(S.tmp, bytes, S.keys, S.keys + n, S.idx, S.idx + n, (size_t)n,
                                      0, 63, s));
     const int64_t *idx = S.idx + n;  // sorted: internal position i takes old position idx[i]
-    if (c->is_double) {
-        NB_TRY(permute((double *)c->x, idx, n, S.spare, s));
-        NB_TRY(permute((double *)c->y, idx, n, S.spare, s));
-        NB_TRY(permute((double *)c->z, idx, n, S.spare, s));
-        NB_TRY(permute((double *)c->vx, idx, n, S.spare, s));
-        NB_TRY(permute((double *)c->vy, idx, n, S.spare, s));
-        NB_TRY(permute((double *)c->vz, idx, n, S.spare, s));
-        NB_TRY(permute((double *)c->m, idx, n, S.spare, s));
-    } else {
-        NB_TRY(permute((float *)c->x, idx, n, S.spare, s));
-        NB_TRY(permute((float *)c->y, idx, n, S.spare, s));
-        NB_TRY(permute((float *)c->z, idx, n, S.spare, s));
-        NB_TRY(permute((float *)c->vx, idx, n, S.spare, s));
-        NB_TRY(permute((float *)c->vy, idx, n, S.spare, s));
-        NB_TRY(permute((float *)c->vz, idx, n, S.spare, s));
-        NB_TRY(permute((float *)c->m, idx, n, S.spare, s));
-    }
-    NB_TRY(permute(c->h, idx, n, S.spare, s));
-    NB_TRY(permute(c->ax, idx, n, S.spare, s));
-    NB_TRY(permute(c->ay, idx, n, S.spare, s));
-    NB_TRY(permute(c->az, idx, n, S.spare, s));
-    NB_TRY(permute(c->pot, idx, n, S.spare, s));
-    NB_TRY(permute(c->coll, idx, n, S.spare, s));
-    NB_TRY(permute(c->central, idx, n, S.spare, s));
+    GatherSet set{};
+    const int es = c->is_double ? 8 : 4;
+    for (void *a : {c->x, c->y, c->z, c->vx, c->vy, c->vz, c->m}) add_array(set, a, es, n);
+    for (void *a : {(void *)c->h, (void *)c->ax, (void *)c->ay, (void *)c->az, (void *)c->pot}) add_array(set, a, 8, n);
+    if (c->coll) add_array(set, c->coll, 1, n);
+    if (c->central) add_array(set, c->central, 1, n);
     if (c->ti_end) {
-        NB_TRY(permute(c->ti_end, idx, n, S.spare, s));
-        NB_TRY(permute(c->ti_beg, idx, n, S.spare, s));
+        add_array(set, c->ti_end, 8, n);
+        add_array(set, c->ti_beg, 8, n);
     }
+    k_gather_set<<<g, 256, 0, s>>>(set, idx, n, (char *)S.spare);
+    NB_TRY(check_launch("k_gather_set"));
+    k_copy_back_set<<<g, 256, 0, s>>>(set, n, (const char *)S.spare);
+    NB_TRY(check_launch("k_copy_back_set"));
     // perm_new[i] = perm_old[idx[i]]  (written to the free key buffer half)
     int64_t *np = (int64_t *)S.keys;
     k_compose_perm<<<g, 256, 0, s>>>(c->perm, idx, n, np);
