@@ -237,6 +237,17 @@ int maybe_sort(nb_ctx *c, int64_t upcoming) {
     return NB_OK;
 }
 
+// The source weights change only with the masses (upload) and their order
+// (a re-sort): computed when a force needs them and they are stale, never
+// inside a captured graph (nb_kdk_steps refreshes them before capturing and
+// before each graph launch).
+static int ensure_source_weights(nb_ctx *c) {
+    if (c->wsrc_valid || c->n <= 0) return NB_OK;
+    NB_TRY(launch_source_weights(c->m, c->is_double, c->n, c->wsrc, c->stream));
+    c->wsrc_valid = true;
+    return NB_OK;
+}
+
 // Work of a KDK step that rides on run_force's kernels: the drift has written
 // the tile boxes already (boxes_ready), and the kick of `kick` (x dt) may be
 // done by the finalize (kicked reports whether it was).
@@ -285,7 +296,7 @@ int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = tru
         a.skip_src = (c->mode == NB_MODE_SPLIT_CENTRAL && c->ncentral > 0) ? c->central : nullptr;
         a.accumulate = 0;
         if (!f64 && (!with_pot || pot_only) && far_path_weighted(c->tun.far_path)) {
-            NB_TRY(launch_source_weights(c->m, true, n, c->wsrc, s));
+            NB_TRY(ensure_source_weights(c));
             a.sw = c->wsrc;
         }
         if (f64) NB_TRY(launch_force_f64(a, with_pot, s));
@@ -310,7 +321,7 @@ int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = tru
         a.accumulate = 0;
         const bool acc_out = !(with_pot && pot_only);
         if ((!with_pot || !acc_out) && c->tun.far_path == FAR_MFMA_BF16) {
-            NB_TRY(launch_source_weights(c->m, false, n, c->wsrc, s));
+            NB_TRY(ensure_source_weights(c));
             a.sw = c->wsrc;
         }
         NB_TRY(launch_force_f32(a, tpt, with_pot, s, acc_out, c->tun.far_path));
@@ -599,6 +610,7 @@ int nb_upload(nb_ctx *c, const double *pos, const double *vel, const double *mas
         NB_HIP(hipStreamSynchronize(s));
         NB_HIP(hipMemcpyAsync(c->stage, mass, sizeof(double) * nn, hipMemcpyHostToDevice, s));
         NB_TRY(scalar_in(c->stage, nullptr, n, c->m, c->is_double, s));
+        c->wsrc_valid = false;
         NB_HIP(hipMemcpyAsync(c->coll, collection, nn, hipMemcpyHostToDevice, s));
         NB_TRY(upload_scalar_h(c));
         NB_HIP(hipStreamSynchronize(s));
@@ -854,6 +866,7 @@ int nb_kdk_steps(nb_ctx *c, int64_t nsteps, double dt, double t_begin, int32_t u
                                   c->tun.j_segments);
             NB_TRY(ensure_part(c, g.nseg));
             NB_TRY(ensure_tbg(c, g));
+            NB_TRY(ensure_source_weights(c));
             NB_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             int rc = NB_OK;
             for (int64_t q = 0; q < chunk && rc == NB_OK; ++q)
@@ -886,6 +899,7 @@ int nb_kdk_steps(nb_ctx *c, int64_t nsteps, double dt, double t_begin, int32_t u
         }
         for (; k + c->g_steps <= nsteps; k += c->g_steps) {
             NB_TRY(maybe_sort(c, c->g_steps));
+            NB_TRY(ensure_source_weights(c));   // after a re-sort the graph's forces need them again
             NB_HIP(hipGraphLaunch(c->gexec, s));
             c->force_calls += c->g_steps;
             c->calls_since_sort += c->g_steps;

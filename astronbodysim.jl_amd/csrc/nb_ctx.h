@@ -79,6 +79,7 @@ struct nb_ctx {
     int part_cap = 0;             // segments the partial buffer holds
     double *stage = nullptr;      // 3n doubles
     float *wsrc = nullptr;        // source weights m^(-2/3) (weighted far paths), n floats
+    bool wsrc_valid = false;      // wsrc holds the weights of the current masses in the current order
     double *red = nullptr;        // energy scratch
     double *time = nullptr;       // two FP64 time slots for device-side drift
     unsigned *flag = nullptr;

@@ -187,6 +187,7 @@ int morton_reorder(nb_ctx *c) {
     NB_HIP(hipMemcpyAsync(c->perm, np, sizeof(int64_t) * (size_t)n, hipMemcpyDeviceToDevice, s));
     NB_TRY(ctx_rebuild_central(c));
     S.valid = true;
+    c->wsrc_valid = false;   // the source weights follow the masses' order
     return NB_OK;
 }
 
