@@ -246,6 +246,30 @@ def test_kdk_step_forces_after_resorts(pkg, orc, far, sort, use_graph):
     assert np.median(e) <= 2e-7 and np.percentile(e, 99) <= 2e-6
 
 
+@pytest.mark.parametrize("mode", ["split", "fp32"])
+def test_source_weights_follow_a_new_upload(pkg, orc, mode):
+    """The matrix-core far path's source weights m^(-2/3) are kept between
+    force calls and recomputed after an upload: new masses on the same
+    context give the new forces."""
+    pos, vel, mass = pkg.ics.collision(8192, seeds=(3, 4))
+    soft = [0.002] * 6
+    rng = np.random.default_rng(5)
+    mass2 = mass * rng.uniform(0.5, 2.0, size=mass.shape)
+    with make_ctx(pkg, pos, vel, mass, soft, mode, tuning=(1, 4, 16, 4)) as ctx:
+        ctx.compute_force()
+        ctx.upload(pos, vel, mass2, None)
+        ctx.compute_force()
+        acc = ctx.download(pos=False, vel=False)["acc"]
+    if mode == "fp32":
+        p32 = pos.astype(np.float32).astype(np.float64)
+        ref = orc.compute_force(p32, mass2.astype(np.float32).astype(np.float64), soft)
+    else:
+        ref = orc.compute_force(pos, mass2, soft)
+    e = orc.rel_err(acc, ref)
+    print(f"{mode}: median {np.median(e):.3e} p99 {np.percentile(e, 99):.3e}")
+    assert np.median(e) <= 2e-7 and np.percentile(e, 99) <= 2e-6
+
+
 def test_split_far_near_passes_repeat_and_match_one_kernel(pkg, orc):
     """The FP64-state bf16 launch with several source segments runs as a far
     pass and a near pass (per-wave flags set by the first, cleared by the
