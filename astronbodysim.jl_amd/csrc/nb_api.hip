@@ -253,6 +253,7 @@ static int ensure_source_weights(nb_ctx *c) {
 // done by the finalize (kicked reports whether it was).
 struct StepFusion {
     bool boxes_ready = false;
+    int geo_ready = 0;     // the drift stored the target geometry (1), with the B operands (2)
     double kick = 0.0;
     bool kicked = false;
 };
@@ -287,6 +288,7 @@ int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = tru
         a.sub_box = c->sub_box;
         a.tgeo = c->tgeo;
         a.tbg = c->tbg;
+        a.tgeo_ready = fuse && !resorted ? fuse->geo_ready : 0;
         a.nflag = c->nflag_cap >= (size_t)g.nblk_i * (size_t)g.nseg * NW ? c->nflag : nullptr;
         a.use_origin = 1;
         a.nseg = g.nseg; a.seg_len = g.seg_len; a.nblk_i = g.nblk_i;
@@ -311,6 +313,7 @@ int run_force(nb_ctx *c, bool with_pot, bool allow_timing, bool allow_sort = tru
         a.sub_box = c->sub_box;
         a.tgeo = c->tgeo;
         a.tbg = c->tbg;
+        a.tgeo_ready = fuse && !resorted ? fuse->geo_ready : 0;
         a.nflag = c->nflag_cap >= (size_t)g.nblk_i * (size_t)g.nseg * NW ? c->nflag : nullptr;
         a.use_origin = 0;
         a.nseg = g.nseg; a.seg_len = g.seg_len; a.nblk_i = g.nblk_i;
@@ -827,8 +830,27 @@ static int enqueue_step(nb_ctx *c, int64_t k, double dt, bool allow_timing, bool
     const int rd = (int)(k & 1), wr = 1 - rd;
     StepFusion fuse;
     if (c->n > 0) {
+        // The target geometry of a force call with one target per lane and
+        // several source segments rides on the drift as well, stored as that
+        // call's launch would store it (origin mode, B operands of far path 4).
+        StepGeometry sg{};
+        const StepGeometry *gp = nullptr;
+        if (c->mode != NB_MODE_FP64 && c->tun.targets_per_thread == 1 && c->tgeo) {
+            Geometry g = geometry(c->n, c->n, 1, c->tun.j_segments);
+            if (g.nseg > 1) {
+                NB_TRY(ensure_part(c, g.nseg));
+                NB_TRY(ensure_tbg(c, g));
+                const bool bf16 = c->tun.far_path == FAR_MFMA_BF16;
+                sg.h = c->h;
+                sg.tgeo = c->tgeo;
+                sg.tbg = bf16 ? c->tbg : nullptr;
+                sg.use_origin = (c->is_double || bf16) ? 1 : 0;
+                gp = &sg;
+                fuse.geo_ready = sg.tbg ? 2 : 1;
+            }
+        }
         NB_TRY(launch_drift_boxes(c->x, c->y, c->z, c->vx, c->vy, c->vz, c->n, dt, c->time, rd, wr, c->is_double,
-                                  c->tile_box, c->sub_box, c->stream));
+                                  c->tile_box, c->sub_box, c->stream, gp));
         fuse.boxes_ready = true;
     } else {
         NB_TRY(drift(c->x, c->y, c->z, c->vx, c->vy, c->vz, c->n, dt, c->time, rd, wr, c->is_double,

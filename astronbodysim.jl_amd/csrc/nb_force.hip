@@ -287,13 +287,19 @@ __global__ __launch_bounds__(BS) void k_tile_boxes(const T *__restrict__ x, cons
 // A step's drift (Pos += Vel * (t_new - t_old), the operations of k_drift,
 // with block 0 publishing t_new = time[rd] + dt to time[wr]) fused with the
 // tile boxes of the drifted positions, which the next force needs.
-template <typename T>
+// GEO: the block's target geometry as k_target_geom<T, 1, ROWS> stores it
+// (the force call that follows has one target per lane and several source
+// segments), from the drifted positions.
+template <typename T, bool GEO, bool ROWS>
 __global__ __launch_bounds__(BS) void k_drift_boxes(T *__restrict__ x, T *__restrict__ y, T *__restrict__ z,
                                                     const T *__restrict__ vx, const T *__restrict__ vy,
                                                     const T *__restrict__ vz, int64_t n, double dt,
                                                     double *time, int rd, int wr, double *__restrict__ box,
-                                                    double *__restrict__ sub) {
+                                                    double *__restrict__ sub, const double *__restrict__ h,
+                                                    double *__restrict__ tgeo, unsigned *__restrict__ tbg,
+                                                    int use_origin, bool hilo) {
     __shared__ double red[6][NW];
+    __shared__ double gred[GEO ? 11 : 1][NW];
     const int64_t j = (int64_t)blockIdx.x * BS + threadIdx.x;
     const double t_old = time[rd];
     const double t_new = t_old + dt;
@@ -309,6 +315,24 @@ __global__ __launch_bounds__(BS) void k_drift_boxes(T *__restrict__ x, T *__rest
         x[j] = px; y[j] = py; z[j] = pz;
     }
     tile_box_block((double)px, (double)py, (double)pz, v, box, sub, red);
+    if constexpr (GEO) {
+        const int tid = threadIdx.x;
+        double gx[1] = {v ? (double)px : 0.0}, gy[1] = {v ? (double)py : 0.0}, gz[1] = {v ? (double)pz : 0.0};
+        double gh[1] = {v ? h[j] : 0.0};
+        bool gv[1] = {v};
+        BlockStats st;
+        block_stats<1>(gx, gy, gz, gh, gv, use_origin != 0, gred, st);
+        const double ox = uniform_f64(st.ox), oy = uniform_f64(st.oy), oz = uniform_f64(st.oz);
+        const WaveGeom wg = wave_geometry<1>(gx, gy, gz, gh, gv, ox, oy, oz, hilo);
+        store_target_geom(tgeo + (int64_t)blockIdx.x * TG_STRIDE, ox, oy, oz, wg, tid >> 6, tid & 63);
+        if constexpr (ROWS) {
+            nb_bf8 Bg[4];
+            target_b_operands(gx[0] - ox, gy[0] - oy, gz[0] - oz, tid & 63, Bg);
+            nb_u4v *b = reinterpret_cast<nb_u4v *>(tbg + ((int64_t)blockIdx.x * BS + tid) * 16);
+#pragma unroll
+            for (int g = 0; g < 4; ++g) b[g] = __builtin_bit_cast(nb_u4v, Bg[g]);
+        }
+    }
 }
 
 // k_finalize followed by the step's kick (Vel += Acc * coef, the operations
@@ -370,6 +394,12 @@ static int prepare_target_geom(ForceArgs<T> &a, int tpt, hipStream_t s, bool row
     if (!a.tgeo || a.nseg <= 1 || a.nblk_i <= 0) {
         a.tgeo = nullptr;
         a.tbg = nullptr;
+        return NB_OK;
+    }
+    // already written by the step's drift (one target per lane), with the B
+    // operands when this launch takes them
+    if (tpt == 1 && a.tgeo_ready && (a.tgeo_ready == 2 || !rows || !a.tbg)) {
+        if (!rows || !a.tbg) a.tbg = nullptr;
         return NB_OK;
     }
     const bool hilo = sizeof(T) == 8;
@@ -487,17 +517,28 @@ int launch_force_f64(const ForceArgs<double> &a, bool pot, hipStream_t s) {
 
 int launch_drift_boxes(void *x, void *y, void *z, const void *vx, const void *vy, const void *vz, int64_t n,
                        double dt, double *time, int rd, int wr, int is_double, double *box, double *sub_box,
-                       hipStream_t s) {
+                       hipStream_t s, const StepGeometry *geo) {
     if (n <= 0) return NB_OK;
-    const int64_t ntiles = (n + BS - 1) / BS;
-    if (is_double)
-        k_drift_boxes<double><<<(unsigned)ntiles, BS, 0, s>>>(
-            (double *)x, (double *)y, (double *)z, (const double *)vx, (const double *)vy, (const double *)vz, n,
-            dt, time, rd, wr, box, sub_box);
-    else
-        k_drift_boxes<float><<<(unsigned)ntiles, BS, 0, s>>>(
-            (float *)x, (float *)y, (float *)z, (const float *)vx, (const float *)vy, (const float *)vz, n, dt,
-            time, rd, wr, box, sub_box);
+    const unsigned nt = (unsigned)((n + BS - 1) / BS);
+    const bool g = geo && geo->tgeo, r = g && geo->tbg;
+    const double *h = g ? geo->h : nullptr;
+    double *tg = g ? geo->tgeo : nullptr;
+    unsigned *tb = r ? geo->tbg : nullptr;
+    const int uo = g ? geo->use_origin : 0;
+#define NB_DB(T_, G_, R_)                                                                                       \
+    k_drift_boxes<T_, G_, R_><<<nt, BS, 0, s>>>((T_ *)x, (T_ *)y, (T_ *)z, (const T_ *)vx, (const T_ *)vy,     \
+                                                (const T_ *)vz, n, dt, time, rd, wr, box, sub_box, h, tg, tb, uo, \
+                                                sizeof(T_) == 8)
+    if (is_double) {
+        if (r) NB_DB(double, true, true);
+        else if (g) NB_DB(double, true, false);
+        else NB_DB(double, false, false);
+    } else {
+        if (r) NB_DB(float, true, true);
+        else if (g) NB_DB(float, true, false);
+        else NB_DB(float, false, false);
+    }
+#undef NB_DB
     return check_launch("k_drift_boxes");
 }
 

@@ -52,6 +52,7 @@ struct ForceArgs {
     const double *tile_box;  // [ntiles][8]: lo xyz, hi xyz, (unused) of each BS-source tile
     const double *sub_box;   // [ntiles][NQ][8]: the same for each tile's quarters, or nullptr
     double *tgeo;            // [nblk_i][TG_STRIDE] workspace for the target blocks' geometry, or nullptr
+    int tgeo_ready;          // 1: tgeo holds this call's geometry (written by the step's drift), 2: tbg too
     unsigned *tbg;           // [nblk_i][BS][16] workspace for their bf16 B operands, or nullptr
     uint8_t *nflag;          // [nblk_i * nseg * NW]: waves with near tiles (split passes), or nullptr
     int use_origin;          // 1: per-block FP64 origin (SPLIT), 0: absolute FP32 coordinates
@@ -98,9 +99,18 @@ int launch_finalize(const double *part, int nseg, int64_t n, double G, double *a
 // The KDK step's drift fused with the tile boxes of the drifted positions, and
 // its kick fused with the finalize of the accelerations (same arithmetic as
 // the separate kernels).
+// What the drift may also store for the force call that follows: the target
+// blocks' geometry (tgeo non-null; one target per lane, several segments),
+// with the bf16 B operands when tbg is non-null.
+struct StepGeometry {
+    const double *h;     // target softening lengths
+    double *tgeo;
+    unsigned *tbg;
+    int use_origin;
+};
 int launch_drift_boxes(void *x, void *y, void *z, const void *vx, const void *vy, const void *vz, int64_t n,
                        double dt, double *time, int rd, int wr, int is_double, double *box, double *sub_box,
-                       hipStream_t s);
+                       hipStream_t s, const StepGeometry *geo = nullptr);
 int launch_finalize_kick(const double *part, int nseg, int64_t n, double G, double *ax, double *ay, double *az,
                          void *vx, void *vy, void *vz, double coef, int is_double, hipStream_t s);
 
