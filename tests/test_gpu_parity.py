@@ -225,24 +225,31 @@ def test_graph_replay_matches_eager(pkg, far):
         assert np.array_equal(out[0]["vel"], out[k]["vel"])
 
 
-@pytest.mark.parametrize("far,sort,use_graph", [(4, 1, False), (4, 2, True), (0, 1, False)])
-def test_kdk_step_forces_after_resorts(pkg, orc, far, sort, use_graph):
+@pytest.mark.parametrize("mode,far,sort,use_graph", [
+    ("split", 4, 1, False), ("split", 4, 2, True), ("split", 0, 1, False), ("split", 0, 2, True),
+    ("fp32", 4, 1, False), ("fp32", 4, 2, True), ("fp32", 0, 2, True)])
+def test_kdk_step_forces_after_resorts(pkg, orc, mode, far, sort, use_graph):
     """Inside nb_kdk_steps the drift writes the tile boxes of the drifted
     positions and the kick rides on the finalize; a step that re-sorts
     first recomputes the boxes.  With a re-sort every step (eager) or between
     graph launches, the accelerations left by the last step must be the
     forces at the final positions: against the oracle on the downloaded
-    state."""
+    state.  The drift also stores the target geometry (and, on far path 4,
+    the B operands) of the force that follows, in that launch's origin mode
+    (per-block origin in split mode and on the FP32 state's far path 4,
+    absolute coordinates on its far path 0)."""
     pos, vel, mass = pkg.ics.collision(8192, seeds=(3, 4))
     soft, dt = [0.002] * 6, 1e-3
-    with make_ctx(pkg, pos, vel, mass, soft, "split", tuning=(1, 4, sort, far)) as ctx:
+    with make_ctx(pkg, pos, vel, mass, soft, mode, tuning=(1, 4, sort, far)) as ctx:
         ctx.compute_force()
         ctx.half_kick(+1, dt)
         ctx.kdk_steps(8, dt, 0.0, use_graph=use_graph)
         d = ctx.download(pos=True, vel=False, acc=True)
-    ref = orc.compute_force(d["pos"], mass, soft)
+    m = mass.astype(np.float32).astype(np.float64) if mode == "fp32" else mass
+    ref = orc.compute_force(d["pos"], m, soft)
     e = orc.rel_err(d["acc"], ref)
-    print(f"far {far} sort {sort} graph {use_graph}: median {np.median(e):.3e} p99 {np.percentile(e, 99):.3e}")
+    print(f"{mode} far {far} sort {sort} graph {use_graph}: median {np.median(e):.3e} "
+          f"p99 {np.percentile(e, 99):.3e}")
     assert np.median(e) <= 2e-7 and np.percentile(e, 99) <= 2e-6
 
 
